@@ -1,0 +1,576 @@
+// ORACLE HARNESS (test infrastructure only). Built by oracle/Makefile against
+// the reference's own sources (src/softwing, src/fsi) into oracle/_ref/; it is
+// never shipped, and it never runs on the GPU host.
+//
+// It drives the reference's CPU XPBD path on the build's seeded synthetic
+// wings and writes golden fixtures:
+//   1. swx::generateWing(preset) -> simwing::fsi::Scene (field for field);
+//   2. simwing::fsi::assembleSceneStructure (scene_structure.cpp:894-1282)
+//      -> the definition tables, written as-is ("assembly" fixture);
+//   3. softwing::SoftBody + SuspensionSystem built with the public softwing
+//      API exactly as Structure's buildBody/buildSuspension do
+//      (structure.cpp:460-639), except that dihedral hinges are inserted in
+//      the build's colour-major order (oracle::dihedralColouring), so the
+//      reference's serial hinge sweep (soft_body.cpp:1353-1365) is the
+//      coloured sweep; per-triangle pressure is set with
+//      setFacePressureDifference (soft_body.h:401-402);
+//   4. each step replays mass-weighted external loads the way
+//      SurfaceFlightCase does (surface_flight_case.cpp:187-214) and calls
+//      SoftBody::stepCoupled (structure.cpp:807-831 without the
+//      checkpoint), with the surface-flight solver settings
+//      (surface_flight_case.cpp:94-114): dt 1/120, 1 substep, 48
+//      iterations, 3 cable pairs, damping 8/s, gravity -9.80665;
+//   5. the full solver state is written at the requested steps.
+//
+// Usage: ref_driver <preset> <out-prefix> <steps> <workers> [snapshot ...]
+//        ref_driver --compare-workers <preset> <steps>
+#include "fixture_io.hpp"
+#include "oracle_colouring.hpp"
+
+#include "../simwing_amd/csrc/scene.hpp"
+
+#include "scene_structure.h"
+#include "structure.h"
+#include <softwing/soft_body.h>
+#include <softwing/suspension.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <set>
+#include <string>
+#include <vector>
+
+namespace fsi = simwing::fsi;
+namespace sw = softwing;
+
+namespace {
+
+fsi::Scene toReferenceScene(const swx::Scene& in) {
+    fsi::Scene s;
+    s.metadata.designChecksum = "synthetic-wing";
+    s.metadata.exporterVersion = "simwing-amd-generator-1";
+    for (const auto& r : in.regions)
+        s.regions.push_back({r.id, static_cast<fsi::RegionKind>(r.kind), r.name});
+    for (const auto& v : in.vertices)
+        s.vertices.push_back({v.id, {v.position.x, v.position.y, v.position.z}});
+    for (const auto& m : in.fabricMaterials) {
+        fsi::FabricMaterial f;
+        f.id = m.id;
+        f.name = m.name;
+        f.warpStiffnessNewtonsPerMeter = m.warpStiffness;
+        f.weftStiffnessNewtonsPerMeter = m.weftStiffness;
+        f.shearStiffnessNewtonsPerMeter = m.shearStiffness;
+        f.bendingStiffnessNewtonMeters = m.bendingStiffness;
+        f.arealDensityKgPerSquareMeter = m.arealDensity;
+        f.dampingSeconds = m.dampingSeconds;
+        f.porosityFraction = m.porosity;
+        f.permeabilitySquareMeters = m.permeability;
+        s.fabricMaterials.push_back(f);
+    }
+    for (const auto& m : in.seamMaterials)
+        s.seamMaterials.push_back({m.id, m.name, m.linearDensity, m.axialStiffness});
+    for (const auto& t : in.triangles) {
+        fsi::Triangle r;
+        r.id = t.id;
+        r.vertexIds = t.vertexIds;
+        for (int k = 0; k < 3; ++k) r.materialCoordinates[k] = {t.chart[k].x, t.chart[k].y};
+        r.negativeSideRegionId = t.negativeRegion;
+        r.positiveSideRegionId = t.positiveRegion;
+        r.materialId = t.materialId;
+        r.sheetId = t.sheetId;
+        r.role = static_cast<fsi::SurfaceRole>(t.role);
+        s.triangles.push_back(r);
+    }
+    for (const auto& m : in.seams) s.seams.push_back({m.id, m.materialId, m.first, m.second});
+    for (const auto& m : in.lineMaterials)
+        s.lineMaterials.push_back({m.id, m.name, m.diameter, m.linearDensity, m.axialStiffness, m.drag});
+    for (const auto& p : in.pilots) {
+        fsi::Pilot r;
+        r.id = p.id;
+        r.name = p.name;
+        r.massKg = p.mass;
+        r.centerOfMassPositionMeters = {p.centerOfMass.x, p.centerOfMass.y, p.centerOfMass.z};
+        r.linearVelocityMetersPerSecond = {p.linearVelocity.x, p.linearVelocity.y, p.linearVelocity.z};
+        r.bodyToWorld = {p.bodyToWorld.w, p.bodyToWorld.x, p.bodyToWorld.y, p.bodyToWorld.z};
+        r.principalInertiaKgSquareMeters = {p.principalInertia.x, p.principalInertia.y,
+                                            p.principalInertia.z};
+        s.pilots.push_back(r);
+    }
+    for (const auto& j : in.junctions)
+        s.suspensionJunctions.push_back({j.id, {j.position.x, j.position.y, j.position.z}, j.mass, j.fixed});
+    for (const auto& a : in.attachments) {
+        fsi::Attachment r;
+        r.id = a.id;
+        r.kind = static_cast<fsi::AttachmentKind>(a.kind);
+        r.vertexId = a.vertexId;
+        r.pilotId = a.pilotId;
+        r.pilotLocalPositionMeters = {a.pilotLocal.x, a.pilotLocal.y, a.pilotLocal.z};
+        r.suspensionJunctionId = a.junctionId;
+        s.attachments.push_back(r);
+    }
+    for (const auto& l : in.lines) {
+        fsi::SuspensionLine r;
+        r.id = l.id;
+        r.startAttachmentId = l.startAttachment;
+        r.endAttachmentId = l.endAttachment;
+        r.materialId = l.materialId;
+        r.restLengthMeters = l.restLength;
+        r.role = static_cast<fsi::SuspensionLineRole>(l.role);
+        s.suspensionLines.push_back(r);
+    }
+    return s;
+}
+
+fsi::SceneStructureSettings flightSettings() {
+    fsi::SceneStructureSettings settings;
+    settings.suspensionSolverIterations = 48;
+    settings.suspensionMaximumLineResidualMeters = 0.10;
+    return settings;
+}
+
+void writeAssembly(swfx::Writer& w, const fsi::SceneStructureAssembly& a,
+                   const std::vector<std::size_t>& hingeOrder) {
+    const fsi::StructureDefinition& d = a.definition;
+    std::vector<double> pos, mass;
+    std::vector<std::uint8_t> fixed;
+    for (const auto& n : d.nodes) {
+        pos.insert(pos.end(), {n.positionMeters.x, n.positionMeters.y, n.positionMeters.z});
+        mass.push_back(n.massKg);
+        fixed.push_back(n.fixed ? 1 : 0);
+    }
+    w.f64("node_position", pos);
+    w.f64("node_mass", mass);
+    w.u8("node_fixed", fixed);
+    std::vector<std::uint64_t> tri;
+    for (const auto& t : d.triangles) tri.insert(tri.end(), t.nodes.begin(), t.nodes.end());
+    w.u64("triangle_nodes", tri);
+    std::vector<std::uint8_t> kind;
+    std::vector<std::uint64_t> cnodes;
+    std::vector<double> rest, comp;
+    for (const auto& c : d.constraints) {
+        kind.push_back(static_cast<std::uint8_t>(c.kind));
+        cnodes.push_back(c.firstNode);
+        cnodes.push_back(c.secondNode);
+        rest.push_back(c.restLengthMeters);
+        comp.push_back(c.complianceMetersPerNewton);
+    }
+    w.u8("constraint_kind", kind);
+    w.u64("constraint_nodes", cnodes);
+    w.f64("constraint_rest", rest);
+    w.f64("constraint_compliance", comp);
+    std::vector<std::uint64_t> mtri;
+    std::vector<double> chart, material;
+    for (const auto& m : d.membranes) {
+        mtri.push_back(m.triangle);
+        for (const auto& c : m.materialCoordinates) chart.insert(chart.end(), {c.x, c.y});
+        const auto& q = m.material;
+        material.insert(material.end(),
+                        {q.warpStiffnessNewtonsPerMeter, q.weftStiffnessNewtonsPerMeter,
+                         q.couplingStiffnessNewtonsPerMeter, q.shearStiffnessNewtonsPerMeter,
+                         q.warpPreTensionNewtonsPerMeter, q.weftPreTensionNewtonsPerMeter,
+                         q.dampingSeconds, q.compressionStiffnessRatio});
+    }
+    w.u64("membrane_triangle", mtri);
+    w.f64("membrane_chart", chart);
+    w.f64("membrane_material", material);
+    std::vector<std::uint64_t> hnodes;
+    std::vector<double> hrest, hcomp;
+    for (const auto& h : d.dihedrals) {
+        hnodes.insert(hnodes.end(), h.nodes.begin(), h.nodes.end());
+        hrest.push_back(h.restAngleRadians);
+        hcomp.push_back(h.complianceRadiansPerNewtonMeter);
+    }
+    w.u64("dihedral_nodes", hnodes);
+    w.f64("dihedral_rest_angle", hrest);
+    w.f64("dihedral_compliance", hcomp);
+    w.u64("dihedral_colour_order", std::vector<std::uint64_t>(hingeOrder.begin(), hingeOrder.end()));
+    if (d.suspension) {
+        const auto& s = *d.suspension;
+        w.f64("pilot", {s.pilotMassKg, s.pilotInitialCenterOfMassWorldMeters.x,
+                        s.pilotInitialCenterOfMassWorldMeters.y, s.pilotInitialCenterOfMassWorldMeters.z,
+                        s.pilotPrincipalInertiaKgSquareMeters.x, s.pilotPrincipalInertiaKgSquareMeters.y,
+                        s.pilotPrincipalInertiaKgSquareMeters.z, s.pilotInitialBodyToWorld.w,
+                        s.pilotInitialBodyToWorld.x, s.pilotInitialBodyToWorld.y,
+                        s.pilotInitialBodyToWorld.z});
+        std::vector<std::uint64_t> att, jun, hid;
+        std::vector<double> hloc;
+        for (const auto& a : s.attachments) att.insert(att.end(), {a.stableId, a.node});
+        for (const auto& j : s.junctions) jun.insert(jun.end(), {j.stableId, j.node});
+        for (const auto& h : s.harnessPoints) {
+            hid.push_back(h.stableId);
+            hloc.insert(hloc.end(), {h.localPositionMeters.x, h.localPositionMeters.y,
+                                     h.localPositionMeters.z});
+        }
+        w.u64("suspension_attachments", att);
+        w.u64("suspension_junctions", jun);
+        w.u64("harness_ids", hid);
+        w.f64("harness_local", hloc);
+        std::vector<std::uint64_t> seg;
+        std::vector<double> segval;
+        for (const auto& g : s.segments) {
+            seg.insert(seg.end(), {g.stableId, static_cast<std::uint64_t>(g.from.kind), g.from.stableId,
+                                   static_cast<std::uint64_t>(g.to.kind), g.to.stableId});
+            segval.insert(segval.end(), {g.restLengthMeters, g.axialStiffnessNewtons,
+                                         g.axialDampingNewtonSecondsPerMeter});
+        }
+        w.u64("segments", seg);
+        w.f64("segment_values", segval);
+    }
+    std::vector<std::uint64_t> ranges;
+    for (const auto& r : a.mappings.constraintSeamRanges)
+        ranges.insert(ranges.end(), {r.seamId, r.firstConstraint, r.constraintCount});
+    w.u64("seam_ranges", ranges);
+    w.u64("node_vertex_ids", a.mappings.nodeVertexIds);
+    w.u64("node_junction_ids", a.mappings.nodeSuspensionJunctionIds);
+    w.f64("total_masses", {a.totalFabricMassKg, a.totalSeamMassKg});
+}
+
+struct Rig {
+    std::unique_ptr<sw::SoftBody> body;
+    std::unique_ptr<sw::SuspensionSystem> suspension;
+    std::vector<sw::Vec3> loads;
+};
+
+// Structure's buildBody (structure.cpp:554-639) with hinges in `hingeOrder`.
+Rig buildRig(const fsi::StructureDefinition& d, const std::vector<std::size_t>& hingeOrder,
+             const std::vector<double>& facePressure) {
+    Rig rig;
+    rig.body = std::make_unique<sw::SoftBody>();
+    sw::SoftBody& body = *rig.body;
+    for (const auto& n : d.nodes) {
+        const sw::Vec3 p{n.positionMeters.x, n.positionMeters.y, n.positionMeters.z};
+        if (n.fixed) body.addFixedNode(p);
+        else body.addNode(p, n.massKg);
+    }
+    for (const auto& t : d.triangles) body.addTriangle(t.nodes[0], t.nodes[1], t.nodes[2]);
+    for (const auto& c : d.constraints) {
+        switch (c.kind) {
+        case fsi::StructureConstraintKind::Distance:
+            body.addDistanceConstraint(c.firstNode, c.secondNode, c.restLengthMeters,
+                                       c.complianceMetersPerNewton);
+            break;
+        case fsi::StructureConstraintKind::Cable:
+            body.addCableConstraint(c.firstNode, c.secondNode, c.restLengthMeters,
+                                    c.complianceMetersPerNewton);
+            break;
+        case fsi::StructureConstraintKind::SuspensionTie:
+            body.addSuspensionTieConstraint(c.firstNode, c.secondNode, c.restLengthMeters,
+                                            c.complianceMetersPerNewton);
+            break;
+        case fsi::StructureConstraintKind::SeamStitch:
+            body.addSeamStitchConstraint(c.firstNode, c.secondNode);
+            break;
+        }
+    }
+    std::vector<sw::MembraneElementDefinition> membranes;
+    for (const auto& m : d.membranes) {
+        sw::MembraneElementDefinition e;
+        e.triangle = m.triangle;
+        for (int k = 0; k < 3; ++k) e.chart[k] = {m.materialCoordinates[k].x, m.materialCoordinates[k].y};
+        e.material.warpStiffness = m.material.warpStiffnessNewtonsPerMeter;
+        e.material.weftStiffness = m.material.weftStiffnessNewtonsPerMeter;
+        e.material.couplingStiffness = m.material.couplingStiffnessNewtonsPerMeter;
+        e.material.shearStiffness = m.material.shearStiffnessNewtonsPerMeter;
+        e.material.warpPreTension = m.material.warpPreTensionNewtonsPerMeter;
+        e.material.weftPreTension = m.material.weftPreTensionNewtonsPerMeter;
+        e.material.dampingTime = m.material.dampingSeconds;
+        e.material.compressionStiffnessRatio = m.material.compressionStiffnessRatio;
+        e.role = sw::MaterialRole::Bulk;
+        membranes.push_back(e);
+    }
+    if (!membranes.empty()) static_cast<void>(body.addMembraneElements(membranes));
+    for (const std::size_t index : hingeOrder) {
+        const auto& h = d.dihedrals[index];
+        body.addDihedralBendingConstraint(h.nodes[0], h.nodes[1], h.nodes[2], h.nodes[3],
+                                          h.restAngleRadians, h.complianceRadiansPerNewtonMeter);
+    }
+    for (std::size_t t = 0; t < facePressure.size(); ++t) body.setFacePressureDifference(t, facePressure[t]);
+
+    if (d.suspension) {
+        // Structure's buildSuspension (structure.cpp:460-552).
+        const fsi::StructureSuspensionDefinition& source = *d.suspension;
+        const char* provenanceId = "simwing-structure-resolved";
+        const auto idOf = [](fsi::StructureSuspensionEndpointKind kind, std::uint64_t id) {
+            switch (kind) {
+            case fsi::StructureSuspensionEndpointKind::SurfaceAttachment:
+                return "attachment:" + std::to_string(id);
+            case fsi::StructureSuspensionEndpointKind::Junction:
+                return "junction:" + std::to_string(id);
+            case fsi::StructureSuspensionEndpointKind::PilotHarness:
+                return "harness:" + std::to_string(id);
+            }
+            return std::string();
+        };
+        const auto kindOf = [](fsi::StructureSuspensionEndpointKind kind) {
+            switch (kind) {
+            case fsi::StructureSuspensionEndpointKind::SurfaceAttachment:
+                return sw::SuspensionEndpointKind::Attachment;
+            case fsi::StructureSuspensionEndpointKind::Junction:
+                return sw::SuspensionEndpointKind::Junction;
+            default:
+                return sw::SuspensionEndpointKind::HangPoint;
+            }
+        };
+        sw::SuspensionDefinition def;
+        def.identifier = "simwing-structure";
+        def.description = "Resolved scene-v2 suspension and rigid pilot";
+        def.unitsFrameTag = std::string(sw::suspensionStage5FrameTag);
+        def.provenance = {{provenanceId, "authoritative scene-v2 stable topology"}};
+        std::vector<sw::ResolvedSuspensionAttachment> resolved;
+        for (const auto& a : source.attachments) {
+            const std::string id = idOf(fsi::StructureSuspensionEndpointKind::SurfaceAttachment, a.stableId);
+            const std::string panel = "resolved-node:" + std::to_string(a.node);
+            def.attachments.push_back({id, panel, {}, sw::SuspensionSide::Centre, {}, provenanceId});
+            resolved.push_back({id, panel, {}, a.node, body.nodes()[a.node].position, provenanceId});
+        }
+        std::vector<std::pair<std::string, std::size_t>> junctionNodes;
+        for (const auto& j : source.junctions) {
+            const std::string id = idOf(fsi::StructureSuspensionEndpointKind::Junction, j.stableId);
+            const sw::Node& node = body.nodes()[j.node];
+            def.junctions.push_back({id, node.position, 1.0 / node.inverseMass,
+                                     sw::SuspensionSide::Centre, provenanceId});
+            junctionNodes.emplace_back(id, j.node);
+        }
+        def.payload.mass = source.pilotMassKg;
+        def.payload.centreOfMassLocal = {source.pilotCenterOfMassLocalMeters.x,
+                                         source.pilotCenterOfMassLocalMeters.y,
+                                         source.pilotCenterOfMassLocalMeters.z};
+        def.payload.inertiaBody = {{source.pilotPrincipalInertiaKgSquareMeters.x, 0.0, 0.0, 0.0,
+                                    source.pilotPrincipalInertiaKgSquareMeters.y, 0.0, 0.0, 0.0,
+                                    source.pilotPrincipalInertiaKgSquareMeters.z}};
+        def.payload.initialState = {
+            {source.pilotInitialCenterOfMassWorldMeters.x, source.pilotInitialCenterOfMassWorldMeters.y,
+             source.pilotInitialCenterOfMassWorldMeters.z},
+            {source.pilotInitialBodyToWorld.w, source.pilotInitialBodyToWorld.x,
+             source.pilotInitialBodyToWorld.y, source.pilotInitialBodyToWorld.z},
+            {source.pilotInitialLinearVelocityMetersPerSecond.x,
+             source.pilotInitialLinearVelocityMetersPerSecond.y,
+             source.pilotInitialLinearVelocityMetersPerSecond.z},
+            {source.pilotInitialAngularVelocityRadiansPerSecond.x,
+             source.pilotInitialAngularVelocityRadiansPerSecond.y,
+             source.pilotInitialAngularVelocityRadiansPerSecond.z}};
+        def.payload.provenanceId = provenanceId;
+        for (const auto& h : source.harnessPoints) {
+            def.payload.hangPoints.push_back(
+                {idOf(fsi::StructureSuspensionEndpointKind::PilotHarness, h.stableId),
+                 {h.localPositionMeters.x, h.localPositionMeters.y, h.localPositionMeters.z},
+                 sw::SuspensionSide::Centre,
+                 provenanceId});
+        }
+        for (const auto& g : source.segments) {
+            def.segments.push_back({"segment:" + std::to_string(g.stableId),
+                                    {kindOf(g.from.kind), idOf(g.from.kind, g.from.stableId)},
+                                    {kindOf(g.to.kind), idOf(g.to.kind, g.to.stableId)},
+                                    g.restLengthMeters,
+                                    g.axialStiffnessNewtons,
+                                    g.axialDampingNewtonSecondsPerMeter,
+                                    sw::SuspensionSide::Centre,
+                                    {"role:" + std::to_string(g.role)},
+                                    provenanceId});
+        }
+        def.solver.lineIterations = source.solverIterations;
+        def.solver.attachmentTolerance = source.attachmentTolerance;
+        def.solver.minimumLineLength = source.minimumLineLengthMeters;
+        def.solver.maximumLineResidual = source.maximumLineResidualMeters;
+        def.solver.maximumControlWork = source.maximumControlWorkJoules;
+        def.ground.mode = sw::PayloadGroundMode::Free;
+        rig.suspension = std::make_unique<sw::SuspensionSystem>(
+            sw::SuspensionSystem::buildResolved(body, def, resolved, junctionNodes));
+    }
+
+    // Steady flight loads (swx::flightLoads): the system weight carried by
+    // the fabric nodes, mass-weighted, radial from the pilot.
+    double total = 0.0;
+    const std::size_t vertexNodes =
+        d.nodes.size() - (d.suspension ? d.suspension->junctions.size() : 0);
+    std::vector<double> pos, mass;
+    for (const auto& n : d.nodes) {
+        total += n.massKg;
+        pos.insert(pos.end(), {n.positionMeters.x, n.positionMeters.y, n.positionMeters.z});
+        mass.push_back(n.massKg);
+    }
+    swx::V3 centre{};
+    if (d.suspension) {
+        total += d.suspension->pilotMassKg;
+        centre = {d.suspension->pilotInitialCenterOfMassWorldMeters.x,
+                  d.suspension->pilotInitialCenterOfMassWorldMeters.y,
+                  d.suspension->pilotInitialCenterOfMassWorldMeters.z};
+    }
+    std::vector<double> loads(3 * d.nodes.size());
+    swx::flightLoads(pos.data(), mass.data(), d.nodes.size(), vertexNodes, total * 9.80665, centre,
+                     loads.data());
+    rig.loads.assign(d.nodes.size(), sw::Vec3{});
+    for (std::size_t i = 0; i < d.nodes.size(); ++i)
+        rig.loads[i] = sw::Vec3{loads[3 * i], loads[3 * i + 1], loads[3 * i + 2]};
+    return rig;
+}
+
+sw::StepSettings stepSettings(unsigned workers) {
+    sw::StepSettings s;
+    s.timeStep = 1.0 / 120.0;
+    s.substeps = 1;
+    s.constraintIterations = 48;
+    s.cableConstraintSweepPairs = 3;
+    s.gravity = {0.0, 0.0, -9.80665};
+    s.velocityDampingPerSecond = 8.0;
+    s.workerThreads = workers;
+    return s;
+}
+
+void stepRig(Rig& rig, const sw::StepSettings& settings) {
+    rig.body->clearExternalForces();
+    for (std::size_t i = 0; i < rig.loads.size(); ++i) rig.body->addForce(i, rig.loads[i]);
+    if (rig.suspension) rig.body->stepCoupled(settings, *rig.suspension);
+    else rig.body->step(settings);
+}
+
+void writeState(swfx::Writer& w, const std::string& tag, const Rig& rig) {
+    const sw::SoftBody& b = *rig.body;
+    std::vector<double> x, prev, v;
+    for (const auto& n : b.nodes()) {
+        x.insert(x.end(), {n.position.x, n.position.y, n.position.z});
+        prev.insert(prev.end(), {n.previousPosition.x, n.previousPosition.y, n.previousPosition.z});
+        v.insert(v.end(), {n.velocity.x, n.velocity.y, n.velocity.z});
+    }
+    w.f64(tag + "/position", x);
+    w.f64(tag + "/previous", prev);
+    w.f64(tag + "/velocity", v);
+    std::vector<double> cl;
+    for (const auto& c : b.constraints()) cl.push_back(c.accumulatedLambda);
+    w.f64(tag + "/constraint_lambda", cl);
+    std::vector<double> ml, mres, mrs;
+    for (const auto& e : b.membraneElements()) {
+        ml.insert(ml.end(), {e.multiplier.x, e.multiplier.y, e.multiplier.z});
+        mrs.insert(mrs.end(), {e.solverResultantEstimate.x, e.solverResultantEstimate.y,
+                               e.solverResultantEstimate.z});
+        mres.push_back(e.normalizedResidual);
+    }
+    w.f64(tag + "/membrane_lambda", ml);
+    w.f64(tag + "/membrane_resultant", mrs);
+    w.f64(tag + "/membrane_residual", mres);
+    std::vector<double> hl;
+    for (const auto& h : b.dihedralConstraints()) hl.push_back(h.accumulatedLambda);
+    w.f64(tag + "/dihedral_lambda", hl);  // colour-major (insertion) order
+    if (rig.suspension) {
+        const auto& p = rig.suspension->payloadState();
+        w.f64(tag + "/payload", {p.centreOfMassWorld.x, p.centreOfMassWorld.y, p.centreOfMassWorld.z,
+                                 p.orientation.w, p.orientation.x, p.orientation.y, p.orientation.z,
+                                 p.linearVelocity.x, p.linearVelocity.y, p.linearVelocity.z,
+                                 p.angularVelocity.x, p.angularVelocity.y, p.angularVelocity.z});
+        std::vector<double> sl;
+        for (const auto& g : rig.suspension->segments()) sl.push_back(g.accumulatedLambda);
+        w.f64(tag + "/segment_lambda", sl);  // string-id order
+        w.f64(tag + "/suspension_residual", {rig.suspension->diagnostics().maximumResidual});
+    }
+}
+
+std::uint64_t stateHash(const Rig& rig) {
+    std::uint64_t h = 14695981039346656037ULL;
+    for (const auto& n : rig.body->nodes()) {
+        h = swfx::fnv1a(&n.position, sizeof(n.position), h);
+        h = swfx::fnv1a(&n.velocity, sizeof(n.velocity), h);
+    }
+    for (const auto& e : rig.body->membraneElements()) h = swfx::fnv1a(&e.multiplier, sizeof(e.multiplier), h);
+    for (const auto& c : rig.body->dihedralConstraints())
+        h = swfx::fnv1a(&c.accumulatedLambda, sizeof(double), h);
+    if (rig.suspension) {
+        const auto& p = rig.suspension->payloadState();
+        h = swfx::fnv1a(&p, sizeof(p), h);
+    }
+    return h;
+}
+
+struct Prepared {
+    swx::GeneratedWing wing;
+    fsi::SceneStructureAssembly assembly;
+    oracle::Colouring hinges;
+};
+
+Prepared prepare(const std::string& preset) {
+    Prepared p;
+    p.wing = swx::generateWing(swx::wingPreset(preset));
+    p.assembly = fsi::assembleSceneStructure(toReferenceScene(p.wing.scene), {}, flightSettings());
+    if (!p.assembly.ok()) {
+        for (const auto& d : p.assembly.diagnostics)
+            std::fprintf(stderr, "assembly diagnostic %d entity %llu: %s\n", static_cast<int>(d.code),
+                         static_cast<unsigned long long>(d.entityId), d.message.c_str());
+        throw std::runtime_error("reference assembly rejected the synthetic scene");
+    }
+    std::vector<std::size_t> hinges;
+    for (const auto& h : p.assembly.definition.dihedrals) hinges.insert(hinges.end(), h.nodes.begin(), h.nodes.end());
+    p.hinges = oracle::dihedralColouring(hinges, p.assembly.definition.nodes.size());
+    return p;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    try {
+        if (argc >= 4 && std::strcmp(argv[1], "--compare-workers") == 0) {
+            const std::string preset = argv[2];
+            const int steps = std::atoi(argv[3]);
+            Prepared p = prepare(preset);
+            Rig a = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure);
+            Rig b = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure);
+            for (int s = 0; s < steps; ++s) {
+                stepRig(a, stepSettings(1));
+                stepRig(b, stepSettings(12));
+            }
+            const std::uint64_t ha = stateHash(a), hb = stateHash(b);
+            std::printf("workers1=%016llx workers12=%016llx %s\n", static_cast<unsigned long long>(ha),
+                        static_cast<unsigned long long>(hb), ha == hb ? "IDENTICAL" : "DIFFERENT");
+            return ha == hb ? 0 : 1;
+        }
+        if (argc < 5) {
+            std::fprintf(stderr, "usage: ref_driver <preset> <out-prefix> <steps> <workers> [snapshot ...]\n");
+            return 2;
+        }
+        const std::string preset = argv[1];
+        const std::string prefix = argv[2];
+        const int steps = std::atoi(argv[3]);
+        const unsigned workers = static_cast<unsigned>(std::atoi(argv[4]));
+        std::set<int> snapshots;
+        for (int k = 5; k < argc; ++k) snapshots.insert(std::atoi(argv[k]));
+
+        Prepared p = prepare(preset);
+        {
+            swfx::Writer w(prefix + ".assembly.swfx");
+            writeAssembly(w, p.assembly, p.hinges.order);
+            w.f64("face_pressure", p.wing.facePressure);
+        }
+        Rig rig = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure);
+        {
+            const auto view = rig.body->constraintColouringView();
+            swfx::Writer w(prefix + ".colouring.swfx");
+            w.u64("distance_order", std::vector<std::uint64_t>(view.order.begin(), view.order.end()));
+            w.u64("distance_offsets",
+                  std::vector<std::uint64_t>(view.colourOffsets.begin(), view.colourOffsets.end()));
+        }
+        swfx::Writer w(prefix + ".states.swfx");
+        const sw::StepSettings settings = stepSettings(workers);
+        std::vector<std::uint64_t> hashes;
+        double seconds = 0.0;
+        for (int s = 1; s <= steps; ++s) {
+            const auto t0 = std::chrono::steady_clock::now();
+            stepRig(rig, settings);
+            seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            hashes.push_back(stateHash(rig));
+            if (snapshots.count(s)) writeState(w, "step" + std::to_string(s), rig);
+        }
+        w.u64("state_hash", hashes);
+        std::printf("preset=%s nodes=%zu triangles=%zu constraints=%zu dihedrals=%zu segments=%zu "
+                    "steps=%d workers=%u ms_per_step=%.3f final_hash=%016llx\n",
+                    preset.c_str(), p.assembly.definition.nodes.size(),
+                    p.assembly.definition.triangles.size(), p.assembly.definition.constraints.size(),
+                    p.assembly.definition.dihedrals.size(),
+                    p.assembly.definition.suspension ? p.assembly.definition.suspension->segments.size() : 0,
+                    steps, workers, 1e3 * seconds / steps,
+                    static_cast<unsigned long long>(hashes.empty() ? 0 : hashes.back()));
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_driver error: %s\n", e.what());
+        return 1;
+    }
+}

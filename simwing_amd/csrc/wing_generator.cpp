@@ -1,0 +1,573 @@
+// Seeded synthetic ram-air wing (the benchmark workload generator).
+//
+// Geometry: an elliptic-arc planform of span `span` with an elliptic chord
+// distribution (floored at `tipChordFloor`), a straight quarter-chord line
+// and a closed-trailing-edge NACA 4-digit symmetric section. The upper and
+// lower skins are structured grids sharing their leading- and trailing-edge
+// rows (welded across sheets, so no hinge forms there). Every `ribEvery`
+// span stations a rib sheet with its own vertices (upper, camber and lower
+// rows) is sewn to both skins by two seams. Suspension: `lineCount` lines of
+// `segmentsPerLine` segments from lower-skin attachment points on the line
+// ribs to one of two harness points of a rigid pilot.
+//
+// Stable-ID layout (it fixes node, triangle and segment order):
+//  * vertices are numbered station by station: the station's skin vertices
+//    (upper row then lower interior row), then its rib vertices if any;
+//  * triangles are numbered station strip by station strip (upper skin,
+//    lower skin, then the rib at the strip's first station);
+//  * suspension line IDs are (1 + hop) * 100000 + line, so every segment
+//    string "segment:<id>" has six digits and the reference's string order
+//    is hop-major: all first hops, then all second hops, ... then the hops
+//    that reach the pilot.
+#include "scene.hpp"
+
+#include <cmath>
+#include <cstddef>
+#include <stdexcept>
+
+namespace swx {
+
+double seriesExp(double x) {
+    if (!(x == x)) return x;
+    int halvings = 0;
+    double r = x;
+    while (r > 0x1.0p-8 || r < -0x1.0p-8) {
+        r *= 0.5;
+        ++halvings;
+        if (halvings > 2000) break;
+    }
+    double term = 1.0;
+    double sum = 1.0;
+    for (int k = 1; k < 30; ++k) {
+        term = term * r / static_cast<double>(k);
+        sum += term;
+        if (term < 1.0e-18 * sum && term > -1.0e-18 * sum) break;
+    }
+    for (int i = 0; i < halvings; ++i) sum *= sum;
+    return sum;
+}
+
+namespace {
+
+// Natural log by Newton iteration on seriesExp (x > 0).
+double seriesLog(double x) {
+    double y = 0.0;
+    double scaled = x;
+    while (scaled > 2.0) {
+        scaled *= 0.5;
+        y += 0.69314718055994530942;
+    }
+    while (scaled < 0.5) {
+        scaled *= 2.0;
+        y -= 0.69314718055994530942;
+    }
+    double z = 0.0;
+    for (int it = 0; it < 60; ++it) {
+        const double e = seriesExp(z);
+        const double next = z + 2.0 * (scaled - e) / (scaled + e);
+        if (next == z) break;
+        z = next;
+    }
+    return y + z;
+}
+
+V3 sub(const V3& a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3 crossv(const V3& a, const V3& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+double dotv(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+double lengthv(const V3& a) { return std::sqrt(dotv(a, a)); }
+
+// Approximate standard normal draw: Irwin-Hall sum of 12 uniforms minus 6.
+double normalDraw(SplitMix64& rng) {
+    double sum = 0.0;
+    for (int k = 0; k < 12; ++k) sum += rng.uniform();
+    return sum - 6.0;
+}
+
+// Isometric material chart of a triangle: a at the origin, b on +x.
+std::array<V2, 3> flatChart(const V3& a, const V3& b, const V3& c) {
+    const V3 ab = sub(b, a);
+    const V3 ac = sub(c, a);
+    const double lab = lengthv(ab);
+    const V3 n = crossv(ab, ac);
+    return {V2{0.0, 0.0}, V2{lab, 0.0},
+            V2{dotv(ac, ab) / lab, lengthv(n) / lab}};
+}
+
+struct Builder {
+    GeneratedWing out;
+    std::vector<V3> positionById{V3{}};  // index = vertex id (id 0 unused)
+
+    StableId addVertex(const V3& p) {
+        const StableId id = positionById.size();
+        positionById.push_back(p);
+        out.scene.vertices.push_back({id, p});
+        return id;
+    }
+};
+
+}  // namespace
+
+GeneratedWing generateWing(const WingParams& P) {
+    if (P.spanNodes < 3 || P.chordNodes < 4 || P.ribEvery < 1 ||
+        P.segmentsPerLine < 2 || P.segmentsPerLine > 8 || P.lineCount < 1 ||
+        P.lineCount >= 100000 || P.lineRows < 1 || P.lineRows > 4 ||
+        !(P.span > 0.0) || !(P.rootChord > 0.0) || !(P.thickness > 0.0) ||
+        !(P.complianceMin > 0.0) || !(P.complianceMax >= P.complianceMin) ||
+        !(P.lineRestScale > 0.0) || P.contactSpheres < 0) {
+        throw std::invalid_argument("invalid synthetic wing parameters");
+    }
+    SplitMix64 rng(P.seed);
+    Builder B;
+    Scene& S = B.out.scene;
+    const int nS = P.spanNodes;
+    const int nC = P.chordNodes;
+
+    // Stations.
+    std::vector<double> ys(nS), zs(nS), chords(nS), les(nS);
+    for (int i = 0; i < nS; ++i) {
+        const double s = -1.0 + 2.0 * static_cast<double>(i) / static_cast<double>(nS - 1);
+        double root = 1.0 - s * s;
+        if (root < 0.0) root = 0.0;
+        const double elliptic = P.rootChord * std::sqrt(root);
+        chords[i] = elliptic > P.tipChordFloor ? elliptic : P.tipChordFloor;
+        ys[i] = 0.5 * P.span * s;
+        // Circular arc about the pilot: radius = line length, so lines meet
+        // the canopy at right angles.
+        const double radius = P.lineLength;
+        zs[i] = std::sqrt(radius * radius - ys[i] * ys[i]) - radius;
+        les[i] = 0.25 * (P.rootChord - chords[i]);
+    }
+    std::vector<double> us(nC);
+    for (int j = 0; j < nC; ++j) {
+        const double t = static_cast<double>(j) / static_cast<double>(nC - 1);
+        us[j] = t * t;
+    }
+    const auto halfThickness = [&](double chord, double u) {
+        const double su = std::sqrt(u);
+        return 5.0 * P.thickness * chord *
+               (0.2969 * su - 0.1260 * u - 0.3516 * u * u + 0.2843 * u * u * u -
+                0.1036 * u * u * u * u);
+    };
+    const auto surfacePoint = [&](int i, int j, int side) {
+        const double x = les[i] + us[j] * chords[i];
+        double dz = 0.0;
+        if (j != 0 && j != nC - 1 && side != 0) {
+            dz = static_cast<double>(side) * halfThickness(chords[i], us[j]);
+        }
+        return V3{x, ys[i], zs[i] + dz};
+    };
+
+    // Rib stations.
+    std::vector<int> ribOfStation(nS, -1);
+    std::vector<int> ribStations;
+    for (int i = 0; i < nS; i += P.ribEvery) {
+        ribOfStation[i] = static_cast<int>(ribStations.size());
+        ribStations.push_back(i);
+    }
+    const int nRibs = static_cast<int>(ribStations.size());
+
+    // Regions: one outside region, one cell per inter-rib bay.
+    S.regions.push_back({1, RegionKind::Outside, "outside"});
+    for (int k = 0; k <= nRibs; ++k) {
+        S.regions.push_back({static_cast<StableId>(2 + k), RegionKind::Cell,
+                             "cell-" + std::to_string(k)});
+    }
+    const auto cellOfStation = [&](int i) {
+        int k = 0;
+        for (int r = 0; r < nRibs; ++r) {
+            if (ribStations[r] <= i) k = r + 1;
+        }
+        return static_cast<StableId>(2 + k);
+    };
+
+    // Vertices, station by station.
+    std::vector<std::vector<StableId>> upper(nS, std::vector<StableId>(nC, 0));
+    std::vector<std::vector<StableId>> lower(nS, std::vector<StableId>(nC, 0));
+    struct RibVertices {
+        std::vector<StableId> up, mid, low;
+    };
+    std::vector<RibVertices> ribs(nRibs);
+    for (int i = 0; i < nS; ++i) {
+        for (int j = 0; j < nC; ++j) upper[i][j] = B.addVertex(surfacePoint(i, j, +1));
+        for (int j = 1; j < nC - 1; ++j) lower[i][j] = B.addVertex(surfacePoint(i, j, -1));
+        lower[i][0] = upper[i][0];
+        lower[i][nC - 1] = upper[i][nC - 1];
+        const int r = ribOfStation[i];
+        if (r >= 0) {
+            RibVertices& rv = ribs[r];
+            rv.up.assign(nC, 0);
+            rv.mid.assign(nC, 0);
+            rv.low.assign(nC, 0);
+            for (int j = 0; j < nC; ++j) rv.up[j] = B.addVertex(surfacePoint(i, j, +1));
+            for (int j = 1; j < nC - 1; ++j) rv.low[j] = B.addVertex(surfacePoint(i, j, -1));
+            for (int j = 1; j < nC - 1; ++j) rv.mid[j] = B.addVertex(surfacePoint(i, j, 0));
+            rv.low[0] = rv.mid[0] = rv.up[0];
+            rv.low[nC - 1] = rv.mid[nC - 1] = rv.up[nC - 1];
+        }
+    }
+
+    // Triangles.
+    const double logRatio = seriesLog(P.complianceMax / P.complianceMin);
+    const auto addTriangle = [&](StableId a, StableId b, StableId c, StableId sheet,
+                                 SurfaceRole role, StableId negative, StableId positive,
+                                 double pressure, const V3& outward) {
+        const V3& pa = B.positionById[a];
+        V3 pb = B.positionById[b];
+        V3 pc = B.positionById[c];
+        if (dotv(crossv(sub(pb, pa), sub(pc, pa)), outward) < 0.0) {
+            std::swap(b, c);
+            std::swap(pb, pc);
+        }
+        Triangle t;
+        t.id = S.triangles.size() + 1;
+        t.vertexIds = {a, b, c};
+        t.chart = flatChart(pa, pb, pc);
+        t.negativeRegion = negative;
+        t.positiveRegion = positive;
+        t.materialId = t.id;
+        t.sheetId = sheet;
+        t.role = role;
+        S.triangles.push_back(t);
+
+        const double compliance = P.complianceMin * seriesExp(rng.uniform() * logRatio);
+        FabricMaterial m;
+        m.id = t.id;
+        m.name = "fabric-" + std::to_string(t.id);
+        m.warpStiffness = 1.0 / compliance;
+        m.weftStiffness = 1.0 / compliance;
+        m.shearStiffness = 0.5 / compliance;
+        m.bendingStiffness = P.bendingRigidity;
+        m.arealDensity = P.fabricDensity;
+        S.fabricMaterials.push_back(m);
+
+        double p = pressure;
+        if (pressure != 0.0 && P.pressureStd > 0.0) {
+            p = pressure + P.pressureStd * normalDraw(rng);
+            if (p < 0.0) p = 0.0;
+        }
+        B.out.facePressure.push_back(p);
+    };
+    const V3 up{0.0, 0.0, 1.0};
+    const V3 down{0.0, 0.0, -1.0};
+    const V3 right{0.0, 1.0, 0.0};
+    const auto addRib = [&](int r) {
+        const RibVertices& rv = ribs[r];
+        const StableId sheet = 3 + static_cast<StableId>(r);
+        const StableId left = r == 0 ? 1 : static_cast<StableId>(2 + r);
+        const StableId rightCell = static_cast<StableId>(3 + r);
+        for (int band = 0; band < 2; ++band) {
+            const std::vector<StableId>& edge = band == 0 ? rv.up : rv.low;
+            for (int j = 0; j < nC - 1; ++j) {
+                const StableId e0 = edge[j], e1 = edge[j + 1];
+                const StableId m0 = rv.mid[j], m1 = rv.mid[j + 1];
+                if (j == 0) {
+                    addTriangle(e0, e1, m1, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
+                } else if (j == nC - 2) {
+                    addTriangle(e0, e1, m0, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
+                } else {
+                    addTriangle(e0, e1, m1, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
+                    addTriangle(e0, m1, m0, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
+                }
+            }
+        }
+    };
+    for (int i = 0; i + 1 < nS; ++i) {
+        const StableId cell = cellOfStation(i);
+        for (int j = 0; j + 1 < nC; ++j) {
+            addTriangle(upper[i][j], upper[i][j + 1], upper[i + 1][j + 1], 1,
+                        SurfaceRole::Skin, cell, 1, P.pressureMean, up);
+            addTriangle(upper[i][j], upper[i + 1][j + 1], upper[i + 1][j], 1,
+                        SurfaceRole::Skin, cell, 1, P.pressureMean, up);
+        }
+        for (int j = 0; j + 1 < nC; ++j) {
+            addTriangle(lower[i][j], lower[i][j + 1], lower[i + 1][j + 1], 2,
+                        SurfaceRole::Skin, cell, 1, P.pressureMean, down);
+            addTriangle(lower[i][j], lower[i + 1][j + 1], lower[i + 1][j], 2,
+                        SurfaceRole::Skin, cell, 1, P.pressureMean, down);
+        }
+        if (ribOfStation[i] >= 0) addRib(ribOfStation[i]);
+    }
+    if (ribOfStation[nS - 1] >= 0) addRib(ribOfStation[nS - 1]);
+
+    // Seams: rib upper row to the upper skin row (leading and trailing edge
+    // pairs included), rib lower interior row to the lower skin row.
+    S.seamMaterials.push_back({1, "seam-tape", P.seamLinearDensity, P.seamAxialStiffness});
+    for (int r = 0; r < nRibs; ++r) {
+        const int i = ribStations[r];
+        Seam upperSeam;
+        upperSeam.id = 2 * static_cast<StableId>(r) + 1;
+        upperSeam.materialId = 1;
+        for (int j = 0; j < nC; ++j) {
+            upperSeam.first.push_back(upper[i][j]);
+            upperSeam.second.push_back(ribs[r].up[j]);
+        }
+        Seam lowerSeam;
+        lowerSeam.id = 2 * static_cast<StableId>(r) + 2;
+        lowerSeam.materialId = 1;
+        for (int j = 1; j < nC - 1; ++j) {
+            lowerSeam.first.push_back(lower[i][j]);
+            lowerSeam.second.push_back(ribs[r].low[j]);
+        }
+        S.seams.push_back(std::move(upperSeam));
+        S.seams.push_back(std::move(lowerSeam));
+    }
+
+    // Suspension.
+    const double rowFractions[4] = {0.12, 0.38, 0.64, 0.85};
+    const int perRow = (P.lineCount + P.lineRows - 1) / P.lineRows;
+    Pilot pilot;
+    pilot.id = 1;
+    pilot.name = "pilot";
+    pilot.mass = P.pilotMass;
+    pilot.centerOfMass = V3{0.35 * P.rootChord, 0.0, -P.lineLength - 0.35};
+    pilot.principalInertia = P.pilotInertia;
+    S.pilots.push_back(pilot);
+    S.lineMaterials.push_back({1, "line", 0.0015, P.lineLinearDensity, P.lineAxialStiffness, 1.0});
+
+    const StableId junctionBase = B.positionById.size() + 1000;
+    const StableId harnessLeft = static_cast<StableId>(P.lineCount) + 1;
+    const StableId harnessRight = static_cast<StableId>(P.lineCount) + 2;
+    const V3 localLeft{0.0, -0.22, 0.35};
+    const V3 localRight{0.0, 0.22, 0.35};
+    Attachment hl;
+    hl.id = harnessLeft;
+    hl.kind = AttachmentKind::PilotHarness;
+    hl.pilotId = 1;
+    hl.pilotLocal = localLeft;
+    Attachment hr = hl;
+    hr.id = harnessRight;
+    hr.pilotLocal = localRight;
+    std::vector<Attachment> surfaceAttachments, junctionAttachments;
+    StableId nextJunctionAttachment = static_cast<StableId>(P.lineCount) + 3;
+    const int segs = P.segmentsPerLine;
+    std::vector<std::vector<StableId>> lineNodes(P.lineCount);  // attachment ids per hop
+    std::vector<std::vector<V3>> linePoints(P.lineCount);
+    for (int l = 0; l < P.lineCount; ++l) {
+        const int row = l % P.lineRows;
+        const int pick = l / P.lineRows;
+        const int r = perRow > 1 ? static_cast<int>(
+                                        (static_cast<long long>(pick) * (nRibs - 1) +
+                                         (perRow - 1) / 2) /
+                                        (perRow - 1))
+                                  : nRibs / 2;
+        const int i = ribStations[r];
+        int bestJ = 1;
+        double bestD = 1.0e300;
+        for (int j = 1; j < nC - 1; ++j) {
+            const double d = us[j] > rowFractions[row] ? us[j] - rowFractions[row]
+                                                       : rowFractions[row] - us[j];
+            if (d < bestD) {
+                bestD = d;
+                bestJ = j;
+            }
+        }
+        const StableId vertex = lower[i][bestJ];
+        const V3 a = B.positionById[vertex];
+        const bool isLeft = ys[i] < 0.0;
+        const V3 local = isLeft ? localLeft : localRight;
+        const V3 h{pilot.centerOfMass.x + local.x, pilot.centerOfMass.y + local.y,
+                   pilot.centerOfMass.z + local.z};
+        Attachment sa;
+        sa.id = static_cast<StableId>(l) + 1;
+        sa.kind = AttachmentKind::SurfaceVertex;
+        sa.vertexId = vertex;
+        surfaceAttachments.push_back(sa);
+        lineNodes[l].push_back(sa.id);
+        linePoints[l].push_back(a);
+        for (int k = 1; k < segs; ++k) {
+            const double f = static_cast<double>(k) / static_cast<double>(segs);
+            const V3 p{a.x + f * (h.x - a.x), a.y + f * (h.y - a.y), a.z + f * (h.z - a.z)};
+            Junction jn;
+            jn.id = junctionBase + static_cast<StableId>(l) * static_cast<StableId>(segs) +
+                    static_cast<StableId>(k);
+            jn.position = p;
+            S.junctions.push_back(jn);
+            Attachment ja;
+            ja.id = nextJunctionAttachment++;
+            ja.kind = AttachmentKind::SuspensionJunction;
+            ja.junctionId = jn.id;
+            junctionAttachments.push_back(ja);
+            lineNodes[l].push_back(ja.id);
+            linePoints[l].push_back(p);
+        }
+        lineNodes[l].push_back(isLeft ? harnessLeft : harnessRight);
+        linePoints[l].push_back(h);
+    }
+    // Junction masses: one segment of line each, nudged upward until the
+    // reciprocal of its reciprocal inverse mass is exact (the reference
+    // re-derives a junction's mass from the node's inverse mass).
+    for (int l = 0; l < P.lineCount; ++l) {
+        for (int k = 1; k < segs; ++k) {
+            Junction& jn = S.junctions[static_cast<std::size_t>(l) * (segs - 1) + (k - 1)];
+            const double segment = lengthv(sub(linePoints[l][k + 1], linePoints[l][k]));
+            double mass = P.lineLinearDensity * segment;
+            if (mass < 1.0e-4) mass = 1.0e-4;
+            for (int guard = 0; guard < 64; ++guard) {
+                const double w = 1.0 / mass;
+                if (1.0 / (1.0 / w) == w) break;
+                mass = std::nextafter(mass, 1.0e300);
+            }
+            jn.mass = mass;
+        }
+    }
+    for (const Attachment& a : surfaceAttachments) S.attachments.push_back(a);
+    S.attachments.push_back(hl);
+    S.attachments.push_back(hr);
+    for (const Attachment& a : junctionAttachments) S.attachments.push_back(a);
+    for (int hop = 0; hop < segs; ++hop) {
+        for (int l = 0; l < P.lineCount; ++l) {
+            Line line;
+            line.id = static_cast<StableId>(1 + hop) * 100000 + static_cast<StableId>(l);
+            line.startAttachment = lineNodes[l][hop];
+            line.endAttachment = lineNodes[l][hop + 1];
+            line.materialId = 1;
+            line.restLength =
+                P.lineRestScale * lengthv(sub(linePoints[l][hop + 1], linePoints[l][hop]));
+            line.role = LineRole::Suspension;
+            S.lines.push_back(line);
+        }
+    }
+
+    // Contact spheres: centred on the first attachment vertices, holding off
+    // the skin vertices of the neighbouring span stations within 4 radii.
+    for (int c = 0; c < P.contactSpheres && c < P.lineCount; ++c) {
+        ContactSphere sphere;
+        sphere.centreVertex = surfaceAttachments[static_cast<std::size_t>(c) *
+                                                 (P.lineCount / P.contactSpheres)]
+                                  .vertexId;
+        sphere.radius = P.contactRadius;
+        const V3 centre = B.positionById[sphere.centreVertex];
+        for (int i = 0; i < nS; ++i) {
+            for (int j = 0; j < nC; ++j) {
+                const StableId candidates[2] = {upper[i][j], lower[i][j]};
+                for (const StableId v : candidates) {
+                    if (v == sphere.centreVertex) continue;
+                    const double d = lengthv(sub(B.positionById[v], centre));
+                    if (d < 4.0 * P.contactRadius && d > 0.0) {
+                        bool seen = false;
+                        for (const StableId q : sphere.vertices) seen = seen || q == v;
+                        if (!seen) sphere.vertices.push_back(v);
+                    }
+                }
+            }
+        }
+        B.out.contactSpheres.push_back(std::move(sphere));
+    }
+    return std::move(B.out);
+}
+
+}  // namespace swx
+
+namespace swx {
+
+WingParams wingPreset(const std::string& name) {
+    WingParams p;
+    if (name == "small") return p;
+    if (name == "tiny") {
+        p.spanNodes = 6;
+        p.chordNodes = 5;
+        p.span = 2.0;
+        p.rootChord = 0.5;
+        p.lineRows = 2;
+        p.lineCount = 4;
+        p.segmentsPerLine = 3;
+        p.lineLength = 2.0;
+        p.pilotMass = 20.0;
+        p.pilotInertia = V3{2.0, 2.0, 2.0};
+        p.seed = 0x5EED0000ULL + 100;
+        return p;
+    }
+    if (name == "export") {
+        p.seed = 0x5EED0001ULL;
+        p.spanNodes = 152;
+        p.chordNodes = 76;
+        p.lineRows = 2;
+        p.lineCount = 50;
+        p.pressureMean = 60.0;
+        p.pressureStd = 15.0;
+        p.contactSpheres = 8;
+        return p;
+    }
+    if (name == "variant10k") {
+        p.seed = 0x5EED0003ULL;
+        p.spanNodes = 56;
+        p.chordNodes = 32;
+        return p;
+    }
+    throw std::invalid_argument("unknown wing preset: " + name);
+}
+
+std::vector<WingParams> batchPresets(std::uint64_t seed, int count, double minNodes,
+                                     double maxNodes) {
+    if (count < 1 || !(minNodes > 0.0) || !(maxNodes >= minNodes)) {
+        throw std::invalid_argument("invalid batch preset request");
+    }
+    SplitMix64 rng(seed);
+    double logRatio = 0.0;
+    {
+        // log(maxNodes / minNodes) by Newton iteration on seriesExp.
+        const double x = maxNodes / minNodes;
+        double z = 0.0;
+        for (int it = 0; it < 200; ++it) {
+            const double e = seriesExp(z);
+            const double next = z + 2.0 * (x - e) / (x + e);
+            if (next == z) break;
+            z = next;
+        }
+        logRatio = z;
+    }
+    std::vector<WingParams> out;
+    for (int w = 0; w < count; ++w) {
+        WingParams p;
+        p.seed = seed + 0x1000ULL * static_cast<std::uint64_t>(w + 1);
+        const double nodes = minNodes * seriesExp(rng.uniform() * logRatio);
+        // nodes ~ 3.5 * span * chord for a 2:1 grid with ribs every 2 stations.
+        int chord = static_cast<int>(std::sqrt(nodes / 7.0) + 0.5);
+        if (chord < 16) chord = 16;
+        p.chordNodes = chord;
+        p.spanNodes = 2 * chord;
+        p.span = 6.0 + 6.0 * rng.uniform();
+        p.rootChord = p.span / 8.0;
+        p.pressureMean = 20.0 + 80.0 * rng.uniform();
+        p.lineRestScale = 0.97 + 0.06 * rng.uniform();
+        out.push_back(p);
+    }
+    return out;
+}
+
+}  // namespace swx
+
+namespace swx {
+
+void flightLoads(const double* position, const double* mass, std::size_t nodeCount,
+                 std::size_t surfaceNodes, double totalWeight, const V3& centre, double* out) {
+    double surfaceMass = 0.0;
+    for (std::size_t i = 0; i < surfaceNodes; ++i) surfaceMass += mass[i];
+    double vertical = 0.0;
+    for (std::size_t i = 0; i < surfaceNodes; ++i) {
+        const double dx = position[3 * i] - centre.x;
+        const double dy = position[3 * i + 1] - centre.y;
+        const double dz = position[3 * i + 2] - centre.z;
+        const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        vertical += (mass[i] / surfaceMass) * (dz / r);
+    }
+    const double scale = totalWeight / vertical;
+    for (std::size_t i = 0; i < nodeCount; ++i) {
+        if (i >= surfaceNodes) {
+            out[3 * i] = out[3 * i + 1] = out[3 * i + 2] = 0.0;
+            continue;
+        }
+        const double dx = position[3 * i] - centre.x;
+        const double dy = position[3 * i + 1] - centre.y;
+        const double dz = position[3 * i + 2] - centre.z;
+        const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        const double f = scale * (mass[i] / surfaceMass) / r;
+        out[3 * i] = f * dx;
+        out[3 * i + 1] = f * dy;
+        out[3 * i + 2] = f * dz;
+    }
+}
+
+}  // namespace swx
