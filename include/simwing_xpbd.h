@@ -1,0 +1,314 @@
+/*
+ * simwing_xpbd.h -- C-ABI of the MI355X XPBD structural substep.
+ *
+ * This is the drop-in boundary for SimWing's structural adapter
+ * simwing::fsi::Structure (reference src/fsi/structure.h:273-309) and the
+ * SoftBody-level extensions it needs (per-face pressure, soft_body.h:401-402;
+ * colouring view, soft_body.h:462-474). Everything is plain C: opaque
+ * handles, SoA arrays of doubles/uint64, status codes. Caller arrays are
+ * borrowed for the duration of the call only.
+ *
+ * Errors: every call returns an sw_status. The code mirrors the C++
+ * exception class the reference throws (std::invalid_argument ->
+ * SW_INVALID_ARGUMENT, std::out_of_range -> SW_OUT_OF_RANGE,
+ * softwing::SuspensionError -> SW_SUSPENSION_ERROR with phase and entity,
+ * std::logic_error -> SW_LOGIC_ERROR); SW_DEVICE_ERROR reports a HIP runtime
+ * failure. The message of the last failure on the calling thread is
+ * available from sw_last_error().
+ *
+ * Threading: a structure is single-owner and not thread-safe, like the
+ * reference adapter. Each structure owns one HIP stream; sw_structure_step
+ * is synchronous and transactional (a failed step restores the exact
+ * pre-step state before returning the error, structure.cpp:807-831).
+ */
+#ifndef SIMWING_XPBD_H
+#define SIMWING_XPBD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sw_status {
+    SW_OK = 0,
+    SW_INVALID_ARGUMENT = 1,
+    SW_OUT_OF_RANGE = 2,
+    SW_SUSPENSION_ERROR = 3,
+    SW_LOGIC_ERROR = 4,
+    SW_DEVICE_ERROR = 5,
+    SW_BAD_ALLOC = 6
+} sw_status;
+
+/* Last failure on this thread: message, and for SW_SUSPENSION_ERROR the
+ * softwing::SuspensionPhase value (suspension.h:29-40) and entity id. */
+const char* sw_last_error(void);
+int sw_last_suspension_phase(void);
+const char* sw_last_suspension_entity(void);
+
+/* ---- Definition (structure.h:30-230, as SoA views) ------------------- */
+
+enum {
+    SW_CONSTRAINT_DISTANCE = 1,
+    SW_CONSTRAINT_CABLE = 2,
+    SW_CONSTRAINT_SUSPENSION_TIE = 3,
+    SW_CONSTRAINT_SEAM_STITCH = 4
+};
+enum { SW_ENDPOINT_ATTACHMENT = 1, SW_ENDPOINT_JUNCTION = 2, SW_ENDPOINT_HARNESS = 3 };
+enum { SW_ROLE_BULK = 1, SW_ROLE_SEAM = 2, SW_ROLE_REINFORCEMENT = 3 };
+
+/* StructureSuspensionDefinition (structure.h:147-166). Arrays are indexed
+ * by entry; segment endpoint kinds use SW_ENDPOINT_*. */
+typedef struct sw_suspension_definition {
+    uint64_t pilot_stable_id;
+    double pilot_mass;
+    double pilot_com_local[3];
+    double pilot_com_world[3];
+    double pilot_velocity[3];
+    double pilot_angular_velocity[3];
+    double pilot_orientation[4]; /* w, x, y, z */
+    double pilot_inertia[3];     /* principal, body frame */
+    size_t attachment_count;
+    const uint64_t* attachment_ids;
+    const uint64_t* attachment_nodes;
+    size_t junction_count;
+    const uint64_t* junction_ids;
+    const uint64_t* junction_nodes;
+    size_t harness_count;
+    const uint64_t* harness_ids;
+    const double* harness_local; /* 3 per harness point */
+    size_t segment_count;
+    const uint64_t* segment_ids;
+    const uint8_t* segment_from_kind;
+    const uint64_t* segment_from_id;
+    const uint8_t* segment_to_kind;
+    const uint64_t* segment_to_id;
+    const double* segment_rest;
+    const double* segment_stiffness;
+    const double* segment_damping;
+    const uint32_t* segment_role;
+    int solver_iterations;
+    double attachment_tolerance;
+    double minimum_line_length;
+    double maximum_line_residual;
+    double maximum_control_work;
+} sw_suspension_definition;
+
+/* StructureDefinition (structure.h:168-176). Node indices are uint64.
+ * membrane_chart: 6 per membrane (three x,y corners); membrane_material: 8
+ * per membrane (warp, weft, coupling, shear, warp pre-tension, weft
+ * pre-tension, damping seconds, compression ratio).
+ * Contact spheres are a build-defined family with no reference
+ * counterpart: sphere k is centred on node sphere_centre[k] with radius
+ * sphere_radius[k] and compliance sphere_compliance[k], and holds off the
+ * nodes sphere_nodes[sphere_offsets[k] .. sphere_offsets[k+1]). */
+typedef struct sw_definition {
+    size_t node_count;
+    const double* node_position; /* 3 per node */
+    const double* node_mass;
+    const uint8_t* node_fixed;
+    size_t triangle_count;
+    const uint64_t* triangle_nodes; /* 3 per triangle */
+    size_t constraint_count;
+    const uint8_t* constraint_kind;
+    const uint64_t* constraint_nodes; /* 2 per constraint */
+    const double* constraint_rest;
+    const double* constraint_compliance;
+    size_t membrane_count;
+    const uint64_t* membrane_triangle;
+    const double* membrane_chart;
+    const double* membrane_material;
+    const uint8_t* membrane_role;
+    size_t dihedral_count;
+    const uint64_t* dihedral_nodes; /* 4 per hinge */
+    const double* dihedral_rest_angle;
+    const double* dihedral_compliance;
+    const sw_suspension_definition* suspension; /* NULL: no pilot */
+    size_t sphere_count;
+    const uint64_t* sphere_centre;
+    const double* sphere_radius;
+    const double* sphere_compliance;
+    const uint64_t* sphere_offsets; /* sphere_count + 1 */
+    const uint64_t* sphere_nodes;
+} sw_definition;
+
+/* StructureStepSettings (structure.h:208-219). worker_threads is accepted
+ * for interface parity and ignored: the device sweep is the reference's
+ * coloured sweep (workerThreads >= 1) at every setting except 0, which the
+ * device does not implement (the serial index-order sweep) and rejects. */
+typedef struct sw_step_settings {
+    double time_step;
+    int substeps;
+    int constraint_iterations;
+    int cable_sweep_pairs;
+    double gravity[3];
+    double velocity_damping;
+    double damping_reference_velocity[3];
+    unsigned worker_threads;
+} sw_step_settings;
+
+/* StructureDiagnostics (structure.h:221-246). */
+typedef struct sw_diagnostics {
+    size_t node_count;
+    size_t dynamic_node_count;
+    size_t triangle_count;
+    size_t constraint_count;
+    size_t membrane_count;
+    size_t dihedral_count;
+    size_t contact_pair_count;
+    size_t active_contact_count;
+    size_t suspension_segment_count;
+    double total_dynamic_mass;
+    double center_of_mass[3];
+    double linear_momentum[3];
+    double kinetic_energy;
+    double maximum_distance_error;
+    double maximum_cable_extension;
+    double maximum_absolute_membrane_strain;
+    double maximum_membrane_residual;
+    double maximum_contact_penetration;
+    double maximum_suspension_residual;
+    double pending_external_force[3];
+    double last_applied_external_force[3];
+    int finite;
+} sw_diagnostics;
+
+/* ---- Structure (structure.h:273-309) ---------------------------------- */
+
+typedef struct sw_structure sw_structure;
+typedef struct sw_checkpoint sw_checkpoint;
+
+/* Structure(StructureDefinition) (structure.cpp:682-685): validates, builds
+ * the colour schedules and uploads the SoA tables to device `device`. */
+sw_status sw_structure_create(const sw_definition* definition, int device, sw_structure** out);
+void sw_structure_destroy(sw_structure* structure);
+
+/* definitionFingerprint() (structure.cpp:326-431), acceptedStepCount(),
+ * simulationTimeSeconds(). */
+sw_status sw_structure_fingerprint(const sw_structure* s, uint64_t* fingerprint);
+sw_status sw_structure_progress(const sw_structure* s, uint64_t* accepted_steps, double* time);
+
+/* clearExternalForces / addExternalForce / setExternalForces
+ * (structure.cpp:742-776): pending loads consumed by the next accepted step. */
+sw_status sw_structure_clear_external_forces(sw_structure* s);
+sw_status sw_structure_add_external_force(sw_structure* s, size_t node, const double force[3]);
+sw_status sw_structure_set_external_forces(sw_structure* s, const double* forces, size_t node_count);
+
+/* SoftBody::setFacePressureDifference for every triangle (soft_body.h:401). */
+sw_status sw_structure_set_face_pressure(sw_structure* s, const double* pressure, size_t triangle_count);
+
+/* step(settings) -> diagnostics (structure.cpp:807-831); diagnostics()
+ * (structure.cpp:833-959). `diagnostics` may be NULL. */
+sw_status sw_structure_step(sw_structure* s, const sw_step_settings* settings, sw_diagnostics* diagnostics);
+sw_status sw_structure_diagnostics(sw_structure* s, sw_diagnostics* diagnostics);
+
+/* Steps `count` times back to back on the device without reading back
+ * diagnostics between steps (the benchmark loop). Same transactional rule
+ * per step; stops at the first failure. */
+sw_status sw_structure_step_many(sw_structure* s, const sw_step_settings* settings, int count);
+
+/* nodeStates() (structure.cpp:715-724): 3 doubles per node per array; any
+ * pointer may be NULL. */
+sw_status sw_structure_node_states(const sw_structure* s, double* position, double* previous, double* velocity);
+
+/* Payload state (structure.h:195-206): com[3], orientation[4] (w,x,y,z),
+ * linear velocity[3], angular velocity[3] = 13 doubles. */
+sw_status sw_structure_payload_state(const sw_structure* s, double* payload13);
+
+/* Solver multipliers and residuals, in definition order (constraints,
+ * membranes, dihedrals) and runtime segment order (string-id order of
+ * "segment:<id>", suspension.cpp:996-1005). Any pointer may be NULL.
+ * membrane_lambda, membrane_resultant: 3 per membrane. */
+sw_status sw_structure_solver_state(const sw_structure* s, double* constraint_lambda,
+                                    double* membrane_lambda, double* membrane_resultant,
+                                    double* membrane_residual, double* dihedral_lambda,
+                                    double* segment_lambda);
+
+/* checkpoint() / restore() (structure.cpp:961-1025). A checkpoint is an
+ * immutable value owned by the caller until released; it stays valid after
+ * its structure is destroyed. */
+sw_status sw_structure_checkpoint(const sw_structure* s, sw_checkpoint** out);
+sw_status sw_structure_restore(sw_structure* s, const sw_checkpoint* checkpoint);
+void sw_checkpoint_release(sw_checkpoint* checkpoint);
+
+/* Colour schedule of a family, colour-major: family 0 = distance
+ * constraints (= SoftBody::constraintColouringView, soft_body.h:462-474),
+ * 1 = membranes, 2 = dihedral hinges. Pass NULL arrays to query sizes. */
+sw_status sw_structure_colouring(const sw_structure* s, int family, uint64_t* order,
+                                 size_t* order_count, uint64_t* offsets, size_t* offset_count);
+
+/* The same colour schedules computed on the host from a definition, with
+ * no device involved (family 0, 1, 2 as above; family 3 = suspension
+ * segments in runtime string-id order, reported as definition indices with
+ * one "colour" per dependency level). */
+sw_status sw_definition_colouring(const sw_definition* definition, int family, uint64_t* order,
+                                  size_t* order_count, uint64_t* offsets, size_t* offset_count);
+
+/* Device math conformance probe: evaluates `function` (0 sqrt, 1 divide
+ * a/b, 2 atan2(a, b), 3 remainder(a, 2*pi), 4 sin, 5 cos, 6 acos) on the
+ * device for `count` inputs (a[i], b[i]) and writes the results to out. */
+sw_status sw_device_math(int device, int function, const double* a, const double* b, double* out, size_t count);
+
+/* Device launch shape (workgroups used by one step) and the autotuner. */
+sw_status sw_structure_set_launch(sw_structure* s, int workgroups, int threads);
+sw_status sw_structure_get_launch(const sw_structure* s, int* workgroups, int* threads);
+
+/* ---- Scene input and assembly (scene_structure.h:162-165) ------------- */
+
+typedef struct sw_scene sw_scene;
+typedef struct sw_assembly sw_assembly;
+
+/* Seeded synthetic wing by preset name ("tiny", "small", "export",
+ * "variant10k"), or a batch member (preset "batch:<seed>:<index>:<count>"). */
+sw_status sw_scene_generate(const char* preset, sw_scene** out);
+void sw_scene_destroy(sw_scene* scene);
+/* Per-triangle pressure generated with the scene (ascending triangle id). */
+sw_status sw_scene_face_pressure(const sw_scene* scene, const double** pressure, size_t* count);
+
+/* SceneStructureSettings (scene_structure.h:27-39) subset. */
+typedef struct sw_assembly_settings {
+    double seam_coincidence_tolerance;
+    int suspension_iterations;
+    double suspension_attachment_tolerance;
+    double suspension_minimum_line_length;
+    double suspension_maximum_line_residual;
+    double suspension_maximum_control_work;
+    double contact_sphere_compliance;
+} sw_assembly_settings;
+
+/* assembleSceneStructure (scene_structure.cpp:894-1282): integer-exact node,
+ * triangle, constraint, membrane, dihedral and suspension tables. */
+sw_status sw_scene_assemble(const sw_scene* scene, const sw_assembly_settings* settings, sw_assembly** out);
+void sw_assembly_destroy(sw_assembly* assembly);
+/* Borrowed view of the assembled definition (valid while `assembly` lives). */
+sw_status sw_assembly_definition(const sw_assembly* assembly, sw_definition* out);
+/* Mappings: stable ids of vertex nodes and junction nodes, fabric and seam
+ * masses (scene_structure.h:83-140). */
+sw_status sw_assembly_info(const sw_assembly* assembly, size_t* vertex_nodes, size_t* junction_nodes,
+                           double* fabric_mass, double* seam_mass);
+
+/* Steady flight loads of the benchmark workload: the system weight carried
+ * by the vertex nodes, mass-weighted, radially away from the pilot. */
+sw_status sw_assembly_flight_loads(const sw_assembly* assembly, double* forces);
+
+/* ---- Batched wings ----------------------------------------------------- */
+
+typedef struct sw_batch sw_batch;
+
+/* Many wings of uneven size stepped together in one persistent launch
+ * over concatenated tables with per-wing offsets. */
+sw_status sw_batch_create(const sw_definition* const* definitions, size_t count, int device, sw_batch** out);
+void sw_batch_destroy(sw_batch* batch);
+sw_status sw_batch_set_face_pressure(sw_batch* b, size_t wing, const double* pressure, size_t count);
+sw_status sw_batch_set_external_forces(sw_batch* b, size_t wing, const double* forces, size_t node_count);
+/* Steps every wing `count` times; loads persist across these steps. */
+sw_status sw_batch_step(sw_batch* b, const sw_step_settings* settings, int count);
+sw_status sw_batch_node_states(const sw_batch* b, size_t wing, double* position, double* velocity);
+sw_status sw_batch_set_launch(sw_batch* b, int workgroups_per_wing_cap, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMWING_XPBD_H */

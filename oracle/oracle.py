@@ -1,0 +1,108 @@
+"""ORACLE (test infrastructure only): ctypes wrapper of liboracle_xpbd.so,
+the CPU restatement of the reference's XPBD substep. Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "liboracle_xpbd.so"
+_lib = None
+
+
+def library():
+    global _lib
+    if _lib is None:
+        if not LIBRARY_PATH.exists():
+            raise RuntimeError(f"{LIBRARY_PATH} is missing: run make -C oracle")
+        lib = C.CDLL(str(LIBRARY_PATH))
+        lib.oracle_last_error.restype = C.c_char_p
+        lib.oracle_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        lib.oracle_destroy.argtypes = [C.c_void_p]
+        lib.oracle_destroy.restype = None
+        lib.oracle_set_face_pressure.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        lib.oracle_set_external_forces.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        lib.oracle_step.argtypes = [C.c_void_p, C.c_void_p]
+        lib.oracle_state.argtypes = [C.c_void_p] + [C.c_void_p] * 11
+        lib.oracle_colouring.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_size_t), C.c_void_p,
+                                         C.POINTER(C.c_size_t)]
+        lib.oracle_set_angle_perturbation.argtypes = [C.c_void_p, C.c_int]
+        lib.oracle_set_angle_perturbation.restype = None
+        _lib = lib
+    return _lib
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+def _check(status):
+    if status != 0:
+        raise OracleError(status, library().oracle_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OracleBody:
+    """CPU XPBD body in the reference's coloured sweep order."""
+
+    def __init__(self, definition):
+        lib = library()
+        view = definition.to_ctypes()
+        self._definition = definition
+        handle = C.c_void_p()
+        _check(lib.oracle_create(C.byref(view), C.byref(handle)))
+        self._handle = handle
+        a = definition.arrays
+        self.n = a["node_mass"].size
+        self.c = a["constraint_kind"].size
+        self.m = a["membrane_triangle"].size
+        self.h = a["dihedral_rest_angle"].size
+        self.s = 0 if definition.suspension is None else definition.suspension["segment_ids"].size
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            library().oracle_destroy(self._handle)
+            self._handle = None
+
+    def set_face_pressure(self, pressure):
+        p = np.ascontiguousarray(pressure, dtype=np.float64)
+        _check(library().oracle_set_face_pressure(self._handle, _p(p), p.size))
+
+    def set_external_forces(self, forces):
+        f = np.ascontiguousarray(forces, dtype=np.float64).reshape(-1)
+        _check(library().oracle_set_external_forces(self._handle, _p(f), f.size // 3))
+
+    def step(self, settings):
+        _check(library().oracle_step(self._handle, C.byref(settings)))
+
+    def perturb_angles(self, ulps: int):
+        library().oracle_set_angle_perturbation(self._handle, ulps)
+
+    def state(self) -> dict:
+        out = {
+            "position": np.zeros(3 * self.n), "previous": np.zeros(3 * self.n), "velocity": np.zeros(3 * self.n),
+            "constraint_lambda": np.zeros(self.c), "membrane_lambda": np.zeros(3 * self.m),
+            "membrane_resultant": np.zeros(3 * self.m), "membrane_residual": np.zeros(self.m),
+            "dihedral_lambda": np.zeros(self.h), "segment_lambda": np.zeros(self.s), "payload": np.zeros(13),
+            "suspension_residual": np.zeros(1),
+        }
+        _check(library().oracle_state(self._handle, *[_p(out[k]) for k in (
+            "position", "previous", "velocity", "constraint_lambda", "membrane_lambda", "membrane_resultant",
+            "membrane_residual", "dihedral_lambda", "segment_lambda", "payload", "suspension_residual")]))
+        return out
+
+    def colouring(self, family):
+        lib = library()
+        nc, no = C.c_size_t(), C.c_size_t()
+        _check(lib.oracle_colouring(self._handle, family, None, C.byref(nc), None, C.byref(no)))
+        order = np.zeros(nc.value, np.uint64)
+        offsets = np.zeros(no.value, np.uint64)
+        _check(lib.oracle_colouring(self._handle, family, _p(order), None, _p(offsets), None))
+        return order, offsets

@@ -1,0 +1,491 @@
+"""MI355X XPBD structural substep: host-side mirror of SimWing's structural
+adapter (reference src/fsi/structure.h:273-309) over the C-ABI in
+include/simwing_xpbd.h.
+
+``Structure`` keeps the reference adapter's method names and error
+behaviour (exceptions carry the C status that mirrors the reference's C++
+exception class); ``generate_scene`` / ``assemble`` produce the seeded
+synthetic wings and the integer-exact structural definition
+(scene_structure.cpp:894-1282). All compute runs in the HIP library; there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import native
+from .native import NativeError, check, f64p, u8p, u32p, u64p
+
+__all__ = [
+    "NativeError",
+    "DefinitionArrays",
+    "Scene",
+    "Assembly",
+    "Structure",
+    "Batch",
+    "generate_scene",
+    "flight_step_settings",
+    "flight_assembly_settings",
+]
+
+flight_step_settings = native.flight_step_settings
+flight_assembly_settings = native.flight_assembly_settings
+
+_DEF_KEYS = {
+    "node_position": np.float64,
+    "node_mass": np.float64,
+    "node_fixed": np.uint8,
+    "triangle_nodes": np.uint64,
+    "constraint_kind": np.uint8,
+    "constraint_nodes": np.uint64,
+    "constraint_rest": np.float64,
+    "constraint_compliance": np.float64,
+    "membrane_triangle": np.uint64,
+    "membrane_chart": np.float64,
+    "membrane_material": np.float64,
+    "membrane_role": np.uint8,
+    "dihedral_nodes": np.uint64,
+    "dihedral_rest_angle": np.float64,
+    "dihedral_compliance": np.float64,
+    "sphere_centre": np.uint64,
+    "sphere_radius": np.float64,
+    "sphere_compliance": np.float64,
+    "sphere_offsets": np.uint64,
+    "sphere_nodes": np.uint64,
+}
+_SUSP_KEYS = {
+    "attachment_ids": np.uint64,
+    "attachment_nodes": np.uint64,
+    "junction_ids": np.uint64,
+    "junction_nodes": np.uint64,
+    "harness_ids": np.uint64,
+    "harness_local": np.float64,
+    "segment_ids": np.uint64,
+    "segment_from_kind": np.uint8,
+    "segment_from_id": np.uint64,
+    "segment_to_kind": np.uint8,
+    "segment_to_id": np.uint64,
+    "segment_rest": np.float64,
+    "segment_stiffness": np.float64,
+    "segment_damping": np.float64,
+    "segment_role": np.uint32,
+}
+
+
+@dataclass
+class DefinitionArrays:
+    """Owning SoA structural definition (StructureDefinition, structure.h:168-176)."""
+
+    arrays: dict
+    suspension: dict | None = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def node_count(self) -> int:
+        return int(self.arrays["node_mass"].size)
+
+    @property
+    def triangle_count(self) -> int:
+        return int(self.arrays["triangle_nodes"].size // 3)
+
+    def copy(self) -> "DefinitionArrays":
+        susp = None if self.suspension is None else {k: (v.copy() if isinstance(v, np.ndarray) else v)
+                                                      for k, v in self.suspension.items()}
+        return DefinitionArrays({k: v.copy() for k, v in self.arrays.items()}, susp)
+
+    def to_ctypes(self) -> native.Definition:
+        a = {k: np.ascontiguousarray(self.arrays.get(k, np.zeros(0, dt)), dtype=dt) for k, dt in _DEF_KEYS.items()}
+        if a["sphere_offsets"].size == 0:
+            a["sphere_offsets"] = np.zeros(1, np.uint64)
+        if a["membrane_role"].size == 0:
+            a["membrane_role"] = np.ones(a["membrane_triangle"].size, np.uint8)
+        keep = list(a.values())
+        d = native.Definition()
+        d.node_count = a["node_mass"].size
+        d.node_position = f64p(a["node_position"])
+        d.node_mass = f64p(a["node_mass"])
+        d.node_fixed = u8p(a["node_fixed"])
+        d.triangle_count = a["triangle_nodes"].size // 3
+        d.triangle_nodes = u64p(a["triangle_nodes"])
+        d.constraint_count = a["constraint_kind"].size
+        d.constraint_kind = u8p(a["constraint_kind"])
+        d.constraint_nodes = u64p(a["constraint_nodes"])
+        d.constraint_rest = f64p(a["constraint_rest"])
+        d.constraint_compliance = f64p(a["constraint_compliance"])
+        d.membrane_count = a["membrane_triangle"].size
+        d.membrane_triangle = u64p(a["membrane_triangle"])
+        d.membrane_chart = f64p(a["membrane_chart"])
+        d.membrane_material = f64p(a["membrane_material"])
+        d.membrane_role = u8p(a["membrane_role"])
+        d.dihedral_count = a["dihedral_rest_angle"].size
+        d.dihedral_nodes = u64p(a["dihedral_nodes"])
+        d.dihedral_rest_angle = f64p(a["dihedral_rest_angle"])
+        d.dihedral_compliance = f64p(a["dihedral_compliance"])
+        d.sphere_count = a["sphere_centre"].size
+        d.sphere_centre = u64p(a["sphere_centre"])
+        d.sphere_radius = f64p(a["sphere_radius"])
+        d.sphere_compliance = f64p(a["sphere_compliance"])
+        d.sphere_offsets = u64p(a["sphere_offsets"])
+        d.sphere_nodes = u64p(a["sphere_nodes"])
+        if self.suspension is not None:
+            s = self.suspension
+            sa = {k: np.ascontiguousarray(s[k], dtype=dt) for k, dt in _SUSP_KEYS.items()}
+            keep.extend(sa.values())
+            sv = native.SuspensionDefinition()
+            sv.pilot_stable_id = int(s["pilot_stable_id"])
+            sv.pilot_mass = float(s["pilot_mass"])
+            for name in ("pilot_com_local", "pilot_com_world", "pilot_velocity", "pilot_angular_velocity",
+                         "pilot_inertia"):
+                getattr(sv, name)[:] = [float(x) for x in s[name]]
+            sv.pilot_orientation[:] = [float(x) for x in s["pilot_orientation"]]
+            sv.attachment_count = sa["attachment_ids"].size
+            sv.attachment_ids = u64p(sa["attachment_ids"])
+            sv.attachment_nodes = u64p(sa["attachment_nodes"])
+            sv.junction_count = sa["junction_ids"].size
+            sv.junction_ids = u64p(sa["junction_ids"])
+            sv.junction_nodes = u64p(sa["junction_nodes"])
+            sv.harness_count = sa["harness_ids"].size
+            sv.harness_ids = u64p(sa["harness_ids"])
+            sv.harness_local = f64p(sa["harness_local"])
+            sv.segment_count = sa["segment_ids"].size
+            sv.segment_ids = u64p(sa["segment_ids"])
+            sv.segment_from_kind = u8p(sa["segment_from_kind"])
+            sv.segment_from_id = u64p(sa["segment_from_id"])
+            sv.segment_to_kind = u8p(sa["segment_to_kind"])
+            sv.segment_to_id = u64p(sa["segment_to_id"])
+            sv.segment_rest = f64p(sa["segment_rest"])
+            sv.segment_stiffness = f64p(sa["segment_stiffness"])
+            sv.segment_damping = f64p(sa["segment_damping"])
+            sv.segment_role = u32p(sa["segment_role"])
+            sv.solver_iterations = int(s["solver_iterations"])
+            sv.attachment_tolerance = float(s["attachment_tolerance"])
+            sv.minimum_line_length = float(s["minimum_line_length"])
+            sv.maximum_line_residual = float(s["maximum_line_residual"])
+            sv.maximum_control_work = float(s["maximum_control_work"])
+            keep.append(sv)
+            d.suspension = C.pointer(sv)
+        self._keep = keep
+        return d
+
+    @staticmethod
+    def from_ctypes(d: native.Definition) -> "DefinitionArrays":
+        def arr(ptr, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+        n, t, c, m, h, s = (d.node_count, d.triangle_count, d.constraint_count, d.membrane_count,
+                            d.dihedral_count, d.sphere_count)
+        arrays = {
+            "node_position": arr(d.node_position, 3 * n, np.float64),
+            "node_mass": arr(d.node_mass, n, np.float64),
+            "node_fixed": arr(d.node_fixed, n, np.uint8),
+            "triangle_nodes": arr(d.triangle_nodes, 3 * t, np.uint64),
+            "constraint_kind": arr(d.constraint_kind, c, np.uint8),
+            "constraint_nodes": arr(d.constraint_nodes, 2 * c, np.uint64),
+            "constraint_rest": arr(d.constraint_rest, c, np.float64),
+            "constraint_compliance": arr(d.constraint_compliance, c, np.float64),
+            "membrane_triangle": arr(d.membrane_triangle, m, np.uint64),
+            "membrane_chart": arr(d.membrane_chart, 6 * m, np.float64),
+            "membrane_material": arr(d.membrane_material, 8 * m, np.float64),
+            "membrane_role": arr(d.membrane_role, m, np.uint8),
+            "dihedral_nodes": arr(d.dihedral_nodes, 4 * h, np.uint64),
+            "dihedral_rest_angle": arr(d.dihedral_rest_angle, h, np.float64),
+            "dihedral_compliance": arr(d.dihedral_compliance, h, np.float64),
+            "sphere_centre": arr(d.sphere_centre, s, np.uint64),
+            "sphere_radius": arr(d.sphere_radius, s, np.float64),
+            "sphere_compliance": arr(d.sphere_compliance, s, np.float64),
+            "sphere_offsets": arr(d.sphere_offsets, s + 1, np.uint64),
+        }
+        arrays["sphere_nodes"] = arr(d.sphere_nodes, int(arrays["sphere_offsets"][-1]) if s else 0, np.uint64)
+        susp = None
+        if d.suspension:
+            sv = d.suspension.contents
+            susp = {
+                "pilot_stable_id": sv.pilot_stable_id,
+                "pilot_mass": sv.pilot_mass,
+                "pilot_com_local": list(sv.pilot_com_local),
+                "pilot_com_world": list(sv.pilot_com_world),
+                "pilot_velocity": list(sv.pilot_velocity),
+                "pilot_angular_velocity": list(sv.pilot_angular_velocity),
+                "pilot_orientation": list(sv.pilot_orientation),
+                "pilot_inertia": list(sv.pilot_inertia),
+                "attachment_ids": arr(sv.attachment_ids, sv.attachment_count, np.uint64),
+                "attachment_nodes": arr(sv.attachment_nodes, sv.attachment_count, np.uint64),
+                "junction_ids": arr(sv.junction_ids, sv.junction_count, np.uint64),
+                "junction_nodes": arr(sv.junction_nodes, sv.junction_count, np.uint64),
+                "harness_ids": arr(sv.harness_ids, sv.harness_count, np.uint64),
+                "harness_local": arr(sv.harness_local, 3 * sv.harness_count, np.float64),
+                "segment_ids": arr(sv.segment_ids, sv.segment_count, np.uint64),
+                "segment_from_kind": arr(sv.segment_from_kind, sv.segment_count, np.uint8),
+                "segment_from_id": arr(sv.segment_from_id, sv.segment_count, np.uint64),
+                "segment_to_kind": arr(sv.segment_to_kind, sv.segment_count, np.uint8),
+                "segment_to_id": arr(sv.segment_to_id, sv.segment_count, np.uint64),
+                "segment_rest": arr(sv.segment_rest, sv.segment_count, np.float64),
+                "segment_stiffness": arr(sv.segment_stiffness, sv.segment_count, np.float64),
+                "segment_damping": arr(sv.segment_damping, sv.segment_count, np.float64),
+                "segment_role": arr(sv.segment_role, sv.segment_count, np.uint32),
+                "solver_iterations": sv.solver_iterations,
+                "attachment_tolerance": sv.attachment_tolerance,
+                "minimum_line_length": sv.minimum_line_length,
+                "maximum_line_residual": sv.maximum_line_residual,
+                "maximum_control_work": sv.maximum_control_work,
+            }
+        return DefinitionArrays(arrays, susp)
+
+
+def definition_colouring(definition: DefinitionArrays, family: int):
+    """Host-side colour schedule of a family (0 distance, 1 membrane, 2 hinge,
+    3 suspension levels) without a device."""
+    lib = native.library()
+    view = definition.to_ctypes()
+    nc, no = C.c_size_t(), C.c_size_t()
+    check(lib.sw_definition_colouring(C.byref(view), family, None, C.byref(nc), None, C.byref(no)))
+    order = np.zeros(nc.value, np.uint64)
+    offsets = np.zeros(no.value, np.uint64)
+    check(lib.sw_definition_colouring(C.byref(view), family, u64p(order), None, u64p(offsets), None))
+    return order, offsets
+
+
+class Scene:
+    """A seeded synthetic wing (scene-v2 entities held by the library)."""
+
+    def __init__(self, preset: str):
+        lib = native.library()
+        handle = C.c_void_p()
+        check(lib.sw_scene_generate(preset.encode(), C.byref(handle)))
+        self._handle = handle
+        self.preset = preset
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            native.library().sw_scene_destroy(self._handle)
+            self._handle = None
+
+    def face_pressure(self) -> np.ndarray:
+        ptr = C.POINTER(C.c_double)()
+        count = C.c_size_t()
+        check(native.library().sw_scene_face_pressure(self._handle, C.byref(ptr), C.byref(count)))
+        return np.ctypeslib.as_array(ptr, shape=(count.value,)).copy() if count.value else np.zeros(0)
+
+    def assemble(self, settings: native.AssemblySettings | None = None) -> "Assembly":
+        return Assembly(self, settings or native.flight_assembly_settings())
+
+
+def generate_scene(preset: str) -> Scene:
+    return Scene(preset)
+
+
+class Assembly:
+    """assembleSceneStructure output: the definition and its mappings."""
+
+    def __init__(self, scene: Scene, settings: native.AssemblySettings):
+        lib = native.library()
+        handle = C.c_void_p()
+        check(lib.sw_scene_assemble(scene._handle, C.byref(settings), C.byref(handle)))
+        self._handle = handle
+        view = native.Definition()
+        check(lib.sw_assembly_definition(handle, C.byref(view)))
+        self.definition = DefinitionArrays.from_ctypes(view)
+        vn, jn = C.c_size_t(), C.c_size_t()
+        fm, sm = C.c_double(), C.c_double()
+        check(lib.sw_assembly_info(handle, C.byref(vn), C.byref(jn), C.byref(fm), C.byref(sm)))
+        self.vertex_nodes, self.junction_nodes = vn.value, jn.value
+        self.fabric_mass, self.seam_mass = fm.value, sm.value
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            native.library().sw_assembly_destroy(self._handle)
+            self._handle = None
+
+    def flight_loads(self) -> np.ndarray:
+        out = np.zeros(3 * self.definition.node_count)
+        check(native.library().sw_assembly_flight_loads(self._handle, f64p(out)))
+        return out
+
+
+def _diagnostics_dict(d: native.Diagnostics) -> dict:
+    out = {}
+    for name, _ in native.Diagnostics._fields_:
+        value = getattr(d, name)
+        out[name] = list(value) if hasattr(value, "__len__") else value
+    return out
+
+
+class Structure:
+    """simwing::fsi::Structure over the device (structure.h:273-309)."""
+
+    def __init__(self, definition: DefinitionArrays, device: int = 0):
+        lib = native.library()
+        self._definition = definition
+        view = definition.to_ctypes()
+        handle = C.c_void_p()
+        check(lib.sw_structure_create(C.byref(view), device, C.byref(handle)))
+        self._handle = handle
+        self.node_count = definition.node_count
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            native.library().sw_structure_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def definition(self) -> DefinitionArrays:
+        return self._definition
+
+    def definition_fingerprint(self) -> int:
+        value = C.c_uint64()
+        check(native.library().sw_structure_fingerprint(self._handle, C.byref(value)))
+        return value.value
+
+    def accepted_step_count(self) -> int:
+        steps = C.c_uint64()
+        check(native.library().sw_structure_progress(self._handle, C.byref(steps), None))
+        return steps.value
+
+    def simulation_time_seconds(self) -> float:
+        time = C.c_double()
+        check(native.library().sw_structure_progress(self._handle, None, C.byref(time)))
+        return time.value
+
+    def clear_external_forces(self) -> None:
+        check(native.library().sw_structure_clear_external_forces(self._handle))
+
+    def add_external_force(self, node: int, force) -> None:
+        f = np.ascontiguousarray(force, dtype=np.float64)
+        check(native.library().sw_structure_add_external_force(self._handle, node, f64p(f)))
+
+    def set_external_forces(self, forces: np.ndarray) -> None:
+        f = np.ascontiguousarray(forces, dtype=np.float64).reshape(-1)
+        check(native.library().sw_structure_set_external_forces(self._handle, f64p(f), f.size // 3))
+
+    def set_face_pressure(self, pressure: np.ndarray) -> None:
+        p = np.ascontiguousarray(pressure, dtype=np.float64)
+        check(native.library().sw_structure_set_face_pressure(self._handle, f64p(p), p.size))
+
+    def step(self, settings: native.StepSettings | None = None) -> dict:
+        d = native.Diagnostics()
+        s = settings or native.flight_step_settings()
+        check(native.library().sw_structure_step(self._handle, C.byref(s), C.byref(d)))
+        return _diagnostics_dict(d)
+
+    def step_many(self, count: int, settings: native.StepSettings | None = None) -> None:
+        s = settings or native.flight_step_settings()
+        check(native.library().sw_structure_step_many(self._handle, C.byref(s), int(count)))
+
+    def diagnostics(self) -> dict:
+        d = native.Diagnostics()
+        check(native.library().sw_structure_diagnostics(self._handle, C.byref(d)))
+        return _diagnostics_dict(d)
+
+    def node_states(self):
+        n = self.node_count
+        x, p, v = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+        check(native.library().sw_structure_node_states(self._handle, f64p(x), f64p(p), f64p(v)))
+        return x.reshape(n, 3), p.reshape(n, 3), v.reshape(n, 3)
+
+    def payload_state(self) -> np.ndarray:
+        out = np.zeros(13)
+        check(native.library().sw_structure_payload_state(self._handle, f64p(out)))
+        return out
+
+    def solver_state(self) -> dict:
+        a = self._definition.arrays
+        c = a["constraint_kind"].size
+        m = a["membrane_triangle"].size
+        h = a["dihedral_rest_angle"].size
+        s = 0 if self._definition.suspension is None else self._definition.suspension["segment_ids"].size
+        out = {"constraint_lambda": np.zeros(c), "membrane_lambda": np.zeros(3 * m),
+               "membrane_resultant": np.zeros(3 * m), "membrane_residual": np.zeros(m),
+               "dihedral_lambda": np.zeros(h), "segment_lambda": np.zeros(s)}
+        check(native.library().sw_structure_solver_state(
+            self._handle, f64p(out["constraint_lambda"]), f64p(out["membrane_lambda"]),
+            f64p(out["membrane_resultant"]), f64p(out["membrane_residual"]), f64p(out["dihedral_lambda"]),
+            f64p(out["segment_lambda"])))
+        return out
+
+    def checkpoint(self) -> "Checkpoint":
+        handle = C.c_void_p()
+        check(native.library().sw_structure_checkpoint(self._handle, C.byref(handle)))
+        return Checkpoint(handle)
+
+    def restore(self, checkpoint: "Checkpoint") -> None:
+        check(native.library().sw_structure_restore(self._handle, checkpoint._handle))
+
+    def colouring(self, family: int):
+        lib = native.library()
+        nc, no = C.c_size_t(), C.c_size_t()
+        check(lib.sw_structure_colouring(self._handle, family, None, C.byref(nc), None, C.byref(no)))
+        order = np.zeros(nc.value, np.uint64)
+        offsets = np.zeros(no.value, np.uint64)
+        check(lib.sw_structure_colouring(self._handle, family, u64p(order), None, u64p(offsets), None))
+        return order, offsets
+
+    def set_launch(self, workgroups: int, threads: int) -> None:
+        check(native.library().sw_structure_set_launch(self._handle, workgroups, threads))
+
+    def launch(self) -> tuple[int, int]:
+        w, t = C.c_int(), C.c_int()
+        check(native.library().sw_structure_get_launch(self._handle, C.byref(w), C.byref(t)))
+        return w.value, t.value
+
+
+class Checkpoint:
+    """Immutable structure checkpoint (StructureCheckpoint, structure.h:253-271)."""
+
+    def __init__(self, handle):
+        self._handle = handle
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            native.library().sw_checkpoint_release(self._handle)
+            self._handle = None
+
+
+class Batch:
+    """Many uneven wings stepped together in one persistent launch."""
+
+    def __init__(self, definitions: list[DefinitionArrays], device: int = 0):
+        lib = native.library()
+        views = [d.to_ctypes() for d in definitions]
+        self._definitions = definitions
+        self._views = views
+        ptrs = (C.POINTER(native.Definition) * len(views))(*[C.pointer(v) for v in views])
+        handle = C.c_void_p()
+        check(lib.sw_batch_create(ptrs, len(views), device, C.byref(handle)))
+        self._handle = handle
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            native.library().sw_batch_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        self.close()
+
+    def set_face_pressure(self, wing: int, pressure: np.ndarray) -> None:
+        p = np.ascontiguousarray(pressure, dtype=np.float64)
+        check(native.library().sw_batch_set_face_pressure(self._handle, wing, f64p(p), p.size))
+
+    def set_external_forces(self, wing: int, forces: np.ndarray) -> None:
+        f = np.ascontiguousarray(forces, dtype=np.float64).reshape(-1)
+        check(native.library().sw_batch_set_external_forces(self._handle, wing, f64p(f), f.size // 3))
+
+    def step(self, count: int, settings: native.StepSettings | None = None) -> None:
+        s = settings or native.flight_step_settings()
+        check(native.library().sw_batch_step(self._handle, C.byref(s), int(count)))
+
+    def node_states(self, wing: int):
+        n = self._definitions[wing].node_count
+        x, v = np.zeros(3 * n), np.zeros(3 * n)
+        check(native.library().sw_batch_node_states(self._handle, wing, f64p(x), f64p(v)))
+        return x.reshape(n, 3), v.reshape(n, 3)
+
+    def set_launch(self, workgroups_cap: int, threads: int) -> None:
+        check(native.library().sw_batch_set_launch(self._handle, workgroups_cap, threads))

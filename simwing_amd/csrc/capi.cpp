@@ -1,0 +1,654 @@
+// extern "C" boundary of the library (include/simwing_xpbd.h): the
+// Structure adapter semantics (src/fsi/structure.cpp:682-1025) over the
+// device wing, scene generation and assembly, and error translation.
+#include "../../include/simwing_xpbd.h"
+
+#include "assembly.hpp"
+#include "definition.hpp"
+#include "device_wing.hpp"
+#include "scene.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string lastError;
+thread_local int lastPhase = -1;
+thread_local std::string lastEntity;
+
+template <typename F>
+sw_status guarded(F&& body) {
+    try {
+        body();
+        return SW_OK;
+    } catch (const swx::SuspensionFailure& e) {
+        lastError = e.what();
+        lastPhase = e.phase();
+        lastEntity = e.entity();
+        return SW_SUSPENSION_ERROR;
+    } catch (const std::out_of_range& e) {
+        lastError = e.what();
+        return SW_OUT_OF_RANGE;
+    } catch (const std::invalid_argument& e) {
+        lastError = e.what();
+        return SW_INVALID_ARGUMENT;
+    } catch (const std::logic_error& e) {
+        lastError = e.what();
+        return SW_LOGIC_ERROR;
+    } catch (const std::bad_alloc& e) {
+        lastError = e.what();
+        return SW_BAD_ALLOC;
+    } catch (const std::exception& e) {
+        lastError = e.what();
+        return SW_DEVICE_ERROR;
+    } catch (...) {
+        lastError = "unknown failure";
+        return SW_LOGIC_ERROR;
+    }
+}
+
+void requirePointer(const void* p, const char* what) {
+    if (!p) throw std::invalid_argument(std::string(what) + " must not be null");
+}
+
+}  // namespace
+
+struct sw_checkpoint {
+    std::uint64_t fingerprint = 0;
+    std::uint64_t acceptedSteps = 0;
+    double time = 0.0;
+    swx::DeviceWing::State state;
+    std::vector<double> pending;
+    double lastApplied[3] = {0.0, 0.0, 0.0};
+    double maxResidual = 0.0;
+};
+
+struct sw_structure {
+    swx::Definition definition;
+    std::uint64_t fingerprint = 0;
+    std::unique_ptr<swx::DeviceWing> wing;
+    std::vector<double> pending;
+    bool pendingDirty = true;
+    double lastApplied[3] = {0.0, 0.0, 0.0};
+    std::uint64_t acceptedSteps = 0;
+    double time = 0.0;
+    double maxResidual = 0.0;
+    std::vector<std::string> segmentNames;  // runtime order
+    bool hasLoadPath = false;
+};
+
+namespace {
+
+// SoftBody::stepCoupled / step setting checks (soft_body.cpp:981-1050).
+swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st) {
+    const bool coupled = s.definition.suspension.has_value();
+    const auto finite3 = [](const double* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); };
+    if (!(st.time_step > 0.0) || !std::isfinite(st.time_step) || st.substeps <= 0 || st.constraint_iterations < 0 ||
+        st.cable_sweep_pairs < 0 || (coupled && st.constraint_iterations == 0) || st.velocity_damping < 0.0 ||
+        !std::isfinite(st.velocity_damping) || !finite3(st.gravity) || !finite3(st.damping_reference_velocity)) {
+        throw std::invalid_argument(coupled ? "Invalid coupled SoftBody step settings" : "Invalid SoftBody step settings");
+    }
+    if (coupled && st.constraint_iterations != s.definition.suspension->solverIterations)
+        throw std::invalid_argument(
+            "Coupled structural iterations must equal the registered suspension lineIterations contract");
+    if (st.worker_threads == 0)
+        throw std::invalid_argument(
+            "workerThreads 0 selects the serial index-order sweep, which the device path does not run; "
+            "use workerThreads >= 1 (the coloured sweep)");
+    if (s.hasLoadPath && st.cable_sweep_pairs > 0)
+        throw std::invalid_argument("load-path cable sweep pairs are not supported by the device path");
+    swx::StepArgs a{};
+    a.dt = st.time_step / st.substeps;
+    a.invDt = 1.0 / a.dt;
+    a.invDt2 = 1.0 / (a.dt * a.dt);
+    a.damping = std::exp(-st.velocity_damping * a.dt);
+    for (int k = 0; k < 3; ++k) {
+        a.gravity[k] = st.gravity[k];
+        a.reference[k] = st.damping_reference_velocity[k];
+    }
+    a.maxLineResidual = coupled ? s.definition.suspension->maximumLineResidual : 0.0;
+    a.iterations = st.constraint_iterations;
+    return a;
+}
+
+[[noreturn]] void throwDeviceError(sw_structure& s, const swx::DeviceError& e) {
+    if (e.code == SW_SUSPENSION_ERROR) {
+        std::string entity = "payload";
+        if (e.kind == 1 && e.index >= 0 && static_cast<std::size_t>(e.index) < s.segmentNames.size())
+            entity = s.segmentNames[e.index];
+        if (e.kind == 3) entity = "line-residual";
+        const char* what = e.phase == 8 ? "suspension certification failed"
+                           : e.phase == 6 ? "line geometry is non-finite, scale-degenerate or invalid"
+                                          : "payload state became non-finite or degenerate";
+        throw swx::SuspensionFailure(e.phase, entity, what);
+    }
+    if (e.code == SW_INVALID_ARGUMENT)
+        throw std::invalid_argument("SymmetricMatrix3 is singular or ill-conditioned; membrane element slot=" +
+                                    std::to_string(e.index));
+    throw std::logic_error("device step failed with code " + std::to_string(e.code));
+}
+
+void runSteps(sw_structure& s, const sw_step_settings& settings, int count, sw_diagnostics* out);
+
+void fillDiagnostics(sw_structure& s, sw_diagnostics* out) {
+    if (!out) return;
+    const swx::DeviceWing::Reductions r = s.wing->reduce();
+    std::memset(out, 0, sizeof(*out));
+    const swx::Definition& d = s.definition;
+    out->node_count = d.nodeCount();
+    out->triangle_count = d.triangleCount();
+    out->constraint_count = d.constraintCount();
+    out->membrane_count = d.membraneCount();
+    out->dihedral_count = d.dihedralCount();
+    out->dynamic_node_count = static_cast<std::size_t>(r.dynamicNodes);
+    double mass = r.mass;
+    double weighted[3] = {r.weighted[0], r.weighted[1], r.weighted[2]};
+    double momentum[3] = {r.momentum[0], r.momentum[1], r.momentum[2]};
+    double kinetic = r.kinetic;
+    bool finite = r.finite != 0;
+    if (d.suspension) {
+        const swx::DeviceWing::State st = s.wing->download();
+        const double m = d.suspension->pilotMass;
+        const double* com = &st.payload[0];
+        const double* vel = &st.payload[7];
+        const double* w = &st.payload[10];
+        for (int k = 0; k < 3; ++k) {
+            weighted[k] += com[k] * m;
+            momentum[k] += m * vel[k];
+        }
+        mass += m;
+        // 0.5 m |v|^2 + 0.5 w . (R I R^T w)
+        const double qw = st.payload[3], qx = st.payload[4], qy = st.payload[5], qz = st.payload[6];
+        const double R[9] = {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy),
+                             2 * (qx * qy + qw * qz),     1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx),
+                             2 * (qx * qz - qw * qy),     2 * (qy * qz + qw * qx),     1 - 2 * (qx * qx + qy * qy)};
+        double Rtw[3];
+        for (int i = 0; i < 3; ++i) Rtw[i] = R[0 + i] * w[0] + R[3 + i] * w[1] + R[6 + i] * w[2];
+        double rot = 0.0;
+        for (int i = 0; i < 3; ++i) rot += d.suspension->inertia[i] * Rtw[i] * Rtw[i];
+        kinetic += 0.5 * m * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]) + 0.5 * rot;
+        out->suspension_segment_count = d.suspension->segmentIds.size();
+        out->maximum_suspension_residual = s.maxResidual;
+        for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(com[k]) && std::isfinite(vel[k]) && std::isfinite(w[k]);
+    }
+    out->total_dynamic_mass = mass;
+    for (int k = 0; k < 3; ++k) {
+        out->center_of_mass[k] = mass > 0.0 ? weighted[k] * (1.0 / mass) : 0.0;
+        out->linear_momentum[k] = momentum[k];
+        out->last_applied_external_force[k] = s.lastApplied[k];
+    }
+    for (std::size_t i = 0; i < s.pending.size() / 3; ++i)
+        for (int k = 0; k < 3; ++k) out->pending_external_force[k] += s.pending[3 * i + k];
+    out->kinetic_energy = kinetic;
+    out->maximum_distance_error = r.maxDistanceError;
+    out->maximum_cable_extension = r.maxCableExtension;
+    out->maximum_absolute_membrane_strain = r.maxStrain;
+    out->maximum_membrane_residual = r.maxResidual;
+    out->finite = (finite && std::isfinite(kinetic) && std::isfinite(r.maxDistanceError) &&
+                   std::isfinite(r.maxStrain) && std::isfinite(r.maxResidual))
+                      ? 1
+                      : 0;
+}
+
+void runSteps(sw_structure& s, const sw_step_settings& settings, int count, sw_diagnostics* out) {
+    if (count < 1) throw std::invalid_argument("step count must be positive");
+    const swx::StepArgs args = prepare(s, settings);
+    if (s.pendingDirty) {
+        s.wing->setExternalForces(s.pending);
+        s.pendingDirty = false;
+    }
+    s.wing->clearError();
+    for (int step = 0; step < count; ++step) {
+        s.wing->snapshot();
+        for (int sub = 0; sub < settings.substeps; ++sub) s.wing->launchSubstep(args);
+    }
+    double residual = 0.0;
+    const swx::DeviceError e = s.wing->readError(&residual);
+    if (e.code != 0) {
+        s.wing->restoreSnapshot();
+        s.wing->clearError();
+        s.wing->synchronize();
+        throwDeviceError(s, e);
+    }
+    double applied[3] = {0.0, 0.0, 0.0};
+    for (std::size_t i = 0; i < s.pending.size() / 3; ++i)
+        for (int k = 0; k < 3; ++k) applied[k] += s.pending[3 * i + k];
+    for (int k = 0; k < 3; ++k) s.lastApplied[k] = applied[k];
+    std::fill(s.pending.begin(), s.pending.end(), 0.0);
+    s.pendingDirty = true;
+    s.acceptedSteps += static_cast<std::uint64_t>(count);
+    for (int step = 0; step < count; ++step) s.time += settings.time_step;
+    s.maxResidual = residual;
+    fillDiagnostics(s, out);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sw_last_error(void) { return lastError.c_str(); }
+int sw_last_suspension_phase(void) { return lastPhase; }
+const char* sw_last_suspension_entity(void) { return lastEntity.c_str(); }
+
+sw_status sw_structure_create(const sw_definition* definition, int device, sw_structure** out) {
+    return guarded([&] {
+        requirePointer(definition, "definition");
+        requirePointer(out, "out");
+        *out = nullptr;
+        auto s = std::make_unique<sw_structure>();
+        s->definition = swx::Definition::fromView(*definition);
+        swx::validateDefinition(s->definition);
+        s->fingerprint = swx::definitionFingerprint(s->definition);
+        for (std::size_t c = 0; c < s->definition.constraintCount(); ++c) {
+            const auto kind = s->definition.constraintKind[c];
+            if (kind == SW_CONSTRAINT_CABLE || kind == SW_CONSTRAINT_SUSPENSION_TIE) s->hasLoadPath = true;
+        }
+        s->wing = std::make_unique<swx::DeviceWing>(s->definition, device);
+        s->pending.assign(3 * s->definition.nodeCount(), 0.0);
+        if (s->definition.suspension) {
+            const auto& layout = s->wing->suspension();
+            for (std::uint32_t def : layout.segmentOrder)
+                s->segmentNames.push_back("segment:" + std::to_string(s->definition.suspension->segmentIds[def]));
+        }
+        *out = s.release();
+    });
+}
+
+void sw_structure_destroy(sw_structure* structure) { delete structure; }
+
+sw_status sw_structure_fingerprint(const sw_structure* s, uint64_t* fingerprint) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(fingerprint, "fingerprint");
+        *fingerprint = s->fingerprint;
+    });
+}
+
+sw_status sw_structure_progress(const sw_structure* s, uint64_t* accepted, double* time) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        if (accepted) *accepted = s->acceptedSteps;
+        if (time) *time = s->time;
+    });
+}
+
+sw_status sw_structure_clear_external_forces(sw_structure* s) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        std::fill(s->pending.begin(), s->pending.end(), 0.0);
+        s->pendingDirty = true;
+    });
+}
+
+sw_status sw_structure_add_external_force(sw_structure* s, size_t node, const double force[3]) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(force, "force");
+        if (node >= s->definition.nodeCount()) throw std::out_of_range("Structure force node is out of range");
+        if (!std::isfinite(force[0]) || !std::isfinite(force[1]) || !std::isfinite(force[2]))
+            throw std::invalid_argument("Structure force must be finite");
+        for (int k = 0; k < 3; ++k) s->pending[3 * node + k] = s->pending[3 * node + k] + force[k];
+        s->pendingDirty = true;
+    });
+}
+
+sw_status sw_structure_set_external_forces(sw_structure* s, const double* forces, size_t nodeCount) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        if (nodeCount != s->definition.nodeCount())
+            throw std::invalid_argument("Structure force count must equal the node count");
+        requirePointer(forces, "forces");
+        for (std::size_t i = 0; i < 3 * nodeCount; ++i)
+            if (!std::isfinite(forces[i])) throw std::invalid_argument("Structure forces must be finite");
+        std::copy(forces, forces + 3 * nodeCount, s->pending.begin());
+        s->pendingDirty = true;
+    });
+}
+
+sw_status sw_structure_set_face_pressure(sw_structure* s, const double* pressure, size_t count) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        if (count != s->definition.triangleCount()) throw std::out_of_range("Triangle index is out of range");
+        if (count) requirePointer(pressure, "pressure");
+        s->wing->setFacePressure(std::vector<double>(pressure, pressure + count));
+    });
+}
+
+sw_status sw_structure_step(sw_structure* s, const sw_step_settings* settings, sw_diagnostics* diagnostics) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(settings, "settings");
+        runSteps(*s, *settings, 1, diagnostics);
+    });
+}
+
+sw_status sw_structure_step_many(sw_structure* s, const sw_step_settings* settings, int count) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(settings, "settings");
+        // Loads apply to every step of the batch; they are consumed at the end.
+        runSteps(*s, *settings, count, nullptr);
+    });
+}
+
+sw_status sw_structure_diagnostics(sw_structure* s, sw_diagnostics* diagnostics) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(diagnostics, "diagnostics");
+        fillDiagnostics(*s, diagnostics);
+    });
+}
+
+sw_status sw_structure_node_states(const sw_structure* s, double* position, double* previous, double* velocity) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        const swx::DeviceWing::State st = s->wing->download();
+        if (position) std::copy(st.position.begin(), st.position.end(), position);
+        if (previous) std::copy(st.previous.begin(), st.previous.end(), previous);
+        if (velocity) std::copy(st.velocity.begin(), st.velocity.end(), velocity);
+    });
+}
+
+sw_status sw_structure_payload_state(const sw_structure* s, double* payload13) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(payload13, "payload");
+        if (!s->definition.suspension) throw std::invalid_argument("structure has no suspension");
+        const swx::DeviceWing::State st = s->wing->download();
+        std::copy(st.payload.begin(), st.payload.begin() + 13, payload13);
+    });
+}
+
+sw_status sw_structure_solver_state(const sw_structure* s, double* constraintLambda, double* membraneLambda,
+                                    double* membraneResultant, double* membraneResidual, double* dihedralLambda,
+                                    double* segmentLambda) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        const swx::DeviceWing::State st = s->wing->download();
+        if (constraintLambda) std::copy(st.constraintLambda.begin(), st.constraintLambda.end(), constraintLambda);
+        if (membraneLambda) std::copy(st.membraneLambda.begin(), st.membraneLambda.end(), membraneLambda);
+        if (membraneResultant) std::copy(st.membraneResultant.begin(), st.membraneResultant.end(), membraneResultant);
+        if (membraneResidual) std::copy(st.membraneResidual.begin(), st.membraneResidual.end(), membraneResidual);
+        if (dihedralLambda) std::copy(st.dihedralLambda.begin(), st.dihedralLambda.end(), dihedralLambda);
+        if (segmentLambda) std::copy(st.segmentLambda.begin(), st.segmentLambda.end(), segmentLambda);
+    });
+}
+
+sw_status sw_structure_checkpoint(const sw_structure* s, sw_checkpoint** out) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(out, "out");
+        auto c = std::make_unique<sw_checkpoint>();
+        c->fingerprint = s->fingerprint;
+        c->acceptedSteps = s->acceptedSteps;
+        c->time = s->time;
+        c->state = s->wing->download();
+        c->pending = s->pending;
+        for (int k = 0; k < 3; ++k) c->lastApplied[k] = s->lastApplied[k];
+        c->maxResidual = s->maxResidual;
+        *out = c.release();
+    });
+}
+
+sw_status sw_structure_restore(sw_structure* s, const sw_checkpoint* c) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(c, "checkpoint");
+        if (c->fingerprint != s->fingerprint)
+            throw std::invalid_argument("Structure checkpoint belongs to a different definition");
+        if (c->pending.size() != s->pending.size() || c->state.position.size() != s->pending.size() ||
+            !std::isfinite(c->time) || c->time < 0.0)
+            throw std::invalid_argument("Invalid structure checkpoint");
+        for (double v : c->state.position)
+            if (!std::isfinite(v)) throw std::invalid_argument("Structure checkpoint is non-finite");
+        s->wing->upload(c->state);
+        s->pending = c->pending;
+        s->pendingDirty = true;
+        for (int k = 0; k < 3; ++k) s->lastApplied[k] = c->lastApplied[k];
+        s->acceptedSteps = c->acceptedSteps;
+        s->time = c->time;
+        s->maxResidual = c->maxResidual;
+    });
+}
+
+void sw_checkpoint_release(sw_checkpoint* checkpoint) { delete checkpoint; }
+
+sw_status sw_structure_colouring(const sw_structure* s, int family, uint64_t* order, size_t* orderCount,
+                                 uint64_t* offsets, size_t* offsetCount) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        const swx::ColourOrder* c = family == 0   ? &s->wing->distanceColours()
+                                    : family == 1 ? &s->wing->membraneColours()
+                                    : family == 2 ? &s->wing->hingeColours()
+                                                  : nullptr;
+        if (!c) throw std::out_of_range("colouring family must be 0, 1 or 2");
+        if (orderCount) *orderCount = c->order.size();
+        if (offsetCount) *offsetCount = c->offsets.size();
+        if (order) std::copy(c->order.begin(), c->order.end(), order);
+        if (offsets) std::copy(c->offsets.begin(), c->offsets.end(), offsets);
+    });
+}
+
+sw_status sw_definition_colouring(const sw_definition* definition, int family, uint64_t* order, size_t* orderCount,
+                                  uint64_t* offsets, size_t* offsetCount) {
+    return guarded([&] {
+        requirePointer(definition, "definition");
+        const swx::Definition d = swx::Definition::fromView(*definition);
+        swx::validateDefinition(d);
+        swx::ColourOrder c;
+        if (family == 0) c = swx::colourDistance(d);
+        else if (family == 1) c = swx::colourMembranes(d);
+        else if (family == 2) c = swx::colourDihedrals(d);
+        else if (family == 3) {
+            const swx::SuspensionLayout layout = swx::suspensionLayout(d);
+            std::vector<std::vector<std::uint32_t>> writes(layout.from.size()), reads(layout.from.size());
+            std::vector<std::uint8_t> payload(layout.from.size(), 0);
+            for (std::size_t k = 0; k < layout.from.size(); ++k)
+                for (int e : {layout.from[k], layout.to[k]}) {
+                    if (e >= 0) {
+                        writes[k].push_back(static_cast<std::uint32_t>(e));
+                        reads[k].push_back(static_cast<std::uint32_t>(e));
+                    } else {
+                        payload[k] = 1;
+                    }
+                }
+            const auto level = swx::dependencyLevels(writes, reads, payload, d.nodeCount());
+            std::uint32_t levels = 0;
+            for (auto l : level) levels = std::max(levels, l);
+            c.offsets.assign(levels + 1, 0);
+            for (auto l : level) ++c.offsets[l];
+            for (std::uint32_t l = 1; l <= levels; ++l) c.offsets[l] += c.offsets[l - 1];
+            std::vector<std::uint32_t> cursor(c.offsets.begin(), c.offsets.end() - 1);
+            c.order.resize(level.size());
+            for (std::size_t k = 0; k < level.size(); ++k) c.order[cursor[level[k] - 1]++] = layout.segmentOrder[k];
+        } else {
+            throw std::out_of_range("colouring family must be 0, 1, 2 or 3");
+        }
+        if (orderCount) *orderCount = c.order.size();
+        if (offsetCount) *offsetCount = c.offsets.size();
+        if (order) std::copy(c.order.begin(), c.order.end(), order);
+        if (offsets) std::copy(c.offsets.begin(), c.offsets.end(), offsets);
+    });
+}
+
+sw_status sw_device_math(int device, int function, const double* a, const double* b, double* out, size_t count) {
+    return guarded([&] {
+        if (count) {
+            requirePointer(a, "a");
+            requirePointer(b, "b");
+            requirePointer(out, "out");
+        }
+        swx::deviceMath(device, function, a, b, out, count);
+    });
+}
+
+sw_status sw_structure_set_launch(sw_structure* s, int workgroups, int threads) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        s->wing->setLaunch(workgroups, threads);
+    });
+}
+
+sw_status sw_structure_get_launch(const sw_structure* s, int* workgroups, int* threads) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        if (workgroups) *workgroups = s->wing->workgroups();
+        if (threads) *threads = s->wing->threads();
+    });
+}
+
+}  // extern "C"
+
+// ---- scene and assembly -------------------------------------------------
+
+struct sw_scene {
+    swx::GeneratedWing wing;
+};
+
+struct sw_assembly {
+    swx::Assembly assembly;
+    sw_suspension_definition suspensionView{};
+    sw_definition view{};
+    swx::V3 pilotCentre{};
+};
+
+extern "C" {
+
+sw_status sw_scene_generate(const char* preset, sw_scene** out) {
+    return guarded([&] {
+        requirePointer(preset, "preset");
+        requirePointer(out, "out");
+        std::string name(preset);
+        swx::WingParams params;
+        if (name.rfind("batch:", 0) == 0) {
+            unsigned long long seed = 0;
+            int index = 0, count = 0;
+            if (std::sscanf(name.c_str(), "batch:%llu:%d:%d", &seed, &index, &count) != 3 || index < 0 ||
+                index >= count)
+                throw std::invalid_argument("batch preset must be batch:<seed>:<index>:<count>");
+            params = swx::batchPresets(seed, count, 2000.0, 40000.0)[static_cast<std::size_t>(index)];
+        } else {
+            params = swx::wingPreset(name);
+        }
+        auto s = std::make_unique<sw_scene>();
+        s->wing = swx::generateWing(params);
+        *out = s.release();
+    });
+}
+
+void sw_scene_destroy(sw_scene* scene) { delete scene; }
+
+sw_status sw_scene_face_pressure(const sw_scene* scene, const double** pressure, size_t* count) {
+    return guarded([&] {
+        requirePointer(scene, "scene");
+        if (pressure) *pressure = scene->wing.facePressure.data();
+        if (count) *count = scene->wing.facePressure.size();
+    });
+}
+
+sw_status sw_scene_assemble(const sw_scene* scene, const sw_assembly_settings* settings, sw_assembly** out) {
+    return guarded([&] {
+        requirePointer(scene, "scene");
+        requirePointer(out, "out");
+        swx::AssemblySettings s;
+        if (settings) {
+            s.seamCoincidenceTolerance = settings->seam_coincidence_tolerance;
+            s.suspensionIterations = settings->suspension_iterations;
+            s.suspensionAttachmentTolerance = settings->suspension_attachment_tolerance;
+            s.suspensionMinimumLineLength = settings->suspension_minimum_line_length;
+            s.suspensionMaximumLineResidual = settings->suspension_maximum_line_residual;
+            s.suspensionMaximumControlWork = settings->suspension_maximum_control_work;
+            s.contactSphereCompliance = settings->contact_sphere_compliance;
+        }
+        auto a = std::make_unique<sw_assembly>();
+        a->assembly = swx::assembleScene(scene->wing.scene, s, scene->wing.contactSpheres);
+        a->view = a->assembly.definition.view(&a->suspensionView);
+        if (!scene->wing.scene.pilots.empty()) {
+            const auto& p = scene->wing.scene.pilots.front();
+            a->pilotCentre = p.centerOfMass;
+        }
+        *out = a.release();
+    });
+}
+
+void sw_assembly_destroy(sw_assembly* assembly) { delete assembly; }
+
+sw_status sw_assembly_definition(const sw_assembly* assembly, sw_definition* out) {
+    return guarded([&] {
+        requirePointer(assembly, "assembly");
+        requirePointer(out, "out");
+        *out = assembly->view;
+    });
+}
+
+sw_status sw_assembly_info(const sw_assembly* a, size_t* vertexNodes, size_t* junctionNodes, double* fabricMass,
+                           double* seamMass) {
+    return guarded([&] {
+        requirePointer(a, "assembly");
+        if (vertexNodes) *vertexNodes = a->assembly.nodeVertexIds.size();
+        if (junctionNodes) *junctionNodes = a->assembly.nodeJunctionIds.size();
+        if (fabricMass) *fabricMass = a->assembly.totalFabricMass;
+        if (seamMass) *seamMass = a->assembly.totalSeamMass;
+    });
+}
+
+sw_status sw_assembly_flight_loads(const sw_assembly* a, double* forces) {
+    return guarded([&] {
+        requirePointer(a, "assembly");
+        requirePointer(forces, "forces");
+        const swx::Definition& d = a->assembly.definition;
+        double total = 0.0;
+        for (double m : d.nodeMass) total += m;
+        if (d.suspension) total += d.suspension->pilotMass;
+        swx::V3 centre{};
+        if (d.suspension) centre = {d.suspension->comWorld[0], d.suspension->comWorld[1], d.suspension->comWorld[2]};
+        swx::flightLoads(d.nodePosition.data(), d.nodeMass.data(), d.nodeCount(), a->assembly.nodeVertexIds.size(),
+                         total * 9.80665, centre, forces);
+    });
+}
+
+}  // extern "C"
+
+// ---- batched wings ---------------------------------------------------------
+
+struct sw_batch {
+    int placeholder = 0;
+};
+
+extern "C" {
+
+sw_status sw_batch_create(const sw_definition* const* definitions, size_t count, int device, sw_batch** out) {
+    return guarded([&] {
+        (void)definitions;
+        (void)count;
+        (void)device;
+        (void)out;
+        throw std::logic_error("batched stepping is not built yet");
+    });
+}
+void sw_batch_destroy(sw_batch* batch) { delete batch; }
+sw_status sw_batch_set_face_pressure(sw_batch*, size_t, const double*, size_t) {
+    return guarded([&] { throw std::logic_error("batched stepping is not built yet"); });
+}
+sw_status sw_batch_set_external_forces(sw_batch*, size_t, const double*, size_t) {
+    return guarded([&] { throw std::logic_error("batched stepping is not built yet"); });
+}
+sw_status sw_batch_step(sw_batch*, const sw_step_settings*, int) {
+    return guarded([&] { throw std::logic_error("batched stepping is not built yet"); });
+}
+sw_status sw_batch_node_states(const sw_batch*, size_t, double*, double*) {
+    return guarded([&] { throw std::logic_error("batched stepping is not built yet"); });
+}
+sw_status sw_batch_set_launch(sw_batch*, int, int) {
+    return guarded([&] { throw std::logic_error("batched stepping is not built yet"); });
+}
+
+}  // extern "C"

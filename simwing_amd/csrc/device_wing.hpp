@@ -1,0 +1,153 @@
+// Device-resident XPBD wing: SoA tables in HBM, the phase program of one
+// substep, and the launch entry points. Host-only header (no device code).
+#pragma once
+
+#include "definition.hpp"
+#include "schedule.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace swx {
+
+// One entry of the per-iteration phase program. Items of a phase are
+// independent; consecutive phases are separated by a barrier.
+enum PhaseType : int {
+    kPhaseDistance = 0,   // distance-constraint colour [begin, end)
+    kPhaseMembrane = 1,   // membrane colour
+    kPhaseHinge = 2,      // dihedral colour
+    kPhaseSegments = 3,   // one suspension dependency level
+    kPhaseSegmentRun = 4, // consecutive single-item levels run by one lane
+    kPhaseSpheres = 5,    // one contact-sphere dependency level
+    kPhaseStitches = 6,   // one level of the seam-stitch closing pass
+};
+
+struct Phase {
+    int type;
+    int begin;
+    int end;
+    int flags;  // bit 0: the phase holds the iteration's last payload segment
+};
+
+// Rigid payload state and definition on the device.
+struct PayloadDev {
+    double com[3];
+    double q[4];  // w, x, y, z
+    double v[3];
+    double w[3];
+    double prevCom[3];
+    double prevQ[4];
+    double mass;
+    double comLocal[3];
+    double inertia[3];
+    double iinv[9];
+    double maxResidual;
+    double minLineLength;
+};
+
+// First failure of a step: code (SW_* status), suspension phase, the
+// failing item's kind (0 membrane, 1 segment, 2 payload) and index.
+struct DeviceError {
+    int code;
+    int phase;
+    int kind;
+    int index;
+};
+
+struct WingView;  // device-side view (defined in the .hip file)
+
+struct StepArgs {
+    double dt;
+    double invDt;
+    double invDt2;
+    double damping;  // exp(-c * dt), evaluated on the host
+    double gravity[3];
+    double reference[3];
+    double maxLineResidual;
+    int iterations;
+};
+
+class DeviceWing {
+public:
+    DeviceWing(const Definition& definition, int device);
+    ~DeviceWing();
+    DeviceWing(const DeviceWing&) = delete;
+    DeviceWing& operator=(const DeviceWing&) = delete;
+
+    // Upload pending loads and per-triangle pressure (host arrays, 3N / T).
+    void setExternalForces(const std::vector<double>& forces);
+    void setFacePressure(const std::vector<double>& pressure);
+
+    // One substep on the wing's stream (asynchronous).
+    void launchSubstep(const StepArgs& args);
+    // Snapshot of the mutable state (taken only while no error is pending),
+    // and restore from it.
+    void snapshot();
+    void restoreSnapshot();
+    // Clears the device error word.
+    void clearError();
+    DeviceError readError(double* maxResidual);
+    void synchronize();
+
+    // Host copies of the mutable state (definition order).
+    struct State {
+        std::vector<double> position, previous, velocity;
+        std::vector<double> constraintLambda, membraneLambda, membraneResultant, membraneResidual;
+        std::vector<double> dihedralLambda, segmentLambda, sphereLambda, pressure;
+        std::vector<double> payload;  // com, q, v, w, prevCom, prevQ (20)
+    };
+    State download() const;
+    void upload(const State& state);
+
+    // Reductions for StructureDiagnostics (see capi.cpp for the fields).
+    struct Reductions {
+        double mass, weighted[3], momentum[3], kinetic;
+        double maxDistanceError, maxCableExtension, maxStrain, maxResidual;
+        int finite;
+        int dynamicNodes;
+    };
+    Reductions reduce();
+
+    const ColourOrder& distanceColours() const { return distance_; }
+    const ColourOrder& membraneColours() const { return membrane_; }
+    const ColourOrder& hingeColours() const { return hinge_; }
+    const SuspensionLayout& suspension() const { return layout_; }
+    const std::vector<Phase>& iterationPhases() const { return iterPhases_; }
+
+    void setLaunch(int workgroups, int threads);
+    int workgroups() const { return workgroups_; }
+    int threads() const { return threads_; }
+    bool sharedNodes() const { return sharedNodes_; }
+    hipStream_t stream() const { return stream_; }
+
+    // Device buffer base pointers exposed for the batched driver.
+    const void* viewBytes() const;
+
+private:
+    void build(const Definition& d);
+    int device_;
+    hipStream_t stream_ = nullptr;
+    std::vector<void*> allocations_;
+    std::unique_ptr<WingView> view_;
+    ColourOrder distance_, membrane_, hinge_;
+    SuspensionLayout layout_;
+    std::vector<Phase> iterPhases_, closePhases_;
+    std::vector<std::uint32_t> segmentDevicePosition_;  // runtime order -> device slot
+    std::vector<std::uint32_t> sphereDevicePosition_;   // definition pair -> device slot
+    std::size_t nodes_ = 0, triangles_ = 0, constraints_ = 0, membranes_ = 0, hinges_ = 0, segments_ = 0,
+                pairs_ = 0;
+    void* partials_ = nullptr;  // diagnostics partial records
+    int workgroups_ = 1;
+    int threads_ = 512;
+    bool sharedNodes_ = false;
+    std::size_t snapshotBytes_ = 0;
+};
+
+// Device math conformance probe (function codes as sw_device_math).
+void deviceMath(int device, int function, const double* a, const double* b, double* out, std::size_t count);
+
+}  // namespace swx

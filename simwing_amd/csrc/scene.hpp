@@ -171,7 +171,7 @@ struct WingParams {
     int ribEvery = 2;
     double span = 8.0;
     double rootChord = 1.0;
-    double tipChordFloor = 0.1;
+    double tipChordFloor = 0.35;
     double thickness = 0.12;
     int lineRows = 3;                // attachment rows per line rib (A, B, C)
     int lineCount = 48;

@@ -5,8 +5,8 @@
 // and a closed-trailing-edge NACA 4-digit symmetric section. The upper and
 // lower skins are structured grids sharing their leading- and trailing-edge
 // rows (welded across sheets, so no hinge forms there). Every `ribEvery`
-// span stations a rib sheet with its own vertices (upper, camber and lower
-// rows) is sewn to both skins by two seams. Suspension: `lineCount` lines of
+// span stations a rib sheet with its own vertices (upper and lower rows, one
+// triangle strip) is sewn to both skins by two seams. Suspension: `lineCount` lines of
 // `segmentsPerLine` segments from lower-skin attachment points on the line
 // ribs to one of two harness points of a rigid pilot.
 //
@@ -139,11 +139,10 @@ GeneratedWing generateWing(const WingParams& P) {
         zs[i] = std::sqrt(radius * radius - ys[i] * ys[i]) - radius;
         les[i] = 0.25 * (P.rootChord - chords[i]);
     }
+    // Uniform chordwise stations: no clustering, so no sub-millimetre
+    // leading-edge elements with vanishing lumped mass.
     std::vector<double> us(nC);
-    for (int j = 0; j < nC; ++j) {
-        const double t = static_cast<double>(j) / static_cast<double>(nC - 1);
-        us[j] = t * t;
-    }
+    for (int j = 0; j < nC; ++j) us[j] = static_cast<double>(j) / static_cast<double>(nC - 1);
     const auto halfThickness = [&](double chord, double u) {
         const double su = std::sqrt(u);
         return 5.0 * P.thickness * chord *
@@ -186,7 +185,7 @@ GeneratedWing generateWing(const WingParams& P) {
     std::vector<std::vector<StableId>> upper(nS, std::vector<StableId>(nC, 0));
     std::vector<std::vector<StableId>> lower(nS, std::vector<StableId>(nC, 0));
     struct RibVertices {
-        std::vector<StableId> up, mid, low;
+        std::vector<StableId> up, low;
     };
     std::vector<RibVertices> ribs(nRibs);
     for (int i = 0; i < nS; ++i) {
@@ -198,13 +197,11 @@ GeneratedWing generateWing(const WingParams& P) {
         if (r >= 0) {
             RibVertices& rv = ribs[r];
             rv.up.assign(nC, 0);
-            rv.mid.assign(nC, 0);
             rv.low.assign(nC, 0);
             for (int j = 0; j < nC; ++j) rv.up[j] = B.addVertex(surfacePoint(i, j, +1));
             for (int j = 1; j < nC - 1; ++j) rv.low[j] = B.addVertex(surfacePoint(i, j, -1));
-            for (int j = 1; j < nC - 1; ++j) rv.mid[j] = B.addVertex(surfacePoint(i, j, 0));
-            rv.low[0] = rv.mid[0] = rv.up[0];
-            rv.low[nC - 1] = rv.mid[nC - 1] = rv.up[nC - 1];
+            rv.low[0] = rv.up[0];
+            rv.low[nC - 1] = rv.up[nC - 1];
         }
     }
 
@@ -252,24 +249,23 @@ GeneratedWing generateWing(const WingParams& P) {
     const V3 up{0.0, 0.0, 1.0};
     const V3 down{0.0, 0.0, -1.0};
     const V3 right{0.0, 1.0, 0.0};
+    // Rib: one strip between its upper and lower rows (every rib vertex is
+    // sewn to a skin, so no rib node is free to flap out of plane).
     const auto addRib = [&](int r) {
         const RibVertices& rv = ribs[r];
         const StableId sheet = 3 + static_cast<StableId>(r);
         const StableId left = r == 0 ? 1 : static_cast<StableId>(2 + r);
         const StableId rightCell = static_cast<StableId>(3 + r);
-        for (int band = 0; band < 2; ++band) {
-            const std::vector<StableId>& edge = band == 0 ? rv.up : rv.low;
-            for (int j = 0; j < nC - 1; ++j) {
-                const StableId e0 = edge[j], e1 = edge[j + 1];
-                const StableId m0 = rv.mid[j], m1 = rv.mid[j + 1];
-                if (j == 0) {
-                    addTriangle(e0, e1, m1, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
-                } else if (j == nC - 2) {
-                    addTriangle(e0, e1, m0, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
-                } else {
-                    addTriangle(e0, e1, m1, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
-                    addTriangle(e0, m1, m0, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
-                }
+        for (int j = 0; j < nC - 1; ++j) {
+            const StableId u0 = rv.up[j], u1 = rv.up[j + 1];
+            const StableId l0 = rv.low[j], l1 = rv.low[j + 1];
+            if (j == 0) {
+                addTriangle(u0, u1, l1, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
+            } else if (j == nC - 2) {
+                addTriangle(u0, u1, l0, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
+            } else {
+                addTriangle(u0, u1, l1, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
+                addTriangle(u0, l1, l0, sheet, SurfaceRole::Rib, left, rightCell, 0.0, right);
             }
         }
     };
@@ -430,29 +426,28 @@ GeneratedWing generateWing(const WingParams& P) {
         }
     }
 
-    // Contact spheres: centred on the first attachment vertices, holding off
-    // the skin vertices of the neighbouring span stations within 4 radii.
+    // Contact spheres: centred on evenly spaced lower-skin attachment
+    // vertices, holding off the upper-skin vertices within four radii (the
+    // skins must not collapse through the line attachment). The radius is
+    // capped below the closest candidate's rest distance, so every sphere
+    // starts inactive.
     for (int c = 0; c < P.contactSpheres && c < P.lineCount; ++c) {
         ContactSphere sphere;
-        sphere.centreVertex = surfaceAttachments[static_cast<std::size_t>(c) *
-                                                 (P.lineCount / P.contactSpheres)]
-                                  .vertexId;
-        sphere.radius = P.contactRadius;
+        sphere.centreVertex =
+            surfaceAttachments[static_cast<std::size_t>(c) * (P.lineCount / P.contactSpheres)].vertexId;
         const V3 centre = B.positionById[sphere.centreVertex];
+        double closest = 1.0e300;
         for (int i = 0; i < nS; ++i) {
-            for (int j = 0; j < nC; ++j) {
-                const StableId candidates[2] = {upper[i][j], lower[i][j]};
-                for (const StableId v : candidates) {
-                    if (v == sphere.centreVertex) continue;
-                    const double d = lengthv(sub(B.positionById[v], centre));
-                    if (d < 4.0 * P.contactRadius && d > 0.0) {
-                        bool seen = false;
-                        for (const StableId q : sphere.vertices) seen = seen || q == v;
-                        if (!seen) sphere.vertices.push_back(v);
-                    }
+            for (int j = 1; j + 1 < nC; ++j) {
+                const StableId v = upper[i][j];
+                const double d = lengthv(sub(B.positionById[v], centre));
+                if (d < 4.0 * P.contactRadius) {
+                    sphere.vertices.push_back(v);
+                    if (d < closest) closest = d;
                 }
             }
         }
+        sphere.radius = P.contactRadius < 0.8 * closest ? P.contactRadius : 0.8 * closest;
         B.out.contactSpheres.push_back(std::move(sphere));
     }
     return std::move(B.out);
@@ -481,8 +476,8 @@ WingParams wingPreset(const std::string& name) {
     }
     if (name == "export") {
         p.seed = 0x5EED0001ULL;
-        p.spanNodes = 152;
-        p.chordNodes = 76;
+        p.spanNodes = 176;
+        p.chordNodes = 82;
         p.lineRows = 2;
         p.lineCount = 50;
         p.pressureMean = 60.0;
@@ -492,8 +487,8 @@ WingParams wingPreset(const std::string& name) {
     }
     if (name == "variant10k") {
         p.seed = 0x5EED0003ULL;
-        p.spanNodes = 56;
-        p.chordNodes = 32;
+        p.spanNodes = 64;
+        p.chordNodes = 34;
         return p;
     }
     throw std::invalid_argument("unknown wing preset: " + name);
@@ -523,8 +518,8 @@ std::vector<WingParams> batchPresets(std::uint64_t seed, int count, double minNo
         WingParams p;
         p.seed = seed + 0x1000ULL * static_cast<std::uint64_t>(w + 1);
         const double nodes = minNodes * seriesExp(rng.uniform() * logRatio);
-        // nodes ~ 3.5 * span * chord for a 2:1 grid with ribs every 2 stations.
-        int chord = static_cast<int>(std::sqrt(nodes / 7.0) + 0.5);
+        // nodes ~ 3 * span * chord for a 2:1 grid with ribs every 2 stations.
+        int chord = static_cast<int>(std::sqrt(nodes / 6.0) + 0.5);
         if (chord < 16) chord = 16;
         p.chordNodes = chord;
         p.spanNodes = 2 * chord;

@@ -1,0 +1,1632 @@
+// gfx950 kernels of the XPBD structural substep and the device-side wing.
+//
+// One persistent launch runs a whole substep (soft_body.cpp:1175-1460):
+//   A  prev = x, multiplier resets, payload prediction (beginSubstep)
+//   B  pressure gather (ascending incident triangles) + predict
+//   48 x [ distance colours | membrane colours | hinge colours |
+//          suspension levels | contact-sphere levels ]
+//   closing seam-stitch levels
+//   membrane residual diagnostics + velocity finalisation + line residuals
+//   payload velocity reconstruction + line-residual certification
+// Phases are separated by a workgroup barrier (one workgroup) or a
+// sense-reversing grid barrier (several workgroups, all resident).
+// Node positions and inverse masses live in LDS when they fit (one
+// workgroup), else in HBM; every constraint family is SoA in colour-major
+// (or level-major) order, so each phase is one coalesced sweep.
+#include "device_wing.hpp"
+#include "xpbd_math.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+namespace swx {
+
+using namespace dev;
+
+struct WingView {
+    int nodeCount;
+    double *px, *py, *pz;
+    double *qx, *qy, *qz;
+    double *vx, *vy, *vz;
+    double *fx, *fy, *fz;
+    double* w;
+    int *incOffsets, *incTriangles;
+    int *ta, *tb, *tc;
+    double* pressure;
+    int distCount;
+    int *da, *db;
+    double *drest, *dcomp, *dlam;
+    unsigned char* dcable;
+    int membCount;
+    int *ma, *mb, *mc;
+    double *m00, *m01, *m10, *m11, *mscale;
+    double *cxx, *cyy, *czz, *cxy, *cxz, *cyz;
+    double *preW, *preT, *mdamp;
+    double *mlx, *mly, *mlz, *mrx, *mry, *mrz, *mres;
+    int anyPreTension, anyDamping;
+    int hingeCount;
+    int *ha, *hb, *hc, *hd;
+    double *hrest, *hcomp, *hlam;
+    int segCount;
+    int *sfrom, *sto;
+    double *srest, *sstiff, *slam;
+    unsigned char* slastPayload;
+    int *sruntime;  // device slot -> runtime index (error reporting)
+    int hangCount;
+    double* hangLocal;
+    PayloadDev* payload;
+    int pairCount;
+    int *spNode, *spCentre;
+    double *spRadius, *spComp, *splam;
+    int stitchCount;
+    int* stitchItems;
+    Phase* iterPhases;
+    int iterPhaseCount;
+    Phase* closePhases;
+    int closePhaseCount;
+    DeviceError* error;
+    unsigned* barrier;
+    double* state;      // contiguous mutable state block
+    double* snapshot;   // copy of `state`
+    long stateDoubles;
+};
+
+namespace {
+
+constexpr int kMaxThreads = 512;
+
+__device__ __forceinline__ void raise(const WingView& v, int code, int phase, int kind, int index) {
+    if (atomicCAS(&v.error->code, 0, code) == 0) {
+        v.error->phase = phase;
+        v.error->kind = kind;
+        v.error->index = index;
+    }
+}
+
+// Sense-reversing grid barrier over all resident workgroups of the launch.
+__device__ __forceinline__ void gridSync(const WingView& v) {
+    if (gridDim.x == 1) {
+        __syncthreads();
+        return;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned* count = v.barrier;
+        unsigned* generation = v.barrier + 32;
+        const unsigned gen = __hip_atomic_load(generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(generation, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            while (__hip_atomic_load(generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+// Node position/inverse-mass store: LDS copies or the HBM arrays.
+struct NodeRefs {
+    double *x, *y, *z, *w;
+    __device__ __forceinline__ d3 get(int i) const { return d3{x[i], y[i], z[i]}; }
+    __device__ __forceinline__ void put(int i, d3 p) const {
+        x[i] = p.x;
+        y[i] = p.y;
+        z[i] = p.z;
+    }
+};
+
+// Distance / cable / tie / stitch projection (soft_body.cpp:1598-1640).
+__device__ __forceinline__ void projectDistance(const WingView& v, const NodeRefs& N, int k, double invDt2) {
+    const int a = v.da[k], b = v.db[k];
+    d3 pa = N.get(a), pb = N.get(b);
+    const double wa = N.w[a], wb = N.w[b];
+    const d3 diff = sub(pb, pa);
+    const double length = len(diff);
+    if (length <= 1.0e-12) return;
+    const double c = length - v.drest[k];
+    const bool cable = v.dcable[k] != 0;
+    double lambda = v.dlam[k];
+    if (cable && c <= 0.0 && lambda == 0.0) return;
+    const double wsum = wa + wb;
+    const double alpha = v.dcomp[k] * invDt2;
+    if (wsum + alpha <= 0.0) return;
+    const double dl = (-c - alpha * lambda) / (wsum + alpha);
+    double applied = dl;
+    if (cable) {
+        const double next = stdmin(0.0, lambda + dl);
+        applied = next - lambda;
+        lambda = next;
+    } else {
+        lambda += dl;
+    }
+    v.dlam[k] = lambda;
+    const double s = applied / length;
+    const double sa = wa * s, sb = wb * s;
+    pa = sub(pa, scale(diff, sa));
+    pb = add(pb, scale(diff, sb));
+    N.put(a, pa);
+    N.put(b, pb);
+}
+
+// Membrane strain and its gradients (soft_body.cpp:204-282, StrainOnly).
+struct MembraneEval {
+    d3 strain;
+    d3 grad[3][3];  // [component][corner]
+};
+
+__device__ __forceinline__ MembraneEval evaluateMembrane(d3 p0, d3 p1, d3 p2, double m00, double m01, double m10,
+                                                         double m11, double preW, double preT) {
+    MembraneEval e;
+    const d3 e1 = sub(p1, p0);
+    const d3 e2 = sub(p2, p0);
+    const d3 fw = add(scale(e1, m00), scale(e2, m10));
+    const d3 ft = add(scale(e1, m01), scale(e2, m11));
+    e.strain = d3{0.5 * (dot(fw, fw) - 1.0) + preW, 0.5 * (dot(ft, ft) - 1.0) + preT, dot(fw, ft)};
+    const d3 zero{0.0, 0.0, 0.0};
+    const d3 byWarp[3] = {fw, zero, ft};
+    const d3 byWeft[3] = {zero, ft, fw};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const d3 g1 = add(scale(byWarp[k], m00), scale(byWeft[k], m01));
+        const d3 g2 = add(scale(byWarp[k], m10), scale(byWeft[k], m11));
+        e.grad[k][0] = neg(add(g1, g2));
+        e.grad[k][1] = g1;
+        e.grad[k][2] = g2;
+    }
+    return e;
+}
+
+// One membrane element (soft_body.cpp:1822-1929).
+__device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRefs& N, int k, const StepArgs& a) {
+    const int i0 = v.ma[k], i1 = v.mb[k], i2 = v.mc[k];
+    const d3 p[3] = {N.get(i0), N.get(i1), N.get(i2)};
+    const double w[3] = {N.w[i0], N.w[i1], N.w[i2]};
+    const double preW = v.anyPreTension ? v.preW[k] : 0.0;
+    const double preT = v.anyPreTension ? v.preT[k] : 0.0;
+    const MembraneEval e = evaluateMembrane(p[0], p[1], p[2], v.m00[k], v.m01[k], v.m10[k], v.m11[k], preW, preT);
+    const double areaScale = v.mscale[k];
+    d3 J[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int corner = 0; corner < 3; ++corner) J[c][corner] = scale(e.grad[c][corner], areaScale);
+    s3 M{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int corner = 0; corner < 3; ++corner) {
+        const double wc = w[corner];
+        M.xx += wc * dot(J[0][corner], J[0][corner]);
+        M.yy += wc * dot(J[1][corner], J[1][corner]);
+        M.zz += wc * dot(J[2][corner], J[2][corner]);
+        M.xy += wc * dot(J[0][corner], J[1][corner]);
+        M.xz += wc * dot(J[0][corner], J[2][corner]);
+        M.yz += wc * dot(J[1][corner], J[2][corner]);
+    }
+    const s3 at{a.invDt2 * v.cxx[k], a.invDt2 * v.cyy[k], a.invDt2 * v.czz[k],
+                a.invDt2 * v.cxy[k], a.invDt2 * v.cxz[k], a.invDt2 * v.cyz[k]};
+    const d3 lambda{v.mlx[k], v.mly[k], v.mlz[k]};
+    const d3 cv = scale(e.strain, areaScale);
+    const d3 atl = mul(at, lambda);
+    s3 S;
+    d3 rhs;
+    if (v.anyDamping) {
+        const double gamma = v.mdamp[k] / a.dt;
+        d3 dm{0.0, 0.0, 0.0};
+        const int idx[3] = {i0, i1, i2};
+#pragma unroll
+        for (int corner = 0; corner < 3; ++corner) {
+            const d3 disp = sub(p[corner], d3{v.qx[idx[corner]], v.qy[idx[corner]], v.qz[idx[corner]]});
+            dm.x += dot(J[0][corner], disp);
+            dm.y += dot(J[1][corner], disp);
+            dm.z += dot(J[2][corner], disp);
+        }
+        const double f = 1.0 + gamma;
+        S = s3{f * M.xx + at.xx, f * M.yy + at.yy, f * M.zz + at.zz,
+               f * M.xy + at.xy, f * M.xz + at.xz, f * M.yz + at.yz};
+        rhs = sub(sub(neg(cv), atl), scale(dm, gamma));
+    } else {
+        // dampingTime == 0: gamma == 0, (1 + 0) * M == M exactly and the
+        // damping term contributes a signed zero only.
+        S = s3{M.xx + at.xx, M.yy + at.yy, M.zz + at.zz, M.xy + at.xy, M.xz + at.xz, M.yz + at.yz};
+        rhs = sub(neg(cv), atl);
+    }
+    d3 dl;
+    if (!checkedSolve(S, rhs, dl)) {
+        raise(v, SW_INVALID_ARGUMENT, -1, 0, k);
+        return;
+    }
+    d3 corr[3];
+#pragma unroll
+    for (int corner = 0; corner < 3; ++corner) {
+        const d3 c = add(add(scale(J[0][corner], dl.x), scale(J[1][corner], dl.y)), scale(J[2][corner], dl.z));
+        corr[corner] = scale(c, w[corner]);
+    }
+    v.mlx[k] = lambda.x + dl.x;
+    v.mly[k] = lambda.y + dl.y;
+    v.mlz[k] = lambda.z + dl.z;
+    N.put(i0, add(p[0], corr[0]));
+    N.put(i1, add(p[1], corr[1]));
+    N.put(i2, add(p[2], corr[2]));
+}
+
+// Dihedral hinge (soft_body.cpp:1648-1711 and 1754-1792).
+__device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& N, int k, double invDt2) {
+    const int idx[4] = {v.ha[k], v.hb[k], v.hc[k], v.hd[k]};
+    const d3 a = N.get(idx[0]), b = N.get(idx[1]), c = N.get(idx[2]), d = N.get(idx[3]);
+    const d3 edge = sub(b, a);
+    const d3 toC = sub(c, a);
+    const d3 toD = sub(d, a);
+    const d3 area1 = cross(edge, toC);
+    const d3 area2 = cross(toD, edge);
+    const double edgeLength = len(edge);
+    const double l1 = len(area1);
+    const double l2 = len(area2);
+    if (!(edgeLength > 1.0e-12) || !(l1 > 1.0e-12 * edgeLength) || !(l2 > 1.0e-12 * edgeLength)) return;
+    const d3 eu = divide(edge, edgeLength);
+    const d3 n1 = divide(area1, l1);
+    const d3 n2 = divide(area2, l2);
+    const double cosine = stdclamp(dot(n1, n2), -1.0, 1.0);
+    const double sine = dot(eu, cross(n1, n2));
+    const d3 n1Adj = sub(scale(cross(n2, eu), cosine), scale(n2, sine));
+    const d3 n2Adj = sub(scale(cross(eu, n1), cosine), scale(n1, sine));
+    const d3 euAdj = scale(cross(n1, n2), cosine);
+    const d3 a1Adj = divide(sub(n1Adj, scale(n1, dot(n1Adj, n1))), l1);
+    const d3 a2Adj = divide(sub(n2Adj, scale(n2, dot(n2Adj, n2))), l2);
+    d3 edgeAdj = divide(sub(euAdj, scale(eu, dot(euAdj, eu))), edgeLength);
+    edgeAdj = add(edgeAdj, cross(toC, a1Adj));
+    const d3 cAdj = cross(a1Adj, edge);
+    edgeAdj = add(edgeAdj, cross(a2Adj, toD));
+    const d3 dAdj = cross(edge, a2Adj);
+    const double angle = atan2(sine, cosine);
+    const d3 g[4] = {scale(add(add(edgeAdj, cAdj), dAdj), -1.0), edgeAdj, cAdj, dAdj};
+    if (!isfinite(angle) || !fin3(g[0]) || !fin3(g[1]) || !fin3(g[2]) || !fin3(g[3])) return;
+    double wgrad = 0.0;
+    double w[4];
+#pragma unroll
+    for (int corner = 0; corner < 4; ++corner) {
+        w[corner] = N.w[idx[corner]];
+        wgrad += w[corner] * dot(g[corner], g[corner]);
+    }
+    const double alpha = v.hcomp[k] * invDt2;
+    const double den = wgrad + alpha;
+    if (!(den > 0.0) || !isfinite(den)) return;
+    const double value = remainder(angle - v.hrest[k], 6.283185307179586476925286766559);
+    const double lambda = v.hlam[k];
+    const double dl = (-value - alpha * lambda) / den;
+    if (!isfinite(dl)) return;
+    const d3 p[4] = {a, b, c, d};
+#pragma unroll
+    for (int corner = 0; corner < 4; ++corner) N.put(idx[corner], add(p[corner], scale(g[corner], w[corner] * dl)));
+    v.hlam[k] = lambda + dl;
+}
+
+// Payload state held in registers by the lane running a chain of
+// payload segments.
+struct Payload {
+    d3 com;
+    quat q;
+};
+
+struct Endpoint {
+    d3 position;
+    d3 offset;
+    double invMass;
+    bool rigid;
+    bool fixed;
+};
+
+__device__ __forceinline__ bool endpointState(const WingView& v, const NodeRefs& N, int e, const Payload& P,
+                                              const PayloadDev& def, Endpoint& out) {
+    if (e >= 0) {
+        out.position = N.get(e);
+        out.invMass = N.w[e];
+        out.fixed = out.invMass == 0.0;
+        out.rigid = false;
+        out.offset = d3{0.0, 0.0, 0.0};
+        return true;
+    }
+    const int h = -e - 1;
+    const d3 local{v.hangLocal[3 * h], v.hangLocal[3 * h + 1], v.hangLocal[3 * h + 2]};
+    m3 r;
+    if (!rotationMatrix(P.q, r)) return false;
+    const d3 rel = sub(local, d3{def.comLocal[0], def.comLocal[1], def.comLocal[2]});
+    out.position = add(P.com, mulM(r, rel));
+    out.offset = sub(out.position, P.com);
+    out.rigid = true;
+    out.fixed = false;
+    out.invMass = 1.0 / def.mass;
+    return true;
+}
+
+// generalizedLineEffectiveInverseMass (suspension.cpp:1104-1123).
+__device__ __forceinline__ bool effectiveInverseMass(const Endpoint& e, bool rigid, d3 g, const m3& iinv,
+                                                     double& out) {
+    if (!(e.invMass >= 0.0) || !isfinite(e.invMass) || !fin3(g) || !fin3(e.offset)) return false;
+    for (int k = 0; k < 9; ++k)
+        if (!isfinite(iinv.a[k])) return false;
+    double r = e.invMass;
+    if (rigid && e.invMass > 0.0) {
+        const d3 rot = cross(e.offset, g);
+        r += dot(rot, mulM(iinv, rot));
+    }
+    if (!(r >= 0.0) || !isfinite(r)) return false;
+    out = r;
+    return true;
+}
+
+// One suspension segment of solveLineIteration (suspension.cpp:2197-2274)
+// with projectUnilateralLine (suspension.cpp:1125-1163). Returns false on
+// a LineSolve failure (already raised).
+__device__ bool projectSegment(const WingView& v, const NodeRefs& N, int k, Payload& P, const PayloadDev& def,
+                               const m3& iinv, const StepArgs& a) {
+    Endpoint from, to;
+    const int fe = v.sfrom[k], te = v.sto[k];
+    if (!endpointState(v, N, fe, P, def, from) || !endpointState(v, N, te, P, def, to)) {
+        raise(v, SW_SUSPENSION_ERROR, 1, 2, v.sruntime[k]);
+        return false;
+    }
+    const d3 diff = sub(to.position, from.position);
+    const double length = len(diff);
+    const double minLength = def.minLineLength;
+    if (!isfinite(length) || length <= minLength) {
+        raise(v, SW_SUSPENSION_ERROR, 6, 1, v.sruntime[k]);
+        return false;
+    }
+    const d3 dir = divide(diff, length);
+    const d3 toGrad = neg(dir);
+    const double rest = v.srest[k];
+    const double stiffness = v.sstiff[k];
+    const double lambda = v.slam[k];
+    const double gradLength = len(dir);
+    if (!(length > 0.0) || !isfinite(length) || !(rest > 0.0) || !isfinite(rest) || !(stiffness > 0.0) ||
+        !isfinite(stiffness) || !(a.dt > 0.0) || !isfinite(a.dt) || !(lambda >= 0.0) || !isfinite(lambda) ||
+        fabs(gradLength - 1.0) > 1.0e-10) {
+        raise(v, SW_SUSPENSION_ERROR, 6, 1, v.sruntime[k]);
+        return false;
+    }
+    double effFrom, effTo;
+    if (!effectiveInverseMass(from, from.rigid && !from.fixed, dir, iinv, effFrom) ||
+        !effectiveInverseMass(to, to.rigid && !to.fixed, neg(dir), iinv, effTo)) {
+        raise(v, SW_SUSPENSION_ERROR, 6, 1, v.sruntime[k]);
+        return false;
+    }
+    const double eff = effFrom + effTo;
+    const double alphaTilde = 1.0 / (stiffness * a.dt * a.dt);
+    if (!(eff + alphaTilde > 0.0)) {
+        raise(v, SW_SUSPENSION_ERROR, 6, 1, v.sruntime[k]);
+        return false;
+    }
+    const double gap = rest - length;
+    const double candidate = lambda + (-gap - alphaTilde * lambda) / (eff + alphaTilde);
+    const double next = stdmax(0.0, candidate);
+    const double increment = next - lambda;
+    v.slam[k] = next;
+    // Corrections: from endpoint first, then to endpoint.
+    const Endpoint* ends[2] = {&from, &to};
+    const int ids[2] = {fe, te};
+    const d3 grads[2] = {dir, toGrad};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const Endpoint& e = *ends[s];
+        if (e.fixed) continue;
+        const d3 step = scale(grads[s], e.invMass * increment);
+        if (ids[s] >= 0) {
+            const d3 p = N.get(ids[s]);
+            N.put(ids[s], add(p, step));
+        } else {
+            P.com = add(P.com, step);
+            const d3 rotation = scale(mulM(iinv, cross(e.offset, grads[s])), increment);
+            quat inc, rotated;
+            if (!rotationIncrement(rotation, inc) || !canonical(qmul(inc, P.q), rotated)) {
+                raise(v, SW_SUSPENSION_ERROR, 5, 2, 0);
+                return false;
+            }
+            P.q = rotated;
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool refreshInverseInertia(PayloadDev& def, const quat& q) {
+    m3 world, inv;
+    if (!worldInertia(q, def.inertia, world) || !inverseM(world, inv)) return false;
+    for (int k = 0; k < 9; ++k) def.iinv[k] = inv.a[k];
+    return true;
+}
+
+__device__ __forceinline__ Payload loadPayload(const PayloadDev& p) {
+    return Payload{d3{p.com[0], p.com[1], p.com[2]}, quat{p.q[0], p.q[1], p.q[2], p.q[3]}};
+}
+__device__ __forceinline__ void storePayload(PayloadDev& p, const Payload& s) {
+    p.com[0] = s.com.x;
+    p.com[1] = s.com.y;
+    p.com[2] = s.com.z;
+    p.q[0] = s.q.w;
+    p.q[1] = s.q.x;
+    p.q[2] = s.q.y;
+    p.q[3] = s.q.z;
+}
+__device__ __forceinline__ m3 loadInverseInertia(const PayloadDev& p) {
+    m3 r;
+    for (int k = 0; k < 9; ++k) r.a[k] = p.iinv[k];
+    return r;
+}
+
+// Contact sphere pair (build-defined): node held off the centre node to the
+// radius; the centre is not moved by contact.
+__device__ __forceinline__ void projectSphere(const WingView& v, const NodeRefs& N, int k, double invDt2) {
+    const int node = v.spNode[k];
+    const d3 c = N.get(v.spCentre[k]);
+    d3 x = N.get(node);
+    const d3 diff = sub(x, c);
+    const double length = len(diff);
+    if (length <= 1.0e-12) return;
+    const double gap = length - v.spRadius[k];
+    const double lambda = v.splam[k];
+    if (gap >= 0.0 && lambda == 0.0) return;
+    const double wn = N.w[node];
+    const double alpha = v.spComp[k] * invDt2;
+    if (wn + alpha <= 0.0) return;
+    const double dl = (-gap - alpha * lambda) / (wn + alpha);
+    const double next = stdmax(0.0, lambda + dl);
+    const double applied = next - lambda;
+    v.splam[k] = next;
+    const double s = applied / length;
+    x = add(x, scale(diff, wn * s));
+    N.put(node, x);
+}
+
+// Payload prediction of beginSubstep (suspension.cpp:2072-2182 with
+// integratePayloadTorqueFree, suspension.cpp:1075-1102).
+__device__ bool beginPayload(PayloadDev& p, const StepArgs& a) {
+    for (int k = 0; k < 3; ++k) p.prevCom[k] = p.com[k];
+    for (int k = 0; k < 4; ++k) p.prevQ[k] = p.q[k];
+    const d3 force{0.0 + p.mass * a.gravity[0], 0.0 + p.mass * a.gravity[1], 0.0 + p.mass * a.gravity[2]};
+    const d3 moment{0.0, 0.0, 0.0};
+    if (!(a.dt > 0.0) || !isfinite(a.dt) || !fin3(force)) return false;
+    quat q;
+    if (!canonical(quat{p.q[0], p.q[1], p.q[2], p.q[3]}, q)) return false;
+    m3 inertia, inv;
+    if (!worldInertia(q, p.inertia, inertia)) return false;
+    d3 angular = mulM(inertia, d3{p.w[0], p.w[1], p.w[2]});
+    d3 vel = add(d3{p.v[0], p.v[1], p.v[2]}, scale(force, a.dt / p.mass));
+    angular = add(angular, scale(moment, a.dt));
+    if (!inverseM(inertia, inv)) return false;
+    d3 omega = mulM(inv, angular);
+    d3 com = add(d3{p.com[0], p.com[1], p.com[2]}, scale(vel, a.dt));
+    quat inc, rotated;
+    if (!rotationIncrement(scale(omega, a.dt), inc) || !canonical(qmul(inc, q), rotated)) return false;
+    m3 world2, inv2;
+    if (!worldInertia(rotated, p.inertia, world2) || !inverseM(world2, inv2)) return false;
+    omega = mulM(inv2, angular);
+    if (!fin3(com) || !fin3(vel) || !fin3(omega) || !finQ(rotated)) return false;
+    p.com[0] = com.x;
+    p.com[1] = com.y;
+    p.com[2] = com.z;
+    p.q[0] = rotated.w;
+    p.q[1] = rotated.x;
+    p.q[2] = rotated.y;
+    p.q[3] = rotated.z;
+    p.v[0] = vel.x;
+    p.v[1] = vel.y;
+    p.v[2] = vel.z;
+    p.w[0] = omega.x;
+    p.w[1] = omega.y;
+    p.w[2] = omega.z;
+    return true;
+}
+
+// reconstructPayloadVelocity (suspension.cpp:2386-2411) and the payload
+// part of certifySubstep (suspension.cpp:2324-2333).
+__device__ bool finishPayload(PayloadDev& p, const StepArgs& a) {
+    const d3 vel{(p.com[0] - p.prevCom[0]) / a.dt, (p.com[1] - p.prevCom[1]) / a.dt,
+                 (p.com[2] - p.prevCom[2]) / a.dt};
+    const quat q{p.q[0], p.q[1], p.q[2], p.q[3]};
+    quat delta;
+    if (!canonical(qmul(q, quat{p.prevQ[0], -p.prevQ[1], -p.prevQ[2], -p.prevQ[3]}), delta)) return false;
+    const double half = acos(stdclamp(delta.w, -1.0, 1.0));
+    const double sine = sin(half);
+    d3 omega;
+    if (fabs(sine) < 1.0e-12) {
+        omega = scale(d3{delta.x, delta.y, delta.z}, 2.0 / a.dt);
+    } else {
+        const d3 axis{delta.x / sine, delta.y / sine, delta.z / sine};
+        omega = scale(axis, 2.0 * half / a.dt);
+    }
+    p.v[0] = vel.x;
+    p.v[1] = vel.y;
+    p.v[2] = vel.z;
+    p.w[0] = omega.x;
+    p.w[1] = omega.y;
+    p.w[2] = omega.z;
+    const double n2 = qnorm2(q);
+    return fin3(d3{p.com[0], p.com[1], p.com[2]}) && finQ(q) && fin3(vel) && fin3(omega) && !(fabs(n2 - 1.0) > 1.0e-12);
+}
+
+__device__ __forceinline__ void atomicMaxNonNegative(double* address, double value) {
+    // Non-negative doubles order like their bit patterns.
+    atomicMax(reinterpret_cast<unsigned long long*>(address), static_cast<unsigned long long>(__double_as_longlong(value)));
+}
+
+template <bool kShared>
+__global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArgs a) {
+    extern __shared__ double lds[];
+    // A pending failure stops later substeps of a queued batch. Only a
+    // single-workgroup launch may leave early: with several workgroups the
+    // error word can change between their starts, and a workgroup that left
+    // would strand the others at the grid barrier.
+    if (gridDim.x == 1 && __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const int n = v.nodeCount;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gstride = gridDim.x * blockDim.x;
+    NodeRefs N;
+    if (kShared) {
+        N = NodeRefs{lds, lds + n, lds + 2 * n, lds + 3 * n};
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            N.x[i] = v.px[i];
+            N.y[i] = v.py[i];
+            N.z[i] = v.pz[i];
+            N.w[i] = v.w[i];
+        }
+        __syncthreads();
+    } else {
+        N = NodeRefs{v.px, v.py, v.pz, v.w};
+    }
+    PayloadDev& payload = *v.payload;
+
+    // A: prev = x, multiplier resets, payload prediction.
+    for (int i = gtid; i < n; i += gstride) {
+        v.qx[i] = N.x[i];
+        v.qy[i] = N.y[i];
+        v.qz[i] = N.z[i];
+    }
+    for (int k = gtid; k < v.distCount; k += gstride) v.dlam[k] = 0.0;
+    for (int k = gtid; k < v.membCount; k += gstride) {
+        v.mlx[k] = 0.0;
+        v.mly[k] = 0.0;
+        v.mlz[k] = 0.0;
+        v.mrx[k] = 0.0;
+        v.mry[k] = 0.0;
+        v.mrz[k] = 0.0;
+        v.mres[k] = 0.0;
+    }
+    for (int k = gtid; k < v.hingeCount; k += gstride) v.hlam[k] = 0.0;
+    for (int k = gtid; k < v.segCount; k += gstride) v.slam[k] = 0.0;
+    for (int k = gtid; k < v.pairCount; k += gstride) v.splam[k] = 0.0;
+    if (gtid == 0 && v.segCount > 0) {
+        payload.maxResidual = 0.0;
+        if (!beginPayload(payload, a) ||
+            !refreshInverseInertia(payload, quat{payload.q[0], payload.q[1], payload.q[2], payload.q[3]}))
+            raise(v, SW_SUSPENSION_ERROR, 5, 2, 0);
+    }
+    gridSync(v);
+
+    // B: pressure gather from the start-of-substep positions, then predict
+    // (soft_body.cpp:1492-1566).
+    for (int i = gtid; i < n; i += gstride) {
+        d3 force{v.fx[i], v.fy[i], v.fz[i]};
+        for (int s = v.incOffsets[i]; s < v.incOffsets[i + 1]; ++s) {
+            const int t = v.incTriangles[s];
+            const int ia = v.ta[t], ib = v.tb[t], ic = v.tc[t];
+            const d3 pa{v.qx[ia], v.qy[ia], v.qz[ia]};
+            const d3 pb{v.qx[ib], v.qy[ib], v.qz[ib]};
+            const d3 pc{v.qx[ic], v.qy[ic], v.qz[ic]};
+            const d3 areaVector = scale(cross(sub(pb, pa), sub(pc, pa)), 0.5);
+            const double p = v.pressure[t];
+            force = add(force, divide(scale(areaVector, p), 3.0));
+        }
+        const double wi = N.w[i];
+        if (wi == 0.0) {
+            v.vx[i] = 0.0;
+            v.vy[i] = 0.0;
+            v.vz[i] = 0.0;
+            continue;
+        }
+        const d3 acc = add(d3{a.gravity[0], a.gravity[1], a.gravity[2]}, scale(force, wi));
+        const d3 ref{a.reference[0], a.reference[1], a.reference[2]};
+        const d3 vel = add(ref, scale(sub(add(d3{v.vx[i], v.vy[i], v.vz[i]}, scale(acc, a.dt)), ref), a.damping));
+        v.vx[i] = vel.x;
+        v.vy[i] = vel.y;
+        v.vz[i] = vel.z;
+        N.put(i, add(N.get(i), scale(vel, a.dt)));
+    }
+    gridSync(v);
+
+    // Constraint iterations.
+    for (int it = 0; it < a.iterations; ++it) {
+        for (int ph = 0; ph < v.iterPhaseCount; ++ph) {
+            const Phase P = v.iterPhases[ph];
+            switch (P.type) {
+            case kPhaseDistance:
+                for (int k = P.begin + gtid; k < P.end; k += gstride) projectDistance(v, N, k, a.invDt2);
+                break;
+            case kPhaseMembrane:
+                for (int k = P.begin + gtid; k < P.end; k += gstride) projectMembrane(v, N, k, a);
+                break;
+            case kPhaseHinge:
+                for (int k = P.begin + gtid; k < P.end; k += gstride) projectHinge(v, N, k, a.invDt2);
+                break;
+            case kPhaseSegments:
+            case kPhaseSegmentRun: {
+                const bool run = P.type == kPhaseSegmentRun;
+                const int first = run ? (gtid == 0 ? P.begin : P.end) : P.begin + gtid;
+                const int step = run ? 1 : gstride;
+                for (int k = first; k < P.end; k += step) {
+                    Payload state = loadPayload(payload);
+                    const m3 iinv = loadInverseInertia(payload);
+                    const bool touches = v.sfrom[k] < 0 || v.sto[k] < 0;
+                    if (!projectSegment(v, N, k, state, payload, iinv, a)) break;
+                    if (touches) {
+                        storePayload(payload, state);
+                        if (v.slastPayload[k] && !refreshInverseInertia(payload, state.q))
+                            raise(v, SW_SUSPENSION_ERROR, 1, 2, 0);
+                    }
+                }
+                break;
+            }
+            case kPhaseSpheres:
+                for (int k = P.begin + gtid; k < P.end; k += gstride) projectSphere(v, N, k, a.invDt2);
+                break;
+            default:
+                break;
+            }
+            gridSync(v);
+        }
+    }
+    // Closing seam-stitch pass (soft_body.cpp:1389-1405).
+    for (int ph = 0; ph < v.closePhaseCount; ++ph) {
+        const Phase P = v.closePhases[ph];
+        for (int k = P.begin + gtid; k < P.end; k += gstride) projectDistance(v, N, v.stitchItems[k], a.invDt2);
+        gridSync(v);
+    }
+
+    // Membrane solver diagnostics (soft_body.cpp:2366-2404), velocity
+    // finalisation (soft_body.cpp:1568-1585) and line residuals
+    // (suspension.cpp:2334-2349).
+    for (int k = gtid; k < v.membCount; k += gstride) {
+        const int i0 = v.ma[k], i1 = v.mb[k], i2 = v.mc[k];
+        const d3 p[3] = {N.get(i0), N.get(i1), N.get(i2)};
+        const double preW = v.anyPreTension ? v.preW[k] : 0.0;
+        const double preT = v.anyPreTension ? v.preT[k] : 0.0;
+        const MembraneEval e = evaluateMembrane(p[0], p[1], p[2], v.m00[k], v.m01[k], v.m10[k], v.m11[k], preW, preT);
+        const double areaScale = v.mscale[k];
+        const s3 at{a.invDt2 * v.cxx[k], a.invDt2 * v.cyy[k], a.invDt2 * v.czz[k],
+                    a.invDt2 * v.cxy[k], a.invDt2 * v.cxz[k], a.invDt2 * v.cyz[k]};
+        const double gamma = (v.anyDamping ? v.mdamp[k] : 0.0) / a.dt;
+        const int idx[3] = {i0, i1, i2};
+        d3 dm{0.0, 0.0, 0.0};
+        for (int corner = 0; corner < 3; ++corner) {
+            const d3 disp = sub(p[corner], d3{v.qx[idx[corner]], v.qy[idx[corner]], v.qz[idx[corner]]});
+            dm.x += areaScale * dot(e.grad[0][corner], disp);
+            dm.y += areaScale * dot(e.grad[1][corner], disp);
+            dm.z += areaScale * dot(e.grad[2][corner], disp);
+        }
+        const d3 lambda{v.mlx[k], v.mly[k], v.mlz[k]};
+        const d3 residual = add(add(scale(e.strain, areaScale), mul(at, lambda)), scale(dm, gamma));
+        v.mres[k] = len(residual) / areaScale;
+        const double f = a.invDt2 / areaScale;
+        v.mrx[k] = -lambda.x * f;
+        v.mry[k] = -lambda.y * f;
+        v.mrz[k] = -lambda.z * f;
+    }
+    for (int i = gtid; i < n; i += gstride) {
+        const double wi = N.w[i];
+        if (wi == 0.0) {
+            v.vx[i] = 0.0;
+            v.vy[i] = 0.0;
+            v.vz[i] = 0.0;
+        } else {
+            v.vx[i] = (N.x[i] - v.qx[i]) * a.invDt;
+            v.vy[i] = (N.y[i] - v.qy[i]) * a.invDt;
+            v.vz[i] = (N.z[i] - v.qz[i]) * a.invDt;
+        }
+    }
+    if (v.segCount > 0) {
+        double worst = 0.0;
+        const Payload state = loadPayload(payload);
+        for (int k = gtid; k < v.segCount; k += gstride) {
+            Endpoint from, to;
+            if (!endpointState(v, N, v.sfrom[k], state, payload, from) ||
+                !endpointState(v, N, v.sto[k], state, payload, to)) {
+                raise(v, SW_SUSPENSION_ERROR, 8, 2, 0);
+                continue;
+            }
+            const double length = len(sub(to.position, from.position));
+            if (!isfinite(length)) {
+                raise(v, SW_SUSPENSION_ERROR, 8, 1, v.sruntime[k]);
+                continue;
+            }
+            const double stretch = stdmax(0.0, length - v.srest[k]);
+            const double compliant = v.slam[k] / (v.sstiff[k] * a.dt * a.dt);
+            worst = stdmax(worst, fabs(stretch - compliant));
+        }
+        double* scratch = kShared ? lds + 4 * n : lds;
+        scratch[threadIdx.x] = worst;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) scratch[threadIdx.x] = stdmax(scratch[threadIdx.x], scratch[threadIdx.x + s]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomicMaxNonNegative(&payload.maxResidual, scratch[0]);
+    }
+    gridSync(v);
+    if (gtid == 0 && v.segCount > 0) {
+        if (!finishPayload(payload, a)) raise(v, SW_SUSPENSION_ERROR, 8, 2, 0);
+        else if (payload.maxResidual > a.maxLineResidual) raise(v, SW_SUSPENSION_ERROR, 8, 3, 0);
+    }
+    if (kShared) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            v.px[i] = N.x[i];
+            v.py[i] = N.y[i];
+            v.pz[i] = N.z[i];
+        }
+    }
+}
+
+// Copies the mutable state block into the snapshot while no error is
+// pending (so a failing step keeps its own pre-step snapshot).
+__global__ void snapshotKernel(WingView v, int restore) {
+    if (!restore && __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < v.stateDoubles; i += stride) {
+        if (restore) v.state[i] = v.snapshot[i];
+        else v.snapshot[i] = v.state[i];
+    }
+}
+
+// StructureDiagnostics reductions (structure.cpp:833-959): one partial
+// record per workgroup, combined on the host in workgroup order.
+struct Partial {
+    double mass, wx, wy, wz, mx, my, mz, kinetic;
+    double maxDistance, maxCable, maxStrain, maxResidual;
+    int finite, dynamicNodes, pad0, pad1;
+};
+
+__global__ void __launch_bounds__(256) diagnosticsKernel(WingView v, Partial* out) {
+    __shared__ Partial block[256];
+    Partial p{};
+    p.finite = 1;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = tid; i < v.nodeCount; i += stride) {
+        const d3 x{v.px[i], v.py[i], v.pz[i]};
+        const d3 q{v.qx[i], v.qy[i], v.qz[i]};
+        const d3 vel{v.vx[i], v.vy[i], v.vz[i]};
+        const double wi = v.w[i];
+        p.finite &= (fin3(x) && fin3(q) && fin3(vel) && isfinite(wi) && wi >= 0.0) ? 1 : 0;
+        if (wi > 0.0) {
+            const double m = 1.0 / wi;
+            p.dynamicNodes += 1;
+            p.mass += m;
+            p.wx += x.x * m;
+            p.wy += x.y * m;
+            p.wz += x.z * m;
+            p.mx += vel.x * m;
+            p.my += vel.y * m;
+            p.mz += vel.z * m;
+            p.kinetic += 0.5 * m * dot(vel, vel);
+        }
+    }
+    for (int k = tid; k < v.distCount; k += stride) {
+        const d3 a{v.px[v.da[k]], v.py[v.da[k]], v.pz[v.da[k]]};
+        const d3 b{v.px[v.db[k]], v.py[v.db[k]], v.pz[v.db[k]]};
+        const double error = len(sub(b, a)) - v.drest[k];
+        if (v.dcable[k]) p.maxCable = stdmax(p.maxCable, stdmax(0.0, error));
+        else p.maxDistance = stdmax(p.maxDistance, fabs(error));
+    }
+    for (int k = tid; k < v.membCount; k += stride) {
+        const int i0 = v.ma[k], i1 = v.mb[k], i2 = v.mc[k];
+        const MembraneEval e = evaluateMembrane(d3{v.px[i0], v.py[i0], v.pz[i0]}, d3{v.px[i1], v.py[i1], v.pz[i1]},
+                                                d3{v.px[i2], v.py[i2], v.pz[i2]}, v.m00[k], v.m01[k], v.m10[k],
+                                                v.m11[k], v.anyPreTension ? v.preW[k] : 0.0,
+                                                v.anyPreTension ? v.preT[k] : 0.0);
+        p.maxStrain = stdmax(p.maxStrain, stdmax(stdmax(fabs(e.strain.x), fabs(e.strain.y)), fabs(e.strain.z)));
+        p.maxResidual = stdmax(p.maxResidual, fabs(v.mres[k]));
+        p.finite &= (fin3(e.strain) && isfinite(v.mres[k])) ? 1 : 0;
+    }
+    block[threadIdx.x] = p;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Partial r{};
+        r.finite = 1;
+        for (int t = 0; t < static_cast<int>(blockDim.x); ++t) {
+            const Partial& s = block[t];
+            r.mass += s.mass;
+            r.wx += s.wx;
+            r.wy += s.wy;
+            r.wz += s.wz;
+            r.mx += s.mx;
+            r.my += s.my;
+            r.mz += s.mz;
+            r.kinetic += s.kinetic;
+            r.maxDistance = stdmax(r.maxDistance, s.maxDistance);
+            r.maxCable = stdmax(r.maxCable, s.maxCable);
+            r.maxStrain = stdmax(r.maxStrain, s.maxStrain);
+            r.maxResidual = stdmax(r.maxResidual, s.maxResidual);
+            r.finite &= s.finite;
+            r.dynamicNodes += s.dynamicNodes;
+        }
+        out[blockIdx.x] = r;
+    }
+}
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// DeviceWing host side.
+
+namespace {
+
+template <typename T>
+T* deviceCopy(std::vector<void*>& owned, const std::vector<T>& host, std::size_t minimum = 1) {
+    const std::size_t count = std::max(host.size(), minimum);
+    void* p = nullptr;
+    check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    owned.push_back(p);
+    if (!host.empty()) check(hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    else check(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+std::vector<T> permute(const std::vector<T>& values, const std::vector<std::uint32_t>& order, std::size_t width = 1) {
+    std::vector<T> out(order.size() * width);
+    for (std::size_t k = 0; k < order.size(); ++k)
+        for (std::size_t c = 0; c < width; ++c) out[k * width + c] = values[order[k] * width + c];
+    return out;
+}
+
+std::vector<int> column(const std::vector<std::uint64_t>& values, const std::vector<std::uint32_t>& order,
+                        std::size_t width, std::size_t col) {
+    std::vector<int> out(order.size());
+    for (std::size_t k = 0; k < order.size(); ++k) out[k] = static_cast<int>(values[order[k] * width + col]);
+    return out;
+}
+
+std::vector<double> columnD(const std::vector<double>& values, const std::vector<std::uint32_t>& order,
+                            std::size_t width, std::size_t col) {
+    std::vector<double> out(order.size());
+    for (std::size_t k = 0; k < order.size(); ++k) out[k] = values[order[k] * width + col];
+    return out;
+}
+
+}  // namespace
+
+DeviceWing::DeviceWing(const Definition& d, int device) : device_(device), view_(std::make_unique<WingView>()) {
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    try {
+        build(d);
+    } catch (...) {
+        for (void* p : allocations_) (void)hipFree(p);
+        allocations_.clear();
+        (void)hipStreamDestroy(stream_);
+        throw;
+    }
+}
+
+DeviceWing::~DeviceWing() {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void* p : allocations_) (void)hipFree(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void DeviceWing::build(const Definition& d) {
+    WingView& v = *view_;
+    std::memset(&v, 0, sizeof(WingView));
+    nodes_ = d.nodeCount();
+    triangles_ = d.triangleCount();
+    constraints_ = d.constraintCount();
+    membranes_ = d.membraneCount();
+    hinges_ = d.dihedralCount();
+    if (nodes_ > 0x3fffffff || 3 * triangles_ > 0x3fffffff || constraints_ > 0x3fffffff)
+        throw std::invalid_argument("structure exceeds the device index range");
+    for (std::size_t m = 0; m < membranes_; ++m)
+        if (d.membraneMaterial[8 * m + 7] != 1.0)
+            throw std::invalid_argument(
+                "compressionStiffnessRatio other than 1 is not supported by the device membrane sweep");
+    if (d.suspension)
+        for (double damping : d.suspension->segmentDamping)
+            if (damping != 0.0)
+                throw std::invalid_argument("axial line damping is not supported by the device suspension sweep");
+
+    distance_ = colourDistance(d);
+    membrane_ = colourMembranes(d);
+    hinge_ = colourDihedrals(d);
+    layout_ = suspensionLayout(d);
+    segments_ = layout_.segmentOrder.size();
+
+    // --- mutable state block layout ---------------------------------------
+    // positions, prev, velocity (3N each), distance lambda (C), membrane
+    // lambda + resultant (6M) + residual (M), hinge lambda (D), segment
+    // lambda (S), sphere lambda (P), face pressure (T), payload.
+    std::size_t pairCount = d.sphereNodes.size();
+    pairs_ = pairCount;
+    const std::size_t payloadDoubles = sizeof(PayloadDev) / sizeof(double);
+    const std::size_t total = 9 * nodes_ + constraints_ + 7 * membranes_ + hinges_ + segments_ + pairCount +
+                              triangles_ + payloadDoubles;
+    snapshotBytes_ = total * sizeof(double);
+    double* state = nullptr;
+    double* snap = nullptr;
+    check(hipMalloc(&state, snapshotBytes_), "hipMalloc state");
+    allocations_.push_back(state);
+    check(hipMalloc(&snap, snapshotBytes_), "hipMalloc snapshot");
+    allocations_.push_back(snap);
+    check(hipMemset(state, 0, snapshotBytes_), "memset state");
+    v.state = state;
+    v.snapshot = snap;
+    v.stateDoubles = static_cast<long>(total);
+    double* cursor = state;
+    const auto take = [&](std::size_t count) {
+        double* p = cursor;
+        cursor += count;
+        return p;
+    };
+    v.px = take(nodes_);
+    v.py = take(nodes_);
+    v.pz = take(nodes_);
+    v.qx = take(nodes_);
+    v.qy = take(nodes_);
+    v.qz = take(nodes_);
+    v.vx = take(nodes_);
+    v.vy = take(nodes_);
+    v.vz = take(nodes_);
+    v.dlam = take(constraints_);
+    v.mlx = take(membranes_);
+    v.mly = take(membranes_);
+    v.mlz = take(membranes_);
+    v.mrx = take(membranes_);
+    v.mry = take(membranes_);
+    v.mrz = take(membranes_);
+    v.mres = take(membranes_);
+    v.hlam = take(hinges_);
+    v.slam = take(segments_);
+    v.splam = take(pairCount);
+    v.pressure = take(triangles_);
+    v.payload = reinterpret_cast<PayloadDev*>(take(payloadDoubles));
+
+    // Nodes.
+    v.nodeCount = static_cast<int>(nodes_);
+    {
+        std::vector<double> x(nodes_), y(nodes_), z(nodes_), w(nodes_);
+        for (std::size_t i = 0; i < nodes_; ++i) {
+            x[i] = d.nodePosition[3 * i];
+            y[i] = d.nodePosition[3 * i + 1];
+            z[i] = d.nodePosition[3 * i + 2];
+            w[i] = d.nodeFixed[i] ? 0.0 : 1.0 / d.nodeMass[i];
+        }
+        check(hipMemcpy(v.px, x.data(), nodes_ * 8, hipMemcpyHostToDevice), "upload x");
+        check(hipMemcpy(v.py, y.data(), nodes_ * 8, hipMemcpyHostToDevice), "upload y");
+        check(hipMemcpy(v.pz, z.data(), nodes_ * 8, hipMemcpyHostToDevice), "upload z");
+        check(hipMemcpy(v.qx, x.data(), nodes_ * 8, hipMemcpyHostToDevice), "upload qx");
+        check(hipMemcpy(v.qy, y.data(), nodes_ * 8, hipMemcpyHostToDevice), "upload qy");
+        check(hipMemcpy(v.qz, z.data(), nodes_ * 8, hipMemcpyHostToDevice), "upload qz");
+        v.w = deviceCopy(allocations_, w);
+        std::vector<double> zeros(nodes_, 0.0);
+        v.fx = deviceCopy(allocations_, zeros);
+        v.fy = deviceCopy(allocations_, zeros);
+        v.fz = deviceCopy(allocations_, zeros);
+    }
+    // Triangles and the node -> incident triangle CSR in ascending order.
+    {
+        std::vector<int> ta(triangles_), tb(triangles_), tc(triangles_);
+        std::vector<int> offsets(nodes_ + 1, 0);
+        for (std::size_t t = 0; t < triangles_; ++t) {
+            ta[t] = static_cast<int>(d.triangleNodes[3 * t]);
+            tb[t] = static_cast<int>(d.triangleNodes[3 * t + 1]);
+            tc[t] = static_cast<int>(d.triangleNodes[3 * t + 2]);
+            ++offsets[ta[t] + 1];
+            ++offsets[tb[t] + 1];
+            ++offsets[tc[t] + 1];
+        }
+        for (std::size_t i = 0; i < nodes_; ++i) offsets[i + 1] += offsets[i];
+        std::vector<int> list(3 * triangles_);
+        std::vector<int> fill(offsets.begin(), offsets.end() - 1);
+        for (std::size_t t = 0; t < triangles_; ++t) {
+            list[fill[ta[t]]++] = static_cast<int>(t);
+            list[fill[tb[t]]++] = static_cast<int>(t);
+            list[fill[tc[t]]++] = static_cast<int>(t);
+        }
+        v.ta = deviceCopy(allocations_, ta);
+        v.tb = deviceCopy(allocations_, tb);
+        v.tc = deviceCopy(allocations_, tc);
+        v.incOffsets = deviceCopy(allocations_, offsets);
+        v.incTriangles = deviceCopy(allocations_, list);
+    }
+    // Distance constraints, colour-major.
+    {
+        const auto& o = distance_.order;
+        v.distCount = static_cast<int>(o.size());
+        v.da = deviceCopy(allocations_, column(d.constraintNodes, o, 2, 0));
+        v.db = deviceCopy(allocations_, column(d.constraintNodes, o, 2, 1));
+        v.drest = deviceCopy(allocations_, permute(d.constraintRest, o));
+        v.dcomp = deviceCopy(allocations_, permute(d.constraintCompliance, o));
+        std::vector<unsigned char> cable(o.size());
+        for (std::size_t k = 0; k < o.size(); ++k) cable[k] = d.constraintKind[o[k]] == SW_CONSTRAINT_CABLE ? 1 : 0;
+        v.dcable = deviceCopy(allocations_, cable);
+    }
+    // Membranes, colour-major, with the element construction of
+    // addMembraneElements (soft_body.cpp:663-690): reference area, inverse
+    // reference matrix (membrane.cpp:131-151) and compliance matrix
+    // (membrane.cpp:268-271).
+    {
+        const auto& o = membrane_.order;
+        v.membCount = static_cast<int>(o.size());
+        std::vector<int> a(o.size()), b(o.size()), c(o.size());
+        std::vector<double> m00(o.size()), m01(o.size()), m10(o.size()), m11(o.size()), sc(o.size());
+        std::vector<double> cxx(o.size()), cyy(o.size()), czz(o.size()), cxy(o.size()), cxz(o.size()), cyz(o.size());
+        std::vector<double> pw(o.size()), pt(o.size()), damp(o.size());
+        int anyPre = 0, anyDamp = 0;
+        for (std::size_t k = 0; k < o.size(); ++k) {
+            const std::size_t m = o[k];
+            const std::uint64_t t = d.membraneTriangle[m];
+            a[k] = static_cast<int>(d.triangleNodes[3 * t]);
+            b[k] = static_cast<int>(d.triangleNodes[3 * t + 1]);
+            c[k] = static_cast<int>(d.triangleNodes[3 * t + 2]);
+            const double* chart = &d.membraneChart[6 * m];
+            const double e1x = chart[2] - chart[0], e1y = chart[3] - chart[1];
+            const double e2x = chart[4] - chart[0], e2y = chart[5] - chart[1];
+            // Matrix2{e1.x, e2.x, e1.y, e2.y}
+            const double r00 = e1x, r01 = e2x, r10 = e1y, r11 = e2y;
+            const double det = r00 * r11 - r01 * r10;
+            const double referenceArea = 0.5 * det;
+            const double scaleValue = std::max({std::abs(r00), std::abs(r01), std::abs(r10), std::abs(r11)});
+            const double tol = 64.0 * 2.220446049250313080847263336181640625e-16 * scaleValue * scaleValue;
+            if (!(scaleValue > 0.0) || !std::isfinite(det) || std::abs(det) <= tol)
+                throw std::invalid_argument("Matrix2 is singular or ill-conditioned");
+            const double inv = 1.0 / det;
+            m00[k] = r11 * inv;
+            m01[k] = -r01 * inv;
+            m10[k] = -r10 * inv;
+            m11[k] = r00 * inv;
+            sc[k] = std::sqrt(referenceArea);
+            const double* mat = &d.membraneMaterial[8 * m];
+            // stiffness {xx warp, yy weft, zz shear, xy coupling, xz 0, yz 0}
+            const double xx = mat[0], yy = mat[1], zz = mat[3], xy = mat[2], xz = 0.0, yz = 0.0;
+            const double sMax = std::max({std::abs(xx), std::abs(yy), std::abs(zz), std::abs(xy), std::abs(xz), std::abs(yz)});
+            const double sDet = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
+            const double sTol = 128.0 * 2.220446049250313080847263336181640625e-16 * sMax * sMax * sMax;
+            if (!(sMax > 0.0) || !std::isfinite(sDet) || std::abs(sDet) <= sTol)
+                throw std::invalid_argument("SymmetricMatrix3 is singular or ill-conditioned");
+            const double r = 1.0 / sDet;
+            cxx[k] = (yy * zz - yz * yz) * r;
+            cyy[k] = (xx * zz - xz * xz) * r;
+            czz[k] = (xx * yy - xy * xy) * r;
+            cxy[k] = (xz * yz - xy * zz) * r;
+            cxz[k] = (xy * yz - xz * yy) * r;
+            cyz[k] = (xy * xz - xx * yz) * r;
+            pw[k] = mat[4];
+            pt[k] = mat[5];
+            damp[k] = mat[6];
+            anyPre |= (mat[4] != 0.0 || mat[5] != 0.0) ? 1 : 0;
+            anyDamp |= mat[6] != 0.0 ? 1 : 0;
+        }
+        v.ma = deviceCopy(allocations_, a);
+        v.mb = deviceCopy(allocations_, b);
+        v.mc = deviceCopy(allocations_, c);
+        v.m00 = deviceCopy(allocations_, m00);
+        v.m01 = deviceCopy(allocations_, m01);
+        v.m10 = deviceCopy(allocations_, m10);
+        v.m11 = deviceCopy(allocations_, m11);
+        v.mscale = deviceCopy(allocations_, sc);
+        v.cxx = deviceCopy(allocations_, cxx);
+        v.cyy = deviceCopy(allocations_, cyy);
+        v.czz = deviceCopy(allocations_, czz);
+        v.cxy = deviceCopy(allocations_, cxy);
+        v.cxz = deviceCopy(allocations_, cxz);
+        v.cyz = deviceCopy(allocations_, cyz);
+        v.preW = deviceCopy(allocations_, pw);
+        v.preT = deviceCopy(allocations_, pt);
+        v.mdamp = deviceCopy(allocations_, damp);
+        v.anyPreTension = anyPre;
+        v.anyDamping = anyDamp;
+    }
+    // Hinges, colour-major.
+    {
+        const auto& o = hinge_.order;
+        v.hingeCount = static_cast<int>(o.size());
+        v.ha = deviceCopy(allocations_, column(d.dihedralNodes, o, 4, 0));
+        v.hb = deviceCopy(allocations_, column(d.dihedralNodes, o, 4, 1));
+        v.hc = deviceCopy(allocations_, column(d.dihedralNodes, o, 4, 2));
+        v.hd = deviceCopy(allocations_, column(d.dihedralNodes, o, 4, 3));
+        v.hrest = deviceCopy(allocations_, permute(d.dihedralRestAngle, o));
+        v.hcomp = deviceCopy(allocations_, permute(d.dihedralCompliance, o));
+    }
+    // Phase program.
+    iterPhases_.clear();
+    for (std::size_t c = 0; c + 1 < distance_.offsets.size(); ++c)
+        iterPhases_.push_back({kPhaseDistance, static_cast<int>(distance_.offsets[c]),
+                               static_cast<int>(distance_.offsets[c + 1]), 0});
+    for (std::size_t c = 0; c + 1 < membrane_.offsets.size(); ++c)
+        iterPhases_.push_back({kPhaseMembrane, static_cast<int>(membrane_.offsets[c]),
+                               static_cast<int>(membrane_.offsets[c + 1]), 0});
+    for (std::size_t c = 0; c + 1 < hinge_.offsets.size(); ++c)
+        iterPhases_.push_back({kPhaseHinge, static_cast<int>(hinge_.offsets[c]), static_cast<int>(hinge_.offsets[c + 1]), 0});
+
+    // Suspension: dependency levels over the runtime (string-id) order.
+    {
+        std::vector<std::vector<std::uint32_t>> writes(segments_), reads(segments_);
+        std::vector<std::uint8_t> touchesPayload(segments_, 0);
+        for (std::size_t k = 0; k < segments_; ++k) {
+            for (int e : {layout_.from[k], layout_.to[k]}) {
+                if (e >= 0) {
+                    writes[k].push_back(static_cast<std::uint32_t>(e));
+                    reads[k].push_back(static_cast<std::uint32_t>(e));
+                } else {
+                    touchesPayload[k] = 1;
+                }
+            }
+        }
+        const auto level = dependencyLevels(writes, reads, touchesPayload, nodes_);
+        std::vector<std::uint32_t> slots(segments_);
+        std::iota(slots.begin(), slots.end(), 0u);
+        std::stable_sort(slots.begin(), slots.end(),
+                         [&](std::uint32_t x, std::uint32_t y) { return level[x] < level[y]; });
+        segmentDevicePosition_.assign(segments_, 0);
+        std::vector<int> from(segments_), to(segments_), runtime(segments_);
+        std::vector<double> rest(segments_), stiff(segments_);
+        std::vector<unsigned char> last(segments_, 0);
+        int lastPayloadSlot = -1;
+        for (std::size_t s = 0; s < segments_; ++s) {
+            const std::uint32_t k = slots[s];
+            segmentDevicePosition_[k] = static_cast<std::uint32_t>(s);
+            from[s] = layout_.from[k];
+            to[s] = layout_.to[k];
+            runtime[s] = static_cast<int>(k);
+            const std::uint32_t def = layout_.segmentOrder[k];
+            rest[s] = d.suspension->segmentRest[def];
+            stiff[s] = d.suspension->segmentStiffness[def];
+            if (touchesPayload[k]) lastPayloadSlot = static_cast<int>(s);
+        }
+        if (lastPayloadSlot >= 0) last[lastPayloadSlot] = 1;
+        v.segCount = static_cast<int>(segments_);
+        v.sfrom = deviceCopy(allocations_, from);
+        v.sto = deviceCopy(allocations_, to);
+        v.sruntime = deviceCopy(allocations_, runtime);
+        v.srest = deviceCopy(allocations_, rest);
+        v.sstiff = deviceCopy(allocations_, stiff);
+        v.slastPayload = deviceCopy(allocations_, last);
+        // Levels -> phases; consecutive single-item levels merge into one
+        // lane-serial run.
+        std::size_t s = 0;
+        while (s < segments_) {
+            std::size_t e = s;
+            while (e < segments_ && level[slots[e]] == level[slots[s]]) ++e;
+            if (e - s == 1) {
+                std::size_t r = e;
+                while (r < segments_) {
+                    std::size_t r2 = r;
+                    while (r2 < segments_ && level[slots[r2]] == level[slots[r]]) ++r2;
+                    if (r2 - r != 1) break;
+                    r = r2;
+                }
+                iterPhases_.push_back({r - s > 1 ? kPhaseSegmentRun : kPhaseSegments, static_cast<int>(s),
+                                       static_cast<int>(r), 0});
+                s = r;
+            } else {
+                iterPhases_.push_back({kPhaseSegments, static_cast<int>(s), static_cast<int>(e), 0});
+                s = e;
+            }
+        }
+        std::vector<double> hang(3 * layout_.hangOrder.size());
+        for (std::size_t r = 0; r < layout_.hangOrder.size(); ++r)
+            for (int c = 0; c < 3; ++c) hang[3 * r + c] = d.suspension->harnessLocal[3 * layout_.hangOrder[r] + c];
+        v.hangCount = static_cast<int>(layout_.hangOrder.size());
+        v.hangLocal = deviceCopy(allocations_, hang);
+    }
+    // Contact-sphere pairs: dependency levels over definition order
+    // (writes: the held-off node; reads: it and the centre).
+    {
+        std::vector<std::vector<std::uint32_t>> writes, reads;
+        std::vector<int> node, centre;
+        std::vector<double> radius, comp;
+        for (std::size_t s = 0; s < d.sphereCount(); ++s)
+            for (std::uint64_t j = d.sphereOffsets[s]; j < d.sphereOffsets[s + 1]; ++j) {
+                node.push_back(static_cast<int>(d.sphereNodes[j]));
+                centre.push_back(static_cast<int>(d.sphereCentre[s]));
+                radius.push_back(d.sphereRadius[s]);
+                comp.push_back(d.sphereCompliance[s]);
+                writes.push_back({static_cast<std::uint32_t>(node.back())});
+                reads.push_back({static_cast<std::uint32_t>(node.back()), static_cast<std::uint32_t>(centre.back())});
+            }
+        const auto level = dependencyLevels(writes, reads, std::vector<std::uint8_t>(writes.size(), 0), nodes_);
+        std::vector<std::uint32_t> slots(writes.size());
+        std::iota(slots.begin(), slots.end(), 0u);
+        std::stable_sort(slots.begin(), slots.end(),
+                         [&](std::uint32_t x, std::uint32_t y) { return level[x] < level[y]; });
+        sphereDevicePosition_.assign(slots.size(), 0);
+        for (std::size_t s = 0; s < slots.size(); ++s) sphereDevicePosition_[slots[s]] = static_cast<std::uint32_t>(s);
+        v.pairCount = static_cast<int>(slots.size());
+        v.spNode = deviceCopy(allocations_, permute(node, slots));
+        v.spCentre = deviceCopy(allocations_, permute(centre, slots));
+        v.spRadius = deviceCopy(allocations_, permute(radius, slots));
+        v.spComp = deviceCopy(allocations_, permute(comp, slots));
+        std::size_t s = 0;
+        while (s < slots.size()) {
+            std::size_t e = s;
+            while (e < slots.size() && level[slots[e]] == level[slots[s]]) ++e;
+            iterPhases_.push_back({kPhaseSpheres, static_cast<int>(s), static_cast<int>(e), 0});
+            s = e;
+        }
+    }
+    // Closing stitch pass: stitches in constraint order, as dependency
+    // levels, addressed by their colour-major slot.
+    {
+        std::vector<std::uint32_t> slotOf(constraints_);
+        for (std::size_t k = 0; k < distance_.order.size(); ++k) slotOf[distance_.order[k]] = static_cast<std::uint32_t>(k);
+        std::vector<std::vector<std::uint32_t>> writes;
+        std::vector<int> items;
+        for (std::size_t c = 0; c < constraints_; ++c) {
+            if (d.constraintKind[c] != SW_CONSTRAINT_SEAM_STITCH) continue;
+            writes.push_back({static_cast<std::uint32_t>(d.constraintNodes[2 * c]),
+                              static_cast<std::uint32_t>(d.constraintNodes[2 * c + 1])});
+            items.push_back(static_cast<int>(slotOf[c]));
+        }
+        const auto level = dependencyLevels(writes, writes, std::vector<std::uint8_t>(writes.size(), 0), nodes_);
+        std::vector<std::uint32_t> slots(items.size());
+        std::iota(slots.begin(), slots.end(), 0u);
+        std::stable_sort(slots.begin(), slots.end(),
+                         [&](std::uint32_t x, std::uint32_t y) { return level[x] < level[y]; });
+        v.stitchCount = static_cast<int>(items.size());
+        v.stitchItems = deviceCopy(allocations_, permute(items, slots));
+        closePhases_.clear();
+        std::size_t s = 0;
+        while (s < slots.size()) {
+            std::size_t e = s;
+            while (e < slots.size() && level[slots[e]] == level[slots[s]]) ++e;
+            closePhases_.push_back({kPhaseStitches, static_cast<int>(s), static_cast<int>(e), 0});
+            s = e;
+        }
+    }
+    v.iterPhases = deviceCopy(allocations_, iterPhases_);
+    v.iterPhaseCount = static_cast<int>(iterPhases_.size());
+    v.closePhases = deviceCopy(allocations_, closePhases_);
+    v.closePhaseCount = static_cast<int>(closePhases_.size());
+
+    // Payload.
+    {
+        PayloadDev p{};
+        if (d.suspension) {
+            const SuspensionDef& s = *d.suspension;
+            for (int k = 0; k < 3; ++k) {
+                p.com[k] = s.comWorld[k];
+                p.v[k] = s.velocity[k];
+                p.w[k] = s.angularVelocity[k];
+                p.comLocal[k] = s.comLocal[k];
+                p.inertia[k] = s.inertia[k];
+            }
+            // validateAndNormalizeSuspensionDefinition normalises the
+            // initial orientation (suspension.cpp:766-767).
+            const double n2 = s.orientation[0] * s.orientation[0] + s.orientation[1] * s.orientation[1] +
+                              s.orientation[2] * s.orientation[2] + s.orientation[3] * s.orientation[3];
+            if (!(n2 > 1.0e-28)) throw SuspensionFailure(kPhaseValidation, "orientation", "quaternion is near zero");
+            const double inv = 1.0 / std::sqrt(n2);
+            double q[4] = {s.orientation[0] * inv, s.orientation[1] * inv, s.orientation[2] * inv, s.orientation[3] * inv};
+            const bool flip = q[0] < 0.0 ||
+                              (q[0] == 0.0 && (q[1] < 0.0 || (q[1] == 0.0 && (q[2] < 0.0 || (q[2] == 0.0 && q[3] < 0.0)))));
+            for (int k = 0; k < 4; ++k) p.q[k] = flip ? -q[k] : q[k];
+            for (int k = 0; k < 3; ++k) p.prevCom[k] = p.com[k];
+            for (int k = 0; k < 4; ++k) p.prevQ[k] = p.q[k];
+            p.mass = s.pilotMass;
+            p.minLineLength = s.minimumLineLength;
+        }
+        check(hipMemcpy(v.payload, &p, sizeof(PayloadDev), hipMemcpyHostToDevice), "upload payload");
+    }
+    {
+        DeviceError e{};
+        std::vector<DeviceError> errors(1, e);
+        v.error = deviceCopy(allocations_, errors);
+        std::vector<unsigned> barrier(64, 0);
+        v.barrier = deviceCopy(allocations_, barrier);
+    }
+    // Launch shape: one workgroup with LDS-resident nodes when they fit.
+    int maxShared = 0;
+    check(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device_), "attribute");
+    const std::size_t ldsNeed = (4 * nodes_ + kMaxThreads) * sizeof(double);
+    sharedNodes_ = ldsNeed <= static_cast<std::size_t>(maxShared);
+    workgroups_ = 1;
+    threads_ = 512;
+    if (sharedNodes_) {
+        check(hipFuncSetAttribute(reinterpret_cast<const void*>(&substepKernel<true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ldsNeed)),
+              "shared memory attribute");
+    }
+    check(hipMalloc(&partials_, 64 * sizeof(Partial)), "diagnostics partials");
+    allocations_.push_back(partials_);
+    check(hipStreamSynchronize(stream_), "sync build");
+    check(hipDeviceSynchronize(), "sync build");
+}
+
+void DeviceWing::setLaunch(int workgroups, int threads) {
+    if (workgroups < 1 || threads < 64 || threads > kMaxThreads || (threads & (threads - 1)) != 0)
+        throw std::invalid_argument("launch shape must be >= 1 workgroup of 64..1024 (power of two) threads");
+    int cus = 0;
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
+    if (workgroups > cus) throw std::invalid_argument("launch shape exceeds one workgroup per compute unit");
+    workgroups_ = workgroups;
+    threads_ = threads;
+    int maxShared = 0;
+    check(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device_), "attribute");
+    const std::size_t ldsNeed = (4 * nodes_ + kMaxThreads) * sizeof(double);
+    sharedNodes_ = workgroups == 1 && ldsNeed <= static_cast<std::size_t>(maxShared);
+    if (sharedNodes_) {
+        check(hipFuncSetAttribute(reinterpret_cast<const void*>(&substepKernel<true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ldsNeed)),
+              "shared memory attribute");
+    }
+}
+
+void DeviceWing::setExternalForces(const std::vector<double>& forces) {
+    WingView& v = *view_;
+    std::vector<double> x(nodes_), y(nodes_), z(nodes_);
+    for (std::size_t i = 0; i < nodes_; ++i) {
+        x[i] = forces[3 * i];
+        y[i] = forces[3 * i + 1];
+        z[i] = forces[3 * i + 2];
+    }
+    check(hipMemcpyAsync(v.fx, x.data(), nodes_ * 8, hipMemcpyHostToDevice, stream_), "forces");
+    check(hipMemcpyAsync(v.fy, y.data(), nodes_ * 8, hipMemcpyHostToDevice, stream_), "forces");
+    check(hipMemcpyAsync(v.fz, z.data(), nodes_ * 8, hipMemcpyHostToDevice, stream_), "forces");
+    check(hipStreamSynchronize(stream_), "forces");
+}
+
+void DeviceWing::setFacePressure(const std::vector<double>& pressure) {
+    check(hipMemcpyAsync(view_->pressure, pressure.data(), triangles_ * 8, hipMemcpyHostToDevice, stream_), "pressure");
+    check(hipStreamSynchronize(stream_), "pressure");
+}
+
+void DeviceWing::launchSubstep(const StepArgs& args) {
+    const WingView& v = *view_;
+    const std::size_t scratch = kMaxThreads * sizeof(double);
+    if (sharedNodes_) {
+        const std::size_t bytes = 4 * nodes_ * sizeof(double) + scratch;
+        hipLaunchKernelGGL(substepKernel<true>, dim3(1), dim3(threads_), bytes, stream_, v, args);
+    } else if (workgroups_ == 1) {
+        hipLaunchKernelGGL(substepKernel<false>, dim3(1), dim3(threads_), scratch, stream_, v, args);
+    } else {
+        void* params[] = {const_cast<WingView*>(&v), const_cast<StepArgs*>(&args)};
+        check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&substepKernel<false>), dim3(workgroups_),
+                                         dim3(threads_), params, scratch, stream_),
+              "cooperative launch");
+    }
+    check(hipGetLastError(), "substep launch");
+}
+
+void DeviceWing::snapshot() {
+    hipLaunchKernelGGL(snapshotKernel, dim3(64), dim3(256), 0, stream_, *view_, 0);
+    check(hipGetLastError(), "snapshot");
+}
+
+void DeviceWing::restoreSnapshot() {
+    hipLaunchKernelGGL(snapshotKernel, dim3(64), dim3(256), 0, stream_, *view_, 1);
+    check(hipGetLastError(), "restore");
+    check(hipStreamSynchronize(stream_), "restore");
+}
+
+void DeviceWing::clearError() {
+    check(hipMemsetAsync(view_->error, 0, sizeof(DeviceError), stream_), "clear error");
+}
+
+DeviceError DeviceWing::readError(double* maxResidual) {
+    DeviceError e{};
+    check(hipMemcpyAsync(&e, view_->error, sizeof(DeviceError), hipMemcpyDeviceToHost, stream_), "read error");
+    double residual = 0.0;
+    check(hipMemcpyAsync(&residual, &view_->payload->maxResidual, sizeof(double), hipMemcpyDeviceToHost, stream_),
+          "read residual");
+    check(hipStreamSynchronize(stream_), "read error");
+    if (maxResidual) *maxResidual = residual;
+    return e;
+}
+
+void DeviceWing::synchronize() { check(hipStreamSynchronize(stream_), "synchronize"); }
+
+DeviceWing::State DeviceWing::download() const {
+    const WingView& v = *view_;
+    std::vector<double> block(v.stateDoubles);
+    check(hipMemcpyAsync(block.data(), v.state, snapshotBytes_, hipMemcpyDeviceToHost, stream_), "download");
+    check(hipStreamSynchronize(stream_), "download");
+    State s;
+    const double* base = block.data();
+    const auto at = [&](const double* dev) { return base + (dev - v.state); };
+    s.position.resize(3 * nodes_);
+    s.previous.resize(3 * nodes_);
+    s.velocity.resize(3 * nodes_);
+    for (std::size_t i = 0; i < nodes_; ++i) {
+        s.position[3 * i] = at(v.px)[i];
+        s.position[3 * i + 1] = at(v.py)[i];
+        s.position[3 * i + 2] = at(v.pz)[i];
+        s.previous[3 * i] = at(v.qx)[i];
+        s.previous[3 * i + 1] = at(v.qy)[i];
+        s.previous[3 * i + 2] = at(v.qz)[i];
+        s.velocity[3 * i] = at(v.vx)[i];
+        s.velocity[3 * i + 1] = at(v.vy)[i];
+        s.velocity[3 * i + 2] = at(v.vz)[i];
+    }
+    s.constraintLambda.resize(constraints_);
+    for (std::size_t k = 0; k < constraints_; ++k) s.constraintLambda[distance_.order[k]] = at(v.dlam)[k];
+    s.membraneLambda.resize(3 * membranes_);
+    s.membraneResultant.resize(3 * membranes_);
+    s.membraneResidual.resize(membranes_);
+    for (std::size_t k = 0; k < membranes_; ++k) {
+        const std::size_t m = membrane_.order[k];
+        s.membraneLambda[3 * m] = at(v.mlx)[k];
+        s.membraneLambda[3 * m + 1] = at(v.mly)[k];
+        s.membraneLambda[3 * m + 2] = at(v.mlz)[k];
+        s.membraneResultant[3 * m] = at(v.mrx)[k];
+        s.membraneResultant[3 * m + 1] = at(v.mry)[k];
+        s.membraneResultant[3 * m + 2] = at(v.mrz)[k];
+        s.membraneResidual[m] = at(v.mres)[k];
+    }
+    s.dihedralLambda.resize(hinges_);
+    for (std::size_t k = 0; k < hinges_; ++k) s.dihedralLambda[hinge_.order[k]] = at(v.hlam)[k];
+    s.segmentLambda.resize(segments_);
+    for (std::size_t k = 0; k < segments_; ++k) s.segmentLambda[k] = at(v.slam)[segmentDevicePosition_[k]];
+    s.sphereLambda.resize(pairs_);
+    for (std::size_t k = 0; k < pairs_; ++k) s.sphereLambda[k] = at(v.splam)[sphereDevicePosition_[k]];
+    s.pressure.assign(at(v.pressure), at(v.pressure) + triangles_);
+    const double* p = at(reinterpret_cast<double*>(v.payload));
+    const PayloadDev* pd = reinterpret_cast<const PayloadDev*>(p);
+    s.payload.assign(20, 0.0);
+    for (int k = 0; k < 3; ++k) {
+        s.payload[k] = pd->com[k];
+        s.payload[7 + k] = pd->v[k];
+        s.payload[10 + k] = pd->w[k];
+        s.payload[13 + k] = pd->prevCom[k];
+    }
+    for (int k = 0; k < 4; ++k) {
+        s.payload[3 + k] = pd->q[k];
+        s.payload[16 + k] = pd->prevQ[k];
+    }
+    return s;
+}
+
+void DeviceWing::upload(const State& s) {
+    const WingView& v = *view_;
+    std::vector<double> block(v.stateDoubles);
+    check(hipMemcpyAsync(block.data(), v.state, snapshotBytes_, hipMemcpyDeviceToHost, stream_), "upload read");
+    check(hipStreamSynchronize(stream_), "upload read");
+    double* base = block.data();
+    const auto at = [&](double* dev) { return base + (dev - v.state); };
+    for (std::size_t i = 0; i < nodes_; ++i) {
+        at(v.px)[i] = s.position[3 * i];
+        at(v.py)[i] = s.position[3 * i + 1];
+        at(v.pz)[i] = s.position[3 * i + 2];
+        at(v.qx)[i] = s.previous[3 * i];
+        at(v.qy)[i] = s.previous[3 * i + 1];
+        at(v.qz)[i] = s.previous[3 * i + 2];
+        at(v.vx)[i] = s.velocity[3 * i];
+        at(v.vy)[i] = s.velocity[3 * i + 1];
+        at(v.vz)[i] = s.velocity[3 * i + 2];
+    }
+    for (std::size_t k = 0; k < constraints_; ++k) at(v.dlam)[k] = s.constraintLambda[distance_.order[k]];
+    for (std::size_t k = 0; k < membranes_; ++k) {
+        const std::size_t m = membrane_.order[k];
+        at(v.mlx)[k] = s.membraneLambda[3 * m];
+        at(v.mly)[k] = s.membraneLambda[3 * m + 1];
+        at(v.mlz)[k] = s.membraneLambda[3 * m + 2];
+        at(v.mrx)[k] = s.membraneResultant[3 * m];
+        at(v.mry)[k] = s.membraneResultant[3 * m + 1];
+        at(v.mrz)[k] = s.membraneResultant[3 * m + 2];
+        at(v.mres)[k] = s.membraneResidual[m];
+    }
+    for (std::size_t k = 0; k < hinges_; ++k) at(v.hlam)[k] = s.dihedralLambda[hinge_.order[k]];
+    for (std::size_t k = 0; k < segments_; ++k) at(v.slam)[segmentDevicePosition_[k]] = s.segmentLambda[k];
+    for (std::size_t k = 0; k < pairs_; ++k) at(v.splam)[sphereDevicePosition_[k]] = s.sphereLambda[k];
+    std::copy(s.pressure.begin(), s.pressure.end(), at(v.pressure));
+    PayloadDev* pd = reinterpret_cast<PayloadDev*>(at(reinterpret_cast<double*>(v.payload)));
+    for (int k = 0; k < 3; ++k) {
+        pd->com[k] = s.payload[k];
+        pd->v[k] = s.payload[7 + k];
+        pd->w[k] = s.payload[10 + k];
+        pd->prevCom[k] = s.payload[13 + k];
+    }
+    for (int k = 0; k < 4; ++k) {
+        pd->q[k] = s.payload[3 + k];
+        pd->prevQ[k] = s.payload[16 + k];
+    }
+    check(hipMemcpyAsync(v.state, block.data(), snapshotBytes_, hipMemcpyHostToDevice, stream_), "upload");
+    check(hipStreamSynchronize(stream_), "upload");
+}
+
+DeviceWing::Reductions DeviceWing::reduce() {
+    constexpr int blocks = 64;
+    Partial* partial = static_cast<Partial*>(partials_);
+    hipLaunchKernelGGL(diagnosticsKernel, dim3(blocks), dim3(256), 0, stream_, *view_, partial);
+    check(hipGetLastError(), "diagnostics");
+    std::vector<Partial> host(blocks);
+    check(hipMemcpyAsync(host.data(), partial, blocks * sizeof(Partial), hipMemcpyDeviceToHost, stream_), "diagnostics");
+    check(hipStreamSynchronize(stream_), "diagnostics");
+    Reductions r{};
+    r.finite = 1;
+    for (const Partial& p : host) {
+        r.mass += p.mass;
+        r.weighted[0] += p.wx;
+        r.weighted[1] += p.wy;
+        r.weighted[2] += p.wz;
+        r.momentum[0] += p.mx;
+        r.momentum[1] += p.my;
+        r.momentum[2] += p.mz;
+        r.kinetic += p.kinetic;
+        r.maxDistanceError = std::max(r.maxDistanceError, p.maxDistance);
+        r.maxCableExtension = std::max(r.maxCableExtension, p.maxCable);
+        r.maxStrain = std::max(r.maxStrain, p.maxStrain);
+        r.maxResidual = std::max(r.maxResidual, p.maxResidual);
+        r.finite &= p.finite;
+        r.dynamicNodes += p.dynamicNodes;
+    }
+    return r;
+}
+
+const void* DeviceWing::viewBytes() const { return view_.get(); }
+
+}  // namespace swx
+
+// ---------------------------------------------------------------------------
+// Device math conformance probe (sw_device_math).
+
+namespace swx {
+namespace {
+
+__global__ void mathKernel(int function, const double* a, const double* b, double* out, long count) {
+    const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const double x = a[i], y = b[i];
+    double r = 0.0;
+    switch (function) {
+    case 0: r = sqrt(x); break;
+    case 1: r = x / y; break;
+    case 2: r = atan2(x, y); break;
+    case 3: r = remainder(x, 6.283185307179586476925286766559); break;
+    case 4: r = sin(x); break;
+    case 5: r = cos(x); break;
+    case 6: r = acos(x); break;
+    default: r = 0.0; break;
+    }
+    out[i] = r;
+}
+
+}  // namespace
+
+void deviceMath(int device, int function, const double* a, const double* b, double* out, std::size_t count) {
+    if (function < 0 || function > 6) throw std::out_of_range("math function must be 0..6");
+    check(hipSetDevice(device), "hipSetDevice");
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    const std::size_t bytes = std::max<std::size_t>(1, count) * sizeof(double);
+    check(hipMalloc(&da, bytes), "hipMalloc");
+    check(hipMalloc(&db, bytes), "hipMalloc");
+    check(hipMalloc(&dout, bytes), "hipMalloc");
+    try {
+        check(hipMemcpy(da, a, count * sizeof(double), hipMemcpyHostToDevice), "upload");
+        check(hipMemcpy(db, b, count * sizeof(double), hipMemcpyHostToDevice), "upload");
+        const int threads = 256;
+        const int blocks = static_cast<int>((count + threads - 1) / threads);
+        if (blocks > 0) {
+            hipLaunchKernelGGL(mathKernel, dim3(blocks), dim3(threads), 0, nullptr, function, da, db, dout,
+                               static_cast<long>(count));
+            check(hipGetLastError(), "math launch");
+        }
+        check(hipMemcpy(out, dout, count * sizeof(double), hipMemcpyDeviceToHost), "download");
+    } catch (...) {
+        (void)hipFree(da);
+        (void)hipFree(db);
+        (void)hipFree(dout);
+        throw;
+    }
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dout);
+}
+
+}  // namespace swx

@@ -1,0 +1,96 @@
+"""Regenerates the golden fixtures from the reference's own CPU solver.
+
+Runs oracle/_ref/ref_driver (the reference's src/softwing and src/fsi
+sources compiled by ``make -C oracle ref``; it needs /root/reference and so
+runs only where the reference is present) on the build's seeded synthetic
+wings, and writes into tests/golden/:
+
+* ``<preset>.assembly.swfx``  -- the reference's assembleSceneStructure
+  tables (tiny, small), plus the build's hinge colour order;
+* ``<preset>.colouring.swfx`` -- SoftBody::constraintColouringView();
+* ``<preset>.states.swfx``    -- full solver state after the listed steps;
+* ``export.digest.swfx``      -- for the ~70k-triangle wing: SHA-256 digests
+  of every assembly and colouring array, and strided samples plus digests of
+  the state after 240 steps (the full arrays are too large to commit).
+
+Usage: python tests/golden/make_fixtures.py [preset ...]
+"""
+from __future__ import annotations
+
+import hashlib
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(HERE))
+import swfx  # noqa: E402
+
+DRIVER = ROOT / "oracle" / "_ref" / "ref_driver"
+WORK = ROOT / "_work" / "fixtures"
+
+PLAN = {
+    "tiny": {"steps": 20, "snapshots": [1, 5, 20], "workers": 12},
+    "small": {"steps": 120, "snapshots": [1, 120], "workers": 12},
+    "export": {"steps": 240, "snapshots": [240], "workers": 8},
+}
+STRIDE = 101
+
+
+def digest(array: np.ndarray) -> np.ndarray:
+    """SHA-256 of the array's little-endian bytes as 4 uint64 words."""
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(array).tobytes()).digest(), dtype=np.uint64).copy()
+
+
+def run(preset: str) -> None:
+    plan = PLAN[preset]
+    WORK.mkdir(parents=True, exist_ok=True)
+    prefix = WORK / preset
+    cmd = [str(DRIVER), preset, str(prefix), str(plan["steps"]), str(plan["workers"]),
+           *map(str, plan["snapshots"])]
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    assembly = swfx.read(f"{prefix}.assembly.swfx")
+    colouring = swfx.read(f"{prefix}.colouring.swfx")
+    states = swfx.read(f"{prefix}.states.swfx")
+    if preset != "export":
+        swfx.write(str(HERE / f"{preset}.assembly.swfx"), assembly)
+        swfx.write(str(HERE / f"{preset}.colouring.swfx"), colouring)
+        keep = {k: v for k, v in states.items() if k != "state_hash"}
+        swfx.write(str(HERE / f"{preset}.states.swfx"), keep)
+        return
+    out = {}
+    for name, value in {**assembly, **colouring}.items():
+        out[f"digest/{name}"] = digest(value)
+        out[f"size/{name}"] = np.array([value.size], np.uint64)
+    tag = f"step{plan['snapshots'][-1]}"
+    for key in ("position", "previous", "velocity", "membrane_residual", "membrane_lambda", "dihedral_lambda",
+                "constraint_lambda"):
+        value = states[f"{tag}/{key}"]
+        out[f"{tag}/{key}/digest"] = digest(value)
+        width = 3 if key in ("position", "previous", "velocity", "membrane_lambda") else 1
+        rows = value.reshape(-1, width)
+        picked = np.arange(0, rows.shape[0], STRIDE)
+        out[f"{tag}/{key}/rows"] = picked.astype(np.uint64)
+        out[f"{tag}/{key}/samples"] = rows[picked].reshape(-1).copy()
+        out[f"{tag}/{key}/maxabs"] = np.array([np.abs(value).max()])
+    for key in ("segment_lambda", "payload", "suspension_residual"):
+        out[f"{tag}/{key}"] = states[f"{tag}/{key}"]
+    out["dihedral_colour_order_digest"] = digest(assembly["dihedral_colour_order"])
+    swfx.write(str(HERE / "export.digest.swfx"), out)
+
+
+def main(argv: list[str]) -> int:
+    if not DRIVER.exists():
+        print(f"{DRIVER} is missing: run make -C oracle ref (needs /root/reference)", file=sys.stderr)
+        return 2
+    for preset in argv or list(PLAN):
+        run(preset)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
