@@ -1,0 +1,282 @@
+#!/usr/bin/env python3
+"""Throughput of the XPBD structural substep on one MI355X.
+
+Workload: the seeded 70k-triangle FP64 export-scale wing (43k nodes, 70.8k
+membrane triangles, 98.5k dihedral hinges, 42.4k seam constraints, 300
+suspension segments with a rigid 90 kg pilot, 8 contact spheres), flight
+settings dt = 1/120 s, one substep of 48 iterations per step. Inputs are
+resident in HBM before the timed region; the timed region is K whole steps
+(`sw_structure_step_many`), bracketed by device synchronisation.
+
+Printed: one JSON line. ``value`` is constraint projections per second over
+the timed steps; ``roofline`` prices the substep kernel's algorithmic bytes
+(per-family byte model below) against its HIP-event duration on the
+structure's stream; ``cpu_baseline`` times the reference's own CPU solver
+(``oracle/_ref/ref_driver``, built from the reference sources) on a bounded
+sample of the same wing, or the serial oracle when that binary is absent.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "constraint projections/s @70k-tri FP64 wing"
+UNIT = "constraint projections/s"
+
+# Algorithmic HBM bytes per projection visit (FP64 values, 32-bit indices,
+# SoA). Node traffic per visit is counted once per visit because the items of
+# one colour are node-disjoint.
+BYTES = {
+    "membrane": 316,        # idx 12, inverse chart 32, sqrt(A) 8, compliance 48, lambda r+w 48, x r+w 144, w 24
+    "membrane_damped": 72,  # + previous positions when the membrane damping is non-zero
+    "hinge": 272,           # idx 16, rest angle 8, compliance 8, lambda r+w 16, x r+w 192, w 32
+    "rail": 152,            # idx 8, rest 8, compliance 8, lambda r+w 16, x r+w 96, w 16
+    "stitch": 136,          # rail without the per-constraint rest/compliance reads
+    "segment": 152,
+    "sphere": 120,          # idx 8, radius 8, compliance 8, lambda r+w 16, x r+w 48, centre x 24, w 8
+    "predict": 152,         # per node: x, v, f, w read; prev, x, v written
+    "finalize": 80,         # per node: x, prev, w read; v written
+    "pressure": 96,         # per node-triangle incidence (3 per triangle)
+    "residual": 300,        # per membrane element diagnostic pass
+}
+
+
+def family_counts(definition) -> dict:
+    a = definition.arrays
+    kinds = np.asarray(a["constraint_kind"])
+    suspension = definition.suspension
+    return {
+        "nodes": int(definition.node_count),
+        "membranes": int(np.asarray(a["membrane_triangle"]).size),
+        "hinges": int(np.asarray(a["dihedral_rest_angle"]).size),
+        "stitches": int(np.count_nonzero(kinds == 4)),
+        "rails": int(np.count_nonzero(kinds != 4)),
+        "segments": 0 if suspension is None else int(np.asarray(suspension["segment_ids"]).size),
+        "sphere_pairs": int(np.asarray(a["sphere_nodes"]).size),
+        "triangles": int(definition.triangle_count),
+    }
+
+
+def projections_per_substep(c: dict, iterations: int) -> int:
+    per_iteration = c["membranes"] + c["hinges"] + c["stitches"] + c["rails"] + c["segments"] + c["sphere_pairs"]
+    return iterations * per_iteration + c["stitches"]  # + the seam-stitch closing pass
+
+
+def bytes_per_substep(c: dict, iterations: int, membrane_damped: bool) -> int:
+    b = BYTES
+    per_iteration = (c["membranes"] * (b["membrane"] + (b["membrane_damped"] if membrane_damped else 0))
+                     + c["hinges"] * b["hinge"] + c["rails"] * b["rail"] + c["stitches"] * b["stitch"]
+                     + c["segments"] * b["segment"] + c["sphere_pairs"] * b["sphere"])
+    once = (c["nodes"] * (b["predict"] + b["finalize"]) + 3 * c["triangles"] * b["pressure"]
+            + c["membranes"] * b["residual"] + c["stitches"] * b["stitch"])
+    return iterations * per_iteration + once
+
+
+def launch_candidates(cus: int) -> list[tuple[int, int]]:
+    shapes = [(1, 512)]
+    for wg in (cus // 4, cus // 2, cus):
+        for threads in (256, 512):
+            if wg >= 1:
+                shapes.append((wg, threads))
+    return shapes
+
+
+def tune(structure, loads, settings, cus: int) -> tuple[tuple[int, int], dict]:
+    """Launch-shape autotuner: one timed step per candidate shape, keep the
+    fastest. The state trajectory does not depend on the shape."""
+    results = {}
+    for shape in launch_candidates(cus):
+        try:
+            structure.set_launch(*shape)
+        except Exception:  # shape outside the device's limits
+            continue
+        structure.set_kernel_timing(True)
+        structure.set_external_forces(loads)
+        structure.step_many(1, settings)
+        ms, launches = structure.kernel_time()
+        structure.set_kernel_timing(False)
+        results[f"{shape[0]}x{shape[1]}"] = ms / max(launches, 1)
+    best = min(results, key=results.get)
+    wg, threads = (int(v) for v in best.split("x"))
+    structure.set_launch(wg, threads)
+    return (wg, threads), results
+
+
+def cpu_share() -> int:
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 16))
+
+
+def cpu_baseline(workload: str, counts_no_spheres: dict, iterations: int, budget_s: float) -> dict | None:
+    driver = ROOT / "oracle" / "_ref" / "ref_driver"
+    workers = cpu_share()
+    if driver.exists():
+        cmd = [str(driver), "--bench", workload, str(workers), str(budget_s)]
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        if out.returncode != 0:
+            raise RuntimeError(f"reference CPU baseline failed: {out.stderr.strip()}")
+        line = json.loads(out.stdout.strip().splitlines()[-1])
+        ms = line["ms_per_step_mean"]
+        return {
+            "value": projections_per_substep(counts_no_spheres, iterations) / (ms * 1e-3),
+            "unit": UNIT,
+            "cores": workers,
+            "kind": "reference",
+            "sample": (f"{workload} wing without contact spheres (the reference has no sphere family), "
+                       f"{line['warmup']} warm-up + {line['steps']} timed steps, workerThreads={workers}, "
+                       f"mean {ms:.1f} ms/step, best {line['ms_per_step_best']:.1f} ms/step"),
+        }
+    # Serial oracle (bit-identical to the reference path), one core.
+    import oracle as orc
+    import simwing_amd as sw
+    definition, pressure, loads = make_workload(workload)
+    body = orc.OracleBody(definition)
+    body.set_face_pressure(pressure)
+    settings = sw.flight_step_settings(iterations)
+    body.set_external_forces(loads)
+    body.step(settings)
+    steps, start = 0, time.perf_counter()
+    while time.perf_counter() - start < budget_s or steps < 2:
+        body.set_external_forces(loads)
+        body.step(settings)
+        steps += 1
+    ms = 1e3 * (time.perf_counter() - start) / steps
+    return {
+        "value": projections_per_substep(family_counts(definition), iterations) / (ms * 1e-3),
+        "unit": UNIT,
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{workload} wing, serial oracle, 1 warm-up + {steps} timed steps, mean {ms:.1f} ms/step",
+    }
+
+
+def make_workload(name: str):
+    import simwing_amd as sw
+    scene = sw.generate_scene(name)
+    assembly = scene.assemble()
+    return assembly.definition, scene.face_pressure(), assembly.flight_loads()
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=20)
+    parser.add_argument("--warmup", type=int, default=3)
+    parser.add_argument("--workload", default="export")
+    parser.add_argument("--iterations", type=int, default=48)
+    parser.add_argument("--cpu-seconds", type=float, default=15.0)
+    parser.add_argument("--no-cpu", action="store_true")
+    parser.add_argument("--no-tune", action="store_true")
+    parser.add_argument("--launch", default=None, help="fixed launch shape WGxTHREADS")
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the structural substep of one wing runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        return 2
+
+    import simwing_amd as sw
+
+    t0 = time.perf_counter()
+    definition, pressure, loads = make_workload(args.workload)
+    counts = family_counts(definition)
+    print(f"# workload {args.workload}: {counts} (assembled in {time.perf_counter() - t0:.1f} s)",
+          file=sys.stderr, flush=True)
+    structure = sw.Structure(definition)
+    structure.set_face_pressure(pressure)
+    settings = sw.flight_step_settings(args.iterations)
+    cus = 256  # MI355X compute units; shapes beyond the device's limits are skipped by the tuner
+
+    structure.set_external_forces(loads)
+    structure.step_many(1, settings)  # first launch: code object load
+    tuning = {}
+    if args.launch:
+        wg, threads = (int(v) for v in args.launch.split("x"))
+        structure.set_launch(wg, threads)
+        shape = (wg, threads)
+    elif args.no_tune:
+        shape = structure.launch()
+    else:
+        shape, tuning = tune(structure, loads, settings, cus)
+    print(f"# launch shape {shape} tuning(ms/launch)={tuning}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        structure.set_external_forces(loads)
+        structure.step_many(1, settings)
+
+    structure.set_kernel_timing(True)
+    structure.set_external_forces(loads)
+    start = time.perf_counter()
+    structure.step_many(args.steps, settings)  # returns after the stream has drained
+    elapsed = time.perf_counter() - start
+    kernel_ms, launches = structure.kernel_time()
+    structure.set_kernel_timing(False)
+    diagnostics = structure.diagnostics()
+    if diagnostics["finite"] != 1:
+        raise RuntimeError("non-finite state after the timed steps")
+
+    substeps = settings.substeps
+    proj_step = projections_per_substep(counts, args.iterations) * substeps
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = proj_step * args.steps / elapsed
+    avg_kernel_ms = kernel_ms / max(launches, 1)
+    algorithmic = bytes_per_substep(counts, args.iterations, membrane_damped=False)
+    achieved = algorithmic / (avg_kernel_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        no_spheres = dict(counts, sphere_pairs=0)
+        cpu = cpu_baseline(args.workload, no_spheres, args.iterations, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload} wing: {counts['nodes']} nodes, {counts['membranes']} membrane triangles, "
+                        f"{counts['hinges']} hinges, {counts['rails'] + counts['stitches']} seam constraints, "
+                        f"{counts['segments']} line segments + rigid pilot, {counts['sphere_pairs']} sphere pairs; "
+                        f"dt 1/120 s, {args.iterations} iterations, {substeps} substep",
+            "projections_per_step": proj_step,
+            "launch": f"{shape[0]}x{shape[1]}",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "substepKernel",
+            "kernel_ms": avg_kernel_ms,
+            "algorithmic_bytes": algorithmic,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

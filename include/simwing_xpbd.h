@@ -254,6 +254,14 @@ sw_status sw_device_math(int device, int function, const double* a, const double
 sw_status sw_structure_set_launch(sw_structure* s, int workgroups, int threads);
 sw_status sw_structure_get_launch(const sw_structure* s, int* workgroups, int* threads);
 
+/* Substep-kernel timing (no counterpart in the reference; measurement only).
+ * While enabled, each substep kernel launch is bracketed by HIP events on the
+ * structure's own stream. sw_structure_kernel_time waits for the stream and
+ * returns the summed kernel time in milliseconds and the launch count since
+ * timing was last enabled. */
+sw_status sw_structure_set_kernel_timing(sw_structure* s, int enabled);
+sw_status sw_structure_kernel_time(sw_structure* s, double* milliseconds, long long* launches);
+
 /* ---- Scene input and assembly (scene_structure.h:162-165) ------------- */
 
 typedef struct sw_scene sw_scene;

@@ -34,6 +34,7 @@
 #include <softwing/soft_body.h>
 #include <softwing/suspension.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -523,8 +524,36 @@ int main(int argc, char** argv) {
                         static_cast<unsigned long long>(hb), ha == hb ? "IDENTICAL" : "DIFFERENT");
             return ha == hb ? 0 : 1;
         }
+        if (argc >= 5 && std::strcmp(argv[1], "--bench") == 0) {
+            // Timing in the softwing-bench manner (tools/softwing_bench.cpp:1783-1801):
+            // warm-up steps, then steady_clock around each step until the budget is spent.
+            const std::string preset = argv[2];
+            const unsigned workers = static_cast<unsigned>(std::atoi(argv[3]));
+            const double budget = std::atof(argv[4]);
+            Prepared p = prepare(preset);
+            Rig rig = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure);
+            const sw::StepSettings settings = stepSettings(workers);
+            const int warmup = 2;
+            for (int s = 0; s < warmup; ++s) stepRig(rig, settings);
+            double total = 0.0, best = 1e300;
+            int steps = 0;
+            while (steps < 3 || (total < budget && steps < 1000)) {
+                const auto t0 = std::chrono::steady_clock::now();
+                stepRig(rig, settings);
+                const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                total += dt;
+                best = std::min(best, dt);
+                ++steps;
+            }
+            std::printf("{\"preset\": \"%s\", \"workers\": %u, \"warmup\": %d, \"steps\": %d, "
+                        "\"ms_per_step_mean\": %.6f, \"ms_per_step_best\": %.6f}\n",
+                        preset.c_str(), workers, warmup, steps, 1e3 * total / steps, 1e3 * best);
+            return 0;
+        }
         if (argc < 5) {
-            std::fprintf(stderr, "usage: ref_driver <preset> <out-prefix> <steps> <workers> [snapshot ...]\n");
+            std::fprintf(stderr, "usage: ref_driver <preset> <out-prefix> <steps> <workers> [snapshot ...]\n"
+                                 "       ref_driver --bench <preset> <workers> <seconds>\n"
+                                 "       ref_driver --compare-workers <preset> <steps>\n");
             return 2;
         }
         const std::string preset = argv[1];

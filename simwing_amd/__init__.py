@@ -430,6 +430,17 @@ class Structure:
     def set_launch(self, workgroups: int, threads: int) -> None:
         check(native.library().sw_structure_set_launch(self._handle, workgroups, threads))
 
+    def set_kernel_timing(self, enabled: bool) -> None:
+        check(native.library().sw_structure_set_kernel_timing(self._handle, int(enabled)))
+
+    def kernel_time(self) -> tuple[float, int]:
+        """Summed substep-kernel time (ms, HIP events on the structure's
+        stream) and launch count since timing was enabled."""
+        ms = C.c_double()
+        launches = C.c_longlong()
+        check(native.library().sw_structure_kernel_time(self._handle, C.byref(ms), C.byref(launches)))
+        return ms.value, launches.value
+
     def launch(self) -> tuple[int, int]:
         w, t = C.c_int(), C.c_int()
         check(native.library().sw_structure_get_launch(self._handle, C.byref(w), C.byref(t)))

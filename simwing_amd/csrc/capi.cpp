@@ -497,6 +497,20 @@ sw_status sw_structure_set_launch(sw_structure* s, int workgroups, int threads) 
     });
 }
 
+sw_status sw_structure_set_kernel_timing(sw_structure* s, int enabled) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        s->wing->setTiming(enabled != 0);
+    });
+}
+
+sw_status sw_structure_kernel_time(sw_structure* s, double* milliseconds, long long* launches) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        s->wing->kernelTime(milliseconds, launches);
+    });
+}
+
 sw_status sw_structure_get_launch(const sw_structure* s, int* workgroups, int* threads) {
     return guarded([&] {
         requirePointer(s, "structure");
@@ -527,6 +541,11 @@ sw_status sw_scene_generate(const char* preset, sw_scene** out) {
         requirePointer(preset, "preset");
         requirePointer(out, "out");
         std::string name(preset);
+        std::string overrides;
+        if (const auto q = name.find('?'); q != std::string::npos) {
+            overrides = name.substr(q + 1);
+            name = name.substr(0, q);
+        }
         swx::WingParams params;
         if (name.rfind("batch:", 0) == 0) {
             unsigned long long seed = 0;
@@ -538,6 +557,7 @@ sw_status sw_scene_generate(const char* preset, sw_scene** out) {
         } else {
             params = swx::wingPreset(name);
         }
+        swx::applyOverrides(params, overrides);
         auto s = std::make_unique<sw_scene>();
         s->wing = swx::generateWing(params);
         *out = s.release();

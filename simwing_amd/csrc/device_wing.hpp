@@ -118,6 +118,13 @@ public:
     const SuspensionLayout& suspension() const { return layout_; }
     const std::vector<Phase>& iterationPhases() const { return iterPhases_; }
 
+    // Substep-kernel timing: while enabled, every substep launch is
+    // bracketed by a pair of HIP events recorded on the wing's stream.
+    // kernelTime() waits for the stream and returns the summed elapsed time
+    // of the pairs recorded since timing was enabled, and their count.
+    void setTiming(bool enabled);
+    void kernelTime(double* milliseconds, long long* launches);
+
     void setLaunch(int workgroups, int threads);
     int workgroups() const { return workgroups_; }
     int threads() const { return threads_; }
@@ -136,6 +143,11 @@ private:
     ColourOrder distance_, membrane_, hinge_;
     SuspensionLayout layout_;
     std::vector<Phase> iterPhases_, closePhases_;
+    bool timing_ = false;
+    std::vector<hipEvent_t> events_;  // start/stop pairs, grown on demand
+    std::size_t eventsUsed_ = 0;
+    double timedMs_ = 0.0;
+    long long timedLaunches_ = 0;
     std::vector<std::uint32_t> segmentDevicePosition_;  // runtime order -> device slot
     std::vector<std::uint32_t> sphereDevicePosition_;   // definition pair -> device slot
     std::size_t nodes_ = 0, triangles_ = 0, constraints_ = 0, membranes_ = 0, hinges_ = 0, segments_ = 0,

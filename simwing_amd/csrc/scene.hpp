@@ -181,6 +181,7 @@ struct WingParams {
     double lineLinearDensity = 0.002;
     double lineRestScale = 1.0;
     double fabricDensity = 0.040;
+    double attachmentReinforcement = 20.0;  // density factor near line attachments
     double complianceMin = 1.0e-7;   // per-triangle stretch compliance, m/N
     double complianceMax = 1.0e-6;
     double bendingRigidity = 1.0e-3; // N*m
@@ -241,6 +242,10 @@ namespace swx {
 // rest-length scale U(0.97, 1.03), all drawn from `seed`.
 [[nodiscard]] std::vector<WingParams> batchPresets(std::uint64_t seed, int count,
                                                    double minNodes, double maxNodes);
+
+// Applies "key=value&key=value" overrides (WingParams field names) to a
+// preset; throws std::invalid_argument on an unknown key or bad value.
+void applyOverrides(WingParams& params, const std::string& overrides);
 
 // Steady flight loads of the workload: the whole system's weight
 // `totalWeight` (N) carried by the first `surfaceNodes` nodes in proportion

@@ -165,6 +165,11 @@ GeneratedWing generateWing(const WingParams& P) {
         ribOfStation[i] = static_cast<int>(ribStations.size());
         ribStations.push_back(i);
     }
+    // Both wing tips are closed by a rib (no free, unsupported skin edge).
+    if (ribOfStation[nS - 1] < 0) {
+        ribOfStation[nS - 1] = static_cast<int>(ribStations.size());
+        ribStations.push_back(nS - 1);
+    }
     const int nRibs = static_cast<int>(ribStations.size());
 
     // Regions: one outside region, one cell per inter-rib bay.
@@ -391,6 +396,25 @@ GeneratedWing generateWing(const WingParams& P) {
         lineNodes[l].push_back(isLeft ? harnessLeft : harnessRight);
         linePoints[l].push_back(h);
     }
+    // Reinforcement patches: the fabric of every triangle within two rings
+    // of a line attachment is `attachmentReinforcement` times denser (the
+    // load tape of a real attachment point), so the attachment nodes are not
+    // the lightest nodes of the mesh while carrying the line loads.
+    if (P.attachmentReinforcement != 1.0) {
+        std::vector<std::vector<std::size_t>> trianglesOf(B.positionById.size());
+        for (std::size_t t = 0; t < S.triangles.size(); ++t)
+            for (StableId v : S.triangles[t].vertexIds) trianglesOf[v].push_back(t);
+        std::vector<char> reinforced(S.triangles.size(), 0);
+        for (const Attachment& a : surfaceAttachments) {
+            std::vector<StableId> ring{a.vertexId};
+            for (std::size_t t : trianglesOf[a.vertexId])
+                for (StableId v : S.triangles[t].vertexIds) ring.push_back(v);
+            for (StableId v : ring)
+                for (std::size_t t : trianglesOf[v]) reinforced[t] = 1;
+        }
+        for (std::size_t t = 0; t < S.triangles.size(); ++t)
+            if (reinforced[t]) S.fabricMaterials[t].arealDensity *= P.attachmentReinforcement;
+    }
     // Junction masses: one segment of line each, nudged upward until the
     // reciprocal of its reciprocal inverse mass is exact (the reference
     // re-derives a junction's mass from the node's inverse mass).
@@ -492,6 +516,50 @@ WingParams wingPreset(const std::string& name) {
         return p;
     }
     throw std::invalid_argument("unknown wing preset: " + name);
+}
+
+void applyOverrides(WingParams& p, const std::string& overrides) {
+    std::size_t start = 0;
+    while (start < overrides.size()) {
+        std::size_t end = overrides.find('&', start);
+        if (end == std::string::npos) end = overrides.size();
+        const std::string item = overrides.substr(start, end - start);
+        start = end + 1;
+        const std::size_t eq = item.find('=');
+        if (eq == std::string::npos) throw std::invalid_argument("override must be key=value: " + item);
+        const std::string key = item.substr(0, eq);
+        const double value = std::stod(item.substr(eq + 1));
+        double* reals[] = {&p.span, &p.rootChord, &p.tipChordFloor, &p.thickness, &p.lineLength,
+                           &p.lineAxialStiffness, &p.lineLinearDensity, &p.lineRestScale, &p.fabricDensity,
+                           &p.attachmentReinforcement, &p.complianceMin, &p.complianceMax, &p.bendingRigidity,
+                           &p.pressureMean, &p.pressureStd, &p.seamAxialStiffness, &p.seamLinearDensity,
+                           &p.pilotMass, &p.contactRadius};
+        const char* realNames[] = {"span", "rootChord", "tipChordFloor", "thickness", "lineLength",
+                                   "lineAxialStiffness", "lineLinearDensity", "lineRestScale", "fabricDensity",
+                                   "attachmentReinforcement", "complianceMin", "complianceMax", "bendingRigidity",
+                                   "pressureMean", "pressureStd", "seamAxialStiffness", "seamLinearDensity",
+                                   "pilotMass", "contactRadius"};
+        int* ints[] = {&p.spanNodes, &p.chordNodes, &p.ribEvery, &p.lineRows, &p.lineCount, &p.segmentsPerLine,
+                       &p.contactSpheres};
+        const char* intNames[] = {"spanNodes", "chordNodes", "ribEvery", "lineRows", "lineCount", "segmentsPerLine",
+                                  "contactSpheres"};
+        bool found = false;
+        for (std::size_t k = 0; k < sizeof(reals) / sizeof(reals[0]); ++k)
+            if (key == realNames[k]) {
+                *reals[k] = value;
+                found = true;
+            }
+        for (std::size_t k = 0; k < sizeof(ints) / sizeof(ints[0]); ++k)
+            if (key == intNames[k]) {
+                *ints[k] = static_cast<int>(value);
+                found = true;
+            }
+        if (key == "seed") {
+            p.seed = static_cast<std::uint64_t>(std::stoull(item.substr(eq + 1)));
+            found = true;
+        }
+        if (!found) throw std::invalid_argument("unknown wing parameter: " + key);
+    }
 }
 
 std::vector<WingParams> batchPresets(std::uint64_t seed, int count, double minNodes,

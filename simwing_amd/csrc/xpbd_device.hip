@@ -920,7 +920,33 @@ DeviceWing::~DeviceWing() {
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : allocations_) (void)hipFree(p);
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void DeviceWing::setTiming(bool enabled) {
+    check(hipSetDevice(device_), "hipSetDevice");
+    if (enabled && !timing_) {
+        check(hipStreamSynchronize(stream_), "timing");
+        eventsUsed_ = 0;
+        timedMs_ = 0.0;
+        timedLaunches_ = 0;
+    }
+    timing_ = enabled;
+}
+
+void DeviceWing::kernelTime(double* milliseconds, long long* launches) {
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipStreamSynchronize(stream_), "timing");
+    for (std::size_t i = 0; i + 1 < eventsUsed_; i += 2) {
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, events_[i], events_[i + 1]), "hipEventElapsedTime");
+        timedMs_ += ms;
+        ++timedLaunches_;
+    }
+    eventsUsed_ = 0;
+    if (milliseconds) *milliseconds = timedMs_;
+    if (launches) *launches = timedLaunches_;
 }
 
 void DeviceWing::build(const Definition& d) {
@@ -1389,6 +1415,14 @@ void DeviceWing::setFacePressure(const std::vector<double>& pressure) {
 
 void DeviceWing::launchSubstep(const StepArgs& args) {
     const WingView& v = *view_;
+    if (timing_) {
+        while (events_.size() < eventsUsed_ + 2) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            events_.push_back(e);
+        }
+        check(hipEventRecord(events_[eventsUsed_], stream_), "hipEventRecord");
+    }
     const std::size_t scratch = kMaxThreads * sizeof(double);
     if (sharedNodes_) {
         const std::size_t bytes = 4 * nodes_ * sizeof(double) + scratch;
@@ -1402,6 +1436,10 @@ void DeviceWing::launchSubstep(const StepArgs& args) {
               "cooperative launch");
     }
     check(hipGetLastError(), "substep launch");
+    if (timing_) {
+        check(hipEventRecord(events_[eventsUsed_ + 1], stream_), "hipEventRecord");
+        eventsUsed_ += 2;
+    }
 }
 
 void DeviceWing::snapshot() {
