@@ -84,9 +84,9 @@ def bytes_per_substep(c: dict, iterations: int, membrane_damped: bool) -> int:
 
 
 def launch_candidates(cus: int) -> list[tuple[int, int]]:
-    shapes = [(1, 512)]
-    for wg in (cus // 4, cus // 2, cus):
-        for threads in (256, 512):
+    shapes = []
+    for wg in (cus // 16, cus // 8, cus // 4, cus // 2):
+        for threads in (128, 256):
             if wg >= 1:
                 shapes.append((wg, threads))
     return shapes
@@ -141,6 +141,7 @@ def cpu_baseline(workload: str, counts_no_spheres: dict, iterations: int, budget
                        f"mean {ms:.1f} ms/step, best {line['ms_per_step_best']:.1f} ms/step"),
         }
     # Serial oracle (bit-identical to the reference path), one core.
+    sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as orc
     import simwing_amd as sw
     definition, pressure, loads = make_workload(workload)

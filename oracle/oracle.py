@@ -30,8 +30,37 @@ def library():
                                          C.POINTER(C.c_size_t)]
         lib.oracle_set_angle_perturbation.argtypes = [C.c_void_p, C.c_int]
         lib.oracle_set_angle_perturbation.restype = None
+        lib.oracle_set_portable_math.argtypes = [C.c_void_p, C.c_int]
+        lib.oracle_set_portable_math.restype = None
+        for name in ("oracle_math",):
+            getattr(lib, name).argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+            getattr(lib, name).restype = None
+        lib.oracle_quad_math.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        lib.oracle_quad_math.restype = None
         _lib = lib
     return _lib
+
+
+MATH_FUNCTIONS = {"atan2": 2, "sin": 4, "cos": 5, "acos": 6}
+
+
+def math(name: str, a: np.ndarray, b: np.ndarray | None = None, portable: bool = True) -> np.ndarray:
+    """The oracle's transcendental `name` on the host: the portable functions
+    (portable=True) or the C library's."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(a if b is None else b, np.float64)
+    out = np.empty_like(a)
+    library().oracle_math(int(portable), MATH_FUNCTIONS[name], _p(a), _p(b), _p(out), a.size)
+    return out
+
+
+def quad_math(name: str, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
+    """`name` in quadruple precision (libquadmath), rounded to double."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(a if b is None else b, np.float64)
+    out = np.empty_like(a)
+    library().oracle_quad_math(MATH_FUNCTIONS[name], _p(a), _p(b), _p(out), a.size)
+    return out
 
 
 class OracleError(RuntimeError):
@@ -81,6 +110,10 @@ class OracleBody:
 
     def step(self, settings):
         _check(library().oracle_step(self._handle, C.byref(settings)))
+
+    def use_portable_math(self, enabled: bool = True):
+        """Steps use the portable transcendentals of the device kernels."""
+        library().oracle_set_portable_math(self._handle, int(enabled))
 
     def perturb_angles(self, ulps: int):
         library().oracle_set_angle_perturbation(self._handle, ulps)

@@ -14,6 +14,10 @@
 
 #include "oracle_colouring.hpp"
 
+#include "../simwing_amd/csrc/portable_math.hpp"
+
+#include <quadmath.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -29,6 +33,15 @@
 namespace {
 
 thread_local std::string lastError;
+
+// Transcendentals: the C library's, as the reference calls them (default),
+// or the portable double-double functions that the device kernels use
+// (oracle_set_portable_math), which return the same bits on every platform.
+thread_local bool portableMath = false;
+double mAtan2(double y, double x) { return portableMath ? swx::pm::atan2(y, x) : std::atan2(y, x); }
+double mSin(double x) { return portableMath ? swx::pm::sin(x) : std::sin(x); }
+double mCos(double x) { return portableMath ? swx::pm::cos(x) : std::cos(x); }
+double mAcos(double x) { return portableMath ? swx::pm::acos(x) : std::acos(x); }
 
 struct V {
     double x = 0.0, y = 0.0, z = 0.0;
@@ -241,8 +254,8 @@ Q increment(const V& v) {
     const double angle = length(v);
     if (angle < 1.0e-12) return canonical({1.0, 0.5 * v.x, 0.5 * v.y, 0.5 * v.z});
     const double half = 0.5 * angle;
-    const double s = std::sin(half) / angle;
-    return {std::cos(half), v.x * s, v.y * s, v.z * s};
+    const double s = mSin(half) / angle;
+    return {mCos(half), v.x * s, v.y * s, v.z * s};
 }
 
 struct Node {
@@ -304,6 +317,7 @@ struct oracle_body {
     std::vector<std::size_t> loadPath;
     std::vector<V> external;
     int perturbUlps = 0;
+    bool portableMath = false;
 };
 
 namespace {
@@ -426,7 +440,7 @@ void projectHinge(oracle_body& b, Hinge& h, double invDt2) {
     const V ca = cross(a1a, edge);
     ea += cross(a2a, toD);
     const V da = cross(edge, a2a);
-    double angle = std::atan2(sine, cosine);
+    double angle = mAtan2(sine, cosine);
     if (b.perturbUlps) angle = perturbed(angle, b.perturbUlps);
     const V g[4] = {-1.0 * (ea + ca + da), ea, ca, da};
     if (!std::isfinite(angle)) return;
@@ -725,8 +739,8 @@ void substep(oracle_body& b, double dt, const sw_step_settings& st) {
         p.v = (p.com - p.prevCom) / dt;
         Q delta = p.q * Q{p.prevQ.w, -p.prevQ.x, -p.prevQ.y, -p.prevQ.z};
         delta = canonical(delta);
-        const double half = std::acos(std::clamp(delta.w, -1.0, 1.0));
-        const double sine = std::sin(half);
+        const double half = mAcos(std::clamp(delta.w, -1.0, 1.0));
+        const double sine = mSin(half);
         if (std::abs(sine) < 1.0e-12) p.w = V{delta.x, delta.y, delta.z} * (2.0 / dt);
         else p.w = V{delta.x / sine, delta.y / sine, delta.z / sine} * (2.0 * half / dt);
         // suspension.cpp:2324-2384
@@ -917,7 +931,39 @@ int oracle_set_external_forces(oracle_body* b, const double* forces, size_t coun
 
 void oracle_set_angle_perturbation(oracle_body* b, int ulps) { b->perturbUlps = ulps; }
 
+void oracle_set_portable_math(oracle_body* b, int portable) { b->portableMath = portable != 0; }
+
+void oracle_math(int portable, int function, const double* a, const double* b, double* out, size_t count) {
+    portableMath = portable != 0;
+    for (size_t i = 0; i < count; ++i) {
+        switch (function) {
+        case 2: out[i] = mAtan2(a[i], b[i]); break;
+        case 4: out[i] = mSin(a[i]); break;
+        case 5: out[i] = mCos(a[i]); break;
+        case 6: out[i] = mAcos(a[i]); break;
+        default: out[i] = std::nan(""); break;
+        }
+    }
+    portableMath = false;
+}
+
+void oracle_quad_math(int function, const double* a, const double* b, double* out, size_t count) {
+    for (size_t i = 0; i < count; ++i) {
+        const __float128 x = a[i], y = b[i];
+        __float128 r;
+        switch (function) {
+        case 2: r = atan2q(x, y); break;
+        case 4: r = sinq(x); break;
+        case 5: r = cosq(x); break;
+        case 6: r = acosq(x); break;
+        default: r = nanq(""); break;
+        }
+        out[i] = static_cast<double>(r);
+    }
+}
+
 int oracle_step(oracle_body* b, const sw_step_settings* st) {
+    portableMath = b->portableMath;
     return guard([&] {
         if (!(st->time_step > 0.0) || st->substeps <= 0 || st->constraint_iterations < 0)
             throw std::invalid_argument("Invalid SoftBody step settings");

@@ -37,6 +37,16 @@ const char* oracle_last_error(void);
  * (used only to measure how far a one-ulp difference grows). */
 void oracle_set_angle_perturbation(oracle_body* body, int ulps);
 
+/* Transcendentals of the body's steps: 0 = the C library (as the reference),
+ * 1 = the portable double-double functions the device kernels use. */
+void oracle_set_portable_math(oracle_body* body, int portable);
+
+/* Evaluates atan2(a, b) (function 2), sin(a) (4), cos(a) (5) or acos(a) (6)
+ * with the C library (portable = 0) or the portable functions (1), and the
+ * same functions in quadruple precision rounded to double. */
+void oracle_math(int portable, int function, const double* a, const double* b, double* out, size_t count);
+void oracle_quad_math(int function, const double* a, const double* b, double* out, size_t count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,8 +30,15 @@ struct Phase {
     int type;
     int begin;
     int end;
-    int flags;  // bit 0: the phase holds the iteration's last payload segment
+    int flags;      // unused (0)
+    int slotBegin;  // kPhaseSegmentRun: the run's staged nodes are
+    int slotEnd;    // runNodes[slotBegin, slotEnd)
 };
+
+// Capacity of one staged serial segment run (kPhaseSegmentRun): items and
+// distinct nodes. Longer runs are split into consecutive phases.
+constexpr int kChainItems = 128;
+constexpr int kChainSlots = 256;
 
 // Rigid payload state and definition on the device.
 struct PayloadDev {
@@ -69,6 +76,7 @@ struct StepArgs {
     double reference[3];
     double maxLineResidual;
     int iterations;
+    int dedicatedChain;  // set at launch: the last workgroup runs the serial segment run
 };
 
 class DeviceWing {
@@ -143,6 +151,7 @@ private:
     ColourOrder distance_, membrane_, hinge_;
     SuspensionLayout layout_;
     std::vector<Phase> iterPhases_, closePhases_;
+    int chainPhase_ = -1;  // program index of the single serial segment run, or -1
     bool timing_ = false;
     std::vector<hipEvent_t> events_;  // start/stop pairs, grown on demand
     std::size_t eventsUsed_ = 0;

@@ -55,6 +55,8 @@ struct WingView {
     double *srest, *sstiff, *slam;
     unsigned char* slastPayload;
     int *sruntime;  // device slot -> runtime index (error reporting)
+    int *sfromSlot, *stoSlot;  // staged-run endpoints: run-local node slot, or the hang-point code
+    int* runNodes;             // node of each staged-run slot (see Phase::slotBegin)
     int hangCount;
     double* hangLocal;
     PayloadDev* payload;
@@ -69,6 +71,12 @@ struct WingView {
     int closePhaseCount;
     DeviceError* error;
     unsigned* barrier;
+    // Dedicated chain workgroup: program index of the serial segment run,
+    // the first later phase (cyclically) that touches its nodes or the
+    // payload (-1: none), and the handshake counters (runReady at [0],
+    // chainDone at [64]).
+    int chainPhase, chainWait;
+    unsigned* chainSync;
     double* state;      // contiguous mutable state block
     double* snapshot;   // copy of `state`
     long stateDoubles;
@@ -76,7 +84,7 @@ struct WingView {
 
 namespace {
 
-constexpr int kMaxThreads = 512;
+constexpr int kMaxThreads = 256;
 
 __device__ __forceinline__ void raise(const WingView& v, int code, int phase, int kind, int index) {
     if (atomicCAS(&v.error->code, 0, code) == 0) {
@@ -86,9 +94,13 @@ __device__ __forceinline__ void raise(const WingView& v, int code, int phase, in
     }
 }
 
-// Sense-reversing grid barrier over all resident workgroups of the launch.
-__device__ __forceinline__ void gridSync(const WingView& v) {
-    if (gridDim.x == 1) {
+// Grid barrier over all resident workgroups of the launch (generation
+// counter). One agent-scope release before arriving and one acquire after
+// leaving make every workgroup's earlier global writes visible to every
+// workgroup's later reads; the counter and generation words are touched only
+// with relaxed atomics in between.
+__device__ __forceinline__ void gridSync(const WingView& v, int workers) {
+    if (workers == 1) {
         __syncthreads();
         return;
     }
@@ -97,25 +109,33 @@ __device__ __forceinline__ void gridSync(const WingView& v) {
         unsigned* count = v.barrier;
         unsigned* generation = v.barrier + 32;
         const unsigned gen = __hip_atomic_load(generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gridDim.x - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == static_cast<unsigned>(workers) - 1u) {
+            // Every other workgroup has arrived: reset the counter, then
+            // publish the next generation (release orders the reset first).
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(generation, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            while (__hip_atomic_load(generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
                 __builtin_amdgcn_s_sleep(1);
-            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        __threadfence();
     }
     __syncthreads();
 }
 
-// Node position/inverse-mass store: LDS copies or the HBM arrays.
+// Node position/inverse-mass store: LDS copies or the HBM arrays. A staged
+// segment run uses the same interface over its run-local slots.
 struct NodeRefs {
     double *x, *y, *z, *w;
+    const double* hangLocal;  // harness points in the payload body frame (3 per point)
     __device__ __forceinline__ d3 get(int i) const { return d3{x[i], y[i], z[i]}; }
+    __device__ __forceinline__ d3 hang(int h) const {
+        return d3{hangLocal[3 * h], hangLocal[3 * h + 1], hangLocal[3 * h + 2]};
+    }
+    __device__ __forceinline__ double invMass(int i) const { return w[i]; }
     __device__ __forceinline__ void put(int i, d3 p) const {
         x[i] = p.x;
         y[i] = p.y;
@@ -284,7 +304,7 @@ __device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& 
     const d3 cAdj = cross(a1Adj, edge);
     edgeAdj = add(edgeAdj, cross(a2Adj, toD));
     const d3 dAdj = cross(edge, a2Adj);
-    const double angle = atan2(sine, cosine);
+    const double angle = pm::atan2(sine, cosine);
     const d3 g[4] = {scale(add(add(edgeAdj, cAdj), dAdj), -1.0), edgeAdj, cAdj, dAdj};
     if (!isfinite(angle) || !fin3(g[0]) || !fin3(g[1]) || !fin3(g[2]) || !fin3(g[3])) return;
     double wgrad = 0.0;
@@ -322,18 +342,18 @@ struct Endpoint {
     bool fixed;
 };
 
-__device__ __forceinline__ bool endpointState(const WingView& v, const NodeRefs& N, int e, const Payload& P,
+template <class Store>
+__device__ __forceinline__ bool endpointState(const WingView& v, const Store& N, int e, const Payload& P,
                                               const PayloadDev& def, Endpoint& out) {
     if (e >= 0) {
         out.position = N.get(e);
-        out.invMass = N.w[e];
+        out.invMass = N.invMass(e);
         out.fixed = out.invMass == 0.0;
         out.rigid = false;
         out.offset = d3{0.0, 0.0, 0.0};
         return true;
     }
-    const int h = -e - 1;
-    const d3 local{v.hangLocal[3 * h], v.hangLocal[3 * h + 1], v.hangLocal[3 * h + 2]};
+    const d3 local = N.hang(-e - 1);
     m3 r;
     if (!rotationMatrix(P.q, r)) return false;
     const d3 rel = sub(local, d3{def.comLocal[0], def.comLocal[1], def.comLocal[2]});
@@ -362,52 +382,51 @@ __device__ __forceinline__ bool effectiveInverseMass(const Endpoint& e, bool rig
 }
 
 // One suspension segment of solveLineIteration (suspension.cpp:2197-2274)
-// with projectUnilateralLine (suspension.cpp:1125-1163). Returns false on
-// a LineSolve failure (already raised).
-__device__ bool projectSegment(const WingView& v, const NodeRefs& N, int k, Payload& P, const PayloadDev& def,
-                               const m3& iinv, const StepArgs& a) {
+// with projectUnilateralLine (suspension.cpp:1125-1163): endpoints fe/te
+// (node or run slot of `N`, or a hang-point code), its multiplier updated
+// in place. Returns false on a LineSolve failure (already raised).
+template <class Store>
+__device__ bool projectSegment(const WingView& v, const Store& N, int fe, int te, double rest, double stiffness,
+                               double& lambda, int runtime, Payload& P, const PayloadDev& def, const m3& iinv,
+                               const StepArgs& a) {
     Endpoint from, to;
-    const int fe = v.sfrom[k], te = v.sto[k];
     if (!endpointState(v, N, fe, P, def, from) || !endpointState(v, N, te, P, def, to)) {
-        raise(v, SW_SUSPENSION_ERROR, 1, 2, v.sruntime[k]);
+        raise(v, SW_SUSPENSION_ERROR, 1, 2, runtime);
         return false;
     }
     const d3 diff = sub(to.position, from.position);
     const double length = len(diff);
     const double minLength = def.minLineLength;
     if (!isfinite(length) || length <= minLength) {
-        raise(v, SW_SUSPENSION_ERROR, 6, 1, v.sruntime[k]);
+        raise(v, SW_SUSPENSION_ERROR, 6, 1, runtime);
         return false;
     }
     const d3 dir = divide(diff, length);
     const d3 toGrad = neg(dir);
-    const double rest = v.srest[k];
-    const double stiffness = v.sstiff[k];
-    const double lambda = v.slam[k];
     const double gradLength = len(dir);
     if (!(length > 0.0) || !isfinite(length) || !(rest > 0.0) || !isfinite(rest) || !(stiffness > 0.0) ||
         !isfinite(stiffness) || !(a.dt > 0.0) || !isfinite(a.dt) || !(lambda >= 0.0) || !isfinite(lambda) ||
         fabs(gradLength - 1.0) > 1.0e-10) {
-        raise(v, SW_SUSPENSION_ERROR, 6, 1, v.sruntime[k]);
+        raise(v, SW_SUSPENSION_ERROR, 6, 1, runtime);
         return false;
     }
     double effFrom, effTo;
     if (!effectiveInverseMass(from, from.rigid && !from.fixed, dir, iinv, effFrom) ||
         !effectiveInverseMass(to, to.rigid && !to.fixed, neg(dir), iinv, effTo)) {
-        raise(v, SW_SUSPENSION_ERROR, 6, 1, v.sruntime[k]);
+        raise(v, SW_SUSPENSION_ERROR, 6, 1, runtime);
         return false;
     }
     const double eff = effFrom + effTo;
     const double alphaTilde = 1.0 / (stiffness * a.dt * a.dt);
     if (!(eff + alphaTilde > 0.0)) {
-        raise(v, SW_SUSPENSION_ERROR, 6, 1, v.sruntime[k]);
+        raise(v, SW_SUSPENSION_ERROR, 6, 1, runtime);
         return false;
     }
     const double gap = rest - length;
     const double candidate = lambda + (-gap - alphaTilde * lambda) / (eff + alphaTilde);
     const double next = stdmax(0.0, candidate);
     const double increment = next - lambda;
-    v.slam[k] = next;
+    lambda = next;
     // Corrections: from endpoint first, then to endpoint.
     const Endpoint* ends[2] = {&from, &to};
     const int ids[2] = {fe, te};
@@ -457,6 +476,120 @@ __device__ __forceinline__ m3 loadInverseInertia(const PayloadDev& p) {
     m3 r;
     for (int k = 0; k < 9; ++k) r.a[k] = p.iinv[k];
     return r;
+}
+
+// LDS staging of one serial segment run, in doubles.
+constexpr int kChainHang = 32;  // harness points staged with the run
+constexpr int kChainStageDoubles = 4 * kChainSlots + 3 * kChainItems + 3 * kChainHang + 4 * kChainItems / 2;
+
+// A serial segment run: consecutive single-item dependency levels, i.e.
+// segments that share the payload or a node, in the reference's string-id
+// order (suspension.cpp:2197-2274). The workgroup stages the run's nodes and
+// per-segment values in LDS; lane 0 then projects the segments one after
+// another with the payload pose, its definition and the inverse inertia in
+// registers, so the serial chain waits on no global memory; the workgroup
+// finally writes nodes, multipliers and the payload back. Identical
+// arithmetic to projecting the segments in place.
+__device__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, PayloadDev& payload,
+                         const StepArgs& a, double* stage) {
+    const int slots = P.slotEnd - P.slotBegin;
+    const int items = P.end - P.begin;
+    const bool stageHang = v.hangCount <= kChainHang;
+    double* hang = stage + 4 * kChainSlots;
+    double* lam = hang + 3 * kChainHang;
+    double* rest = lam + kChainItems;
+    double* stiff = rest + kChainItems;
+    int* ends = reinterpret_cast<int*>(stiff + kChainItems);  // from, to
+    const NodeRefs S{stage, stage + kChainSlots, stage + 2 * kChainSlots, stage + 3 * kChainSlots,
+                     stageHang ? hang : v.hangLocal};
+    for (int s = threadIdx.x; s < slots; s += blockDim.x) {
+        const int node = v.runNodes[P.slotBegin + s];
+        S.x[s] = N.x[node];
+        S.y[s] = N.y[node];
+        S.z[s] = N.z[node];
+        S.w[s] = N.w[node];
+    }
+    for (int j = threadIdx.x; j < items; j += blockDim.x) {
+        const int k = P.begin + j;
+        lam[j] = v.slam[k];
+        rest[j] = v.srest[k];
+        stiff[j] = v.sstiff[k];
+        ends[2 * j] = v.sfromSlot[k];
+        ends[2 * j + 1] = v.stoSlot[k];
+    }
+    if (stageHang)
+        for (int h = threadIdx.x; h < 3 * v.hangCount; h += blockDim.x) hang[h] = v.hangLocal[h];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const PayloadDev def = payload;
+        Payload state = loadPayload(def);
+        m3 iinv = loadInverseInertia(def);
+        bool touched = false;
+        for (int j = 0; j < items; ++j) {
+            const int k = P.begin + j;
+            const int fe = ends[2 * j], te = ends[2 * j + 1];
+            double lambda = lam[j];
+            if (!projectSegment(v, S, fe, te, rest[j], stiff[j], lambda, v.sruntime[k], state, def, iinv, a)) break;
+            lam[j] = lambda;
+            if (fe >= 0 && te >= 0) continue;
+            touched = true;
+            if (v.slastPayload[k]) {
+                m3 world, inv;
+                if (!worldInertia(state.q, def.inertia, world) || !inverseM(world, inv)) {
+                    raise(v, SW_SUSPENSION_ERROR, 1, 2, 0);
+                } else {
+                    iinv = inv;
+                    for (int c = 0; c < 9; ++c) payload.iinv[c] = inv.a[c];
+                }
+            }
+        }
+        if (touched) storePayload(payload, state);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < slots; s += blockDim.x) {
+        const int node = v.runNodes[P.slotBegin + s];
+        N.x[node] = S.x[s];
+        N.y[node] = S.y[s];
+        N.z[node] = S.z[s];
+    }
+    for (int j = threadIdx.x; j < items; j += blockDim.x) v.slam[P.begin + j] = lam[j];
+}
+
+// Waits until the handshake counter reaches `target`, then acquires.
+__device__ __forceinline__ void waitCounter(unsigned* counter, unsigned target) {
+    if (threadIdx.x == 0) {
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+            __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+// Publishes `value` on the handshake counter after releasing this
+// workgroup's earlier writes (call after a workgroup barrier).
+__device__ __forceinline__ void publishCounter(unsigned* counter, unsigned value) {
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(counter, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// The dedicated chain workgroup: for every iteration, waits until the
+// worker workgroups have finished the phases before the serial segment run,
+// runs it, and publishes its completion. The workers meanwhile continue with
+// the phases that follow the run and wait for it only before the first phase
+// that touches its nodes or the payload, so the serial chain overlaps the
+// canopy sweeps; the data each side reads is the data the in-order program
+// would give it.
+__device__ void chainWorkgroup(const WingView& v, const NodeRefs& N, PayloadDev& payload, const StepArgs& a,
+                               double* stage) {
+    const Phase P = v.iterPhases[v.chainPhase];
+    for (int it = 0; it < a.iterations; ++it) {
+        waitCounter(v.chainSync, static_cast<unsigned>(it + 1));
+        runChain(v, N, P, payload, a, stage);
+        __syncthreads();
+        publishCounter(v.chainSync + 64, static_cast<unsigned>(it + 1));
+    }
 }
 
 // Contact sphere pair (build-defined): node held off the centre node to the
@@ -531,8 +664,8 @@ __device__ bool finishPayload(PayloadDev& p, const StepArgs& a) {
     const quat q{p.q[0], p.q[1], p.q[2], p.q[3]};
     quat delta;
     if (!canonical(qmul(q, quat{p.prevQ[0], -p.prevQ[1], -p.prevQ[2], -p.prevQ[3]}), delta)) return false;
-    const double half = acos(stdclamp(delta.w, -1.0, 1.0));
-    const double sine = sin(half);
+    const double half = pm::acos(stdclamp(delta.w, -1.0, 1.0));
+    const double sine = pm::sin(half);
     d3 omega;
     if (fabs(sine) < 1.0e-12) {
         omega = scale(d3{delta.x, delta.y, delta.z}, 2.0 / a.dt);
@@ -564,11 +697,13 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
     // would strand the others at the grid barrier.
     if (gridDim.x == 1 && __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int n = v.nodeCount;
+    const bool dedicated = !kShared && a.dedicatedChain != 0 && gridDim.x > 1 && v.chainPhase >= 0;
+    const int workers = dedicated ? static_cast<int>(gridDim.x) - 1 : static_cast<int>(gridDim.x);
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gstride = gridDim.x * blockDim.x;
+    const int gstride = workers * blockDim.x;
     NodeRefs N;
     if (kShared) {
-        N = NodeRefs{lds, lds + n, lds + 2 * n, lds + 3 * n};
+        N = NodeRefs{lds, lds + n, lds + 2 * n, lds + 3 * n, v.hangLocal};
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             N.x[i] = v.px[i];
             N.y[i] = v.py[i];
@@ -577,9 +712,14 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
         }
         __syncthreads();
     } else {
-        N = NodeRefs{v.px, v.py, v.pz, v.w};
+        N = NodeRefs{v.px, v.py, v.pz, v.w, v.hangLocal};
     }
     PayloadDev& payload = *v.payload;
+    double* stage = lds + (kShared ? 4 * n : 0) + kMaxThreads;  // kChainStageDoubles
+    if (dedicated && static_cast<int>(blockIdx.x) == workers) {
+        chainWorkgroup(v, N, payload, a, stage);
+        return;
+    }
 
     // A: prev = x, multiplier resets, payload prediction.
     for (int i = gtid; i < n; i += gstride) {
@@ -606,7 +746,7 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
             !refreshInverseInertia(payload, quat{payload.q[0], payload.q[1], payload.q[2], payload.q[3]}))
             raise(v, SW_SUSPENSION_ERROR, 5, 2, 0);
     }
-    gridSync(v);
+    gridSync(v, workers);
 
     // B: pressure gather from the start-of-substep positions, then predict
     // (soft_body.cpp:1492-1566).
@@ -637,12 +777,24 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
         v.vz[i] = vel.z;
         N.put(i, add(N.get(i), scale(vel, a.dt)));
     }
-    gridSync(v);
+    gridSync(v, workers);
 
     // Constraint iterations.
     for (int it = 0; it < a.iterations; ++it) {
         for (int ph = 0; ph < v.iterPhaseCount; ++ph) {
             const Phase P = v.iterPhases[ph];
+            if (dedicated) {
+                if (ph == v.chainWait) {
+                    const int needed = ph > v.chainPhase ? it + 1 : it;  // this iteration's run, or the last one
+                    if (needed > 0) waitCounter(v.chainSync + 64, static_cast<unsigned>(needed));
+                }
+                if (ph == v.chainPhase) {
+                    // Every worker has passed the barrier after the phases
+                    // before the run: hand it to the chain workgroup.
+                    if (blockIdx.x == 0) publishCounter(v.chainSync, static_cast<unsigned>(it + 1));
+                    continue;
+                }
+            }
             switch (P.type) {
             case kPhaseDistance:
                 for (int k = P.begin + gtid; k < P.end; k += gstride) projectDistance(v, N, k, a.invDt2);
@@ -654,37 +806,40 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
                 for (int k = P.begin + gtid; k < P.end; k += gstride) projectHinge(v, N, k, a.invDt2);
                 break;
             case kPhaseSegments:
-            case kPhaseSegmentRun: {
-                const bool run = P.type == kPhaseSegmentRun;
-                const int first = run ? (gtid == 0 ? P.begin : P.end) : P.begin + gtid;
-                const int step = run ? 1 : gstride;
-                for (int k = first; k < P.end; k += step) {
+                for (int k = P.begin + gtid; k < P.end; k += gstride) {
                     Payload state = loadPayload(payload);
                     const m3 iinv = loadInverseInertia(payload);
-                    const bool touches = v.sfrom[k] < 0 || v.sto[k] < 0;
-                    if (!projectSegment(v, N, k, state, payload, iinv, a)) break;
-                    if (touches) {
+                    const int fe = v.sfrom[k], te = v.sto[k];
+                    double lambda = v.slam[k];
+                    if (!projectSegment(v, N, fe, te, v.srest[k], v.sstiff[k], lambda, v.sruntime[k], state, payload,
+                                        iinv, a))
+                        break;
+                    v.slam[k] = lambda;
+                    if (fe < 0 || te < 0) {
                         storePayload(payload, state);
                         if (v.slastPayload[k] && !refreshInverseInertia(payload, state.q))
                             raise(v, SW_SUSPENSION_ERROR, 1, 2, 0);
                     }
                 }
                 break;
-            }
+            case kPhaseSegmentRun:
+                if (blockIdx.x == 0) runChain(v, N, P, payload, a, stage);
+                break;
             case kPhaseSpheres:
                 for (int k = P.begin + gtid; k < P.end; k += gstride) projectSphere(v, N, k, a.invDt2);
                 break;
             default:
                 break;
             }
-            gridSync(v);
+            gridSync(v, workers);
         }
     }
+    if (dedicated) waitCounter(v.chainSync + 64, static_cast<unsigned>(a.iterations));
     // Closing seam-stitch pass (soft_body.cpp:1389-1405).
     for (int ph = 0; ph < v.closePhaseCount; ++ph) {
         const Phase P = v.closePhases[ph];
         for (int k = P.begin + gtid; k < P.end; k += gstride) projectDistance(v, N, v.stitchItems[k], a.invDt2);
-        gridSync(v);
+        gridSync(v, workers);
     }
 
     // Membrane solver diagnostics (soft_body.cpp:2366-2404), velocity
@@ -747,7 +902,7 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
             const double compliant = v.slam[k] / (v.sstiff[k] * a.dt * a.dt);
             worst = stdmax(worst, fabs(stretch - compliant));
         }
-        double* scratch = kShared ? lds + 4 * n : lds;
+        double* scratch = kShared ? lds + 4 * n : lds;  // kMaxThreads doubles
         scratch[threadIdx.x] = worst;
         __syncthreads();
         for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -756,7 +911,13 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
         }
         if (threadIdx.x == 0) atomicMaxNonNegative(&payload.maxResidual, scratch[0]);
     }
-    gridSync(v);
+    gridSync(v, workers);
+    if (dedicated && gtid == 0) {
+        // The chain workgroup has finished (every worker waited for its last
+        // run before the closing pass): reset the handshake for the next launch.
+        __hip_atomic_store(v.chainSync, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(v.chainSync + 64, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (gtid == 0 && v.segCount > 0) {
         if (!finishPayload(payload, a)) raise(v, SW_SUSPENSION_ERROR, 8, 2, 0);
         else if (payload.maxResidual > a.maxLineResidual) raise(v, SW_SUSPENSION_ERROR, 8, 3, 0);
@@ -968,6 +1129,7 @@ void DeviceWing::build(const Definition& d) {
             if (damping != 0.0)
                 throw std::invalid_argument("axial line damping is not supported by the device suspension sweep");
 
+    std::vector<int> segFromHost, segToHost, sphereNodeHost, sphereCentreHost;  // device slot order
     distance_ = colourDistance(d);
     membrane_ = colourMembranes(d);
     hinge_ = colourDihedrals(d);
@@ -1225,7 +1387,42 @@ void DeviceWing::build(const Definition& d) {
         v.sstiff = deviceCopy(allocations_, stiff);
         v.slastPayload = deviceCopy(allocations_, last);
         // Levels -> phases; consecutive single-item levels merge into one
-        // lane-serial run.
+        // serial run, staged in LDS in chunks of at most kChainItems
+        // segments and kChainSlots distinct nodes.
+        std::vector<int> fromSlot(segments_, 0), toSlot(segments_, 0), runNodes;
+        std::vector<int> nodeSlot(nodes_, -1);
+        const auto addRun = [&](std::size_t b, std::size_t e) {
+            std::size_t c = b;
+            while (c < e) {
+                Phase ph{kPhaseSegmentRun, static_cast<int>(c), 0, 0, static_cast<int>(runNodes.size()), 0};
+                std::size_t j = c;
+                for (; j < e && j - c < static_cast<std::size_t>(kChainItems); ++j) {
+                    int fresh = 0;
+                    for (int endpoint : {from[j], to[j]})
+                        if (endpoint >= 0 && nodeSlot[endpoint] < 0) ++fresh;
+                    if (from[j] >= 0 && from[j] == to[j] && nodeSlot[from[j]] < 0) fresh = 1;
+                    if (runNodes.size() - ph.slotBegin + fresh > static_cast<std::size_t>(kChainSlots)) break;
+                    int* out[2] = {&fromSlot[j], &toSlot[j]};
+                    const int ends[2] = {from[j], to[j]};
+                    for (int q = 0; q < 2; ++q) {
+                        if (ends[q] < 0) {
+                            *out[q] = ends[q];
+                            continue;
+                        }
+                        if (nodeSlot[ends[q]] < 0) {
+                            nodeSlot[ends[q]] = static_cast<int>(runNodes.size() - ph.slotBegin);
+                            runNodes.push_back(ends[q]);
+                        }
+                        *out[q] = nodeSlot[ends[q]];
+                    }
+                }
+                ph.end = static_cast<int>(j);
+                ph.slotEnd = static_cast<int>(runNodes.size());
+                for (int q = ph.slotBegin; q < ph.slotEnd; ++q) nodeSlot[runNodes[q]] = -1;
+                iterPhases_.push_back(ph);
+                c = j;
+            }
+        };
         std::size_t s = 0;
         while (s < segments_) {
             std::size_t e = s;
@@ -1238,14 +1435,19 @@ void DeviceWing::build(const Definition& d) {
                     if (r2 - r != 1) break;
                     r = r2;
                 }
-                iterPhases_.push_back({r - s > 1 ? kPhaseSegmentRun : kPhaseSegments, static_cast<int>(s),
-                                       static_cast<int>(r), 0});
+                if (r - s > 1) addRun(s, r);
+                else iterPhases_.push_back({kPhaseSegments, static_cast<int>(s), static_cast<int>(r), 0});
                 s = r;
             } else {
                 iterPhases_.push_back({kPhaseSegments, static_cast<int>(s), static_cast<int>(e), 0});
                 s = e;
             }
         }
+        segFromHost = from;
+        segToHost = to;
+        v.sfromSlot = deviceCopy(allocations_, fromSlot);
+        v.stoSlot = deviceCopy(allocations_, toSlot);
+        v.runNodes = deviceCopy(allocations_, runNodes);
         std::vector<double> hang(3 * layout_.hangOrder.size());
         for (std::size_t r = 0; r < layout_.hangOrder.size(); ++r)
             for (int c = 0; c < 3; ++c) hang[3 * r + c] = d.suspension->harnessLocal[3 * layout_.hangOrder[r] + c];
@@ -1275,8 +1477,10 @@ void DeviceWing::build(const Definition& d) {
         sphereDevicePosition_.assign(slots.size(), 0);
         for (std::size_t s = 0; s < slots.size(); ++s) sphereDevicePosition_[slots[s]] = static_cast<std::uint32_t>(s);
         v.pairCount = static_cast<int>(slots.size());
-        v.spNode = deviceCopy(allocations_, permute(node, slots));
-        v.spCentre = deviceCopy(allocations_, permute(centre, slots));
+        sphereNodeHost = permute(node, slots);
+        sphereCentreHost = permute(centre, slots);
+        v.spNode = deviceCopy(allocations_, sphereNodeHost);
+        v.spCentre = deviceCopy(allocations_, sphereCentreHost);
         v.spRadius = deviceCopy(allocations_, permute(radius, slots));
         v.spComp = deviceCopy(allocations_, permute(comp, slots));
         std::size_t s = 0;
@@ -1315,6 +1519,75 @@ void DeviceWing::build(const Definition& d) {
             closePhases_.push_back({kPhaseStitches, static_cast<int>(s), static_cast<int>(e), 0});
             s = e;
         }
+    }
+    // Dedicated chain workgroup: with exactly one serial segment run per
+    // iteration, the run may execute on its own workgroup while the others go
+    // on with the phases after it, up to the first phase (cyclically) that
+    // touches one of its nodes or the payload.
+    chainPhase_ = -1;
+    v.chainPhase = -1;
+    v.chainWait = -1;
+    {
+        int runs = 0, runIndex = -1;
+        for (std::size_t q = 0; q < iterPhases_.size(); ++q)
+            if (iterPhases_[q].type == kPhaseSegmentRun) {
+                ++runs;
+                runIndex = static_cast<int>(q);
+            }
+        if (runs == 1) {
+            std::vector<char> mark(nodes_, 0);
+            const Phase& run = iterPhases_[runIndex];
+            for (int k = run.begin; k < run.end; ++k)
+                for (int e : {segFromHost[k], segToHost[k]})
+                    if (e >= 0) mark[e] = 1;
+            const auto hit = [&](std::uint64_t node) { return mark[node] != 0; };
+            const auto touches = [&](const Phase& P) {
+                for (int k = P.begin; k < P.end; ++k) {
+                    switch (P.type) {
+                    case kPhaseDistance: {
+                        const std::size_t c = distance_.order[k];
+                        if (hit(d.constraintNodes[2 * c]) || hit(d.constraintNodes[2 * c + 1])) return true;
+                        break;
+                    }
+                    case kPhaseMembrane: {
+                        const std::size_t t = d.membraneTriangle[membrane_.order[k]];
+                        for (int c = 0; c < 3; ++c)
+                            if (hit(d.triangleNodes[3 * t + c])) return true;
+                        break;
+                    }
+                    case kPhaseHinge: {
+                        const std::size_t h = hinge_.order[k];
+                        for (int c = 0; c < 4; ++c)
+                            if (hit(d.dihedralNodes[4 * h + c])) return true;
+                        break;
+                    }
+                    case kPhaseSegments:
+                    case kPhaseSegmentRun:
+                        for (int e : {segFromHost[k], segToHost[k]})
+                            if (e < 0 || hit(static_cast<std::uint64_t>(e))) return true;
+                        break;
+                    case kPhaseSpheres:
+                        if (hit(sphereNodeHost[k]) || hit(sphereCentreHost[k])) return true;
+                        break;
+                    default:
+                        return true;
+                    }
+                }
+                return false;
+            };
+            int wait = -1;
+            for (std::size_t step = 1; step < iterPhases_.size(); ++step) {
+                const std::size_t q = (runIndex + step) % iterPhases_.size();
+                if (touches(iterPhases_[q])) {
+                    wait = static_cast<int>(q);
+                    break;
+                }
+            }
+            chainPhase_ = runIndex;
+            v.chainPhase = runIndex;
+            v.chainWait = wait;
+        }
+        v.chainSync = deviceCopy(allocations_, std::vector<unsigned>(128, 0u));
     }
     v.iterPhases = deviceCopy(allocations_, iterPhases_);
     v.iterPhaseCount = static_cast<int>(iterPhases_.size());
@@ -1360,14 +1633,21 @@ void DeviceWing::build(const Definition& d) {
     // Launch shape: one workgroup with LDS-resident nodes when they fit.
     int maxShared = 0;
     check(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device_), "attribute");
-    const std::size_t ldsNeed = (4 * nodes_ + kMaxThreads) * sizeof(double);
+    const std::size_t ldsNeed = (4 * nodes_ + kMaxThreads + kChainStageDoubles) * sizeof(double);
     sharedNodes_ = ldsNeed <= static_cast<std::size_t>(maxShared);
     workgroups_ = 1;
-    threads_ = 512;
+    threads_ = kMaxThreads;
     if (sharedNodes_) {
         check(hipFuncSetAttribute(reinterpret_cast<const void*>(&substepKernel<true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ldsNeed)),
               "shared memory attribute");
+    } else {
+        // Nodes stay in HBM: a cooperative grid of one workgroup per compute
+        // unit, capped where the grid barrier starts to cost more than the
+        // extra lanes save.
+        int cus = 0;
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
+        workgroups_ = std::max(1, std::min(cus, 64));
     }
     check(hipMalloc(&partials_, 64 * sizeof(Partial)), "diagnostics partials");
     allocations_.push_back(partials_);
@@ -1377,7 +1657,7 @@ void DeviceWing::build(const Definition& d) {
 
 void DeviceWing::setLaunch(int workgroups, int threads) {
     if (workgroups < 1 || threads < 64 || threads > kMaxThreads || (threads & (threads - 1)) != 0)
-        throw std::invalid_argument("launch shape must be >= 1 workgroup of 64..1024 (power of two) threads");
+        throw std::invalid_argument("launch shape must be >= 1 workgroup of 64..256 (power of two) threads");
     int cus = 0;
     check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
     if (workgroups > cus) throw std::invalid_argument("launch shape exceeds one workgroup per compute unit");
@@ -1385,7 +1665,7 @@ void DeviceWing::setLaunch(int workgroups, int threads) {
     threads_ = threads;
     int maxShared = 0;
     check(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device_), "attribute");
-    const std::size_t ldsNeed = (4 * nodes_ + kMaxThreads) * sizeof(double);
+    const std::size_t ldsNeed = (4 * nodes_ + kMaxThreads + kChainStageDoubles) * sizeof(double);
     sharedNodes_ = workgroups == 1 && ldsNeed <= static_cast<std::size_t>(maxShared);
     if (sharedNodes_) {
         check(hipFuncSetAttribute(reinterpret_cast<const void*>(&substepKernel<true>),
@@ -1413,8 +1693,10 @@ void DeviceWing::setFacePressure(const std::vector<double>& pressure) {
     check(hipStreamSynchronize(stream_), "pressure");
 }
 
-void DeviceWing::launchSubstep(const StepArgs& args) {
+void DeviceWing::launchSubstep(const StepArgs& stepArgs) {
     const WingView& v = *view_;
+    StepArgs args = stepArgs;
+    args.dedicatedChain = (!sharedNodes_ && workgroups_ > 1 && chainPhase_ >= 0) ? 1 : 0;
     if (timing_) {
         while (events_.size() < eventsUsed_ + 2) {
             hipEvent_t e;
@@ -1423,7 +1705,7 @@ void DeviceWing::launchSubstep(const StepArgs& args) {
         }
         check(hipEventRecord(events_[eventsUsed_], stream_), "hipEventRecord");
     }
-    const std::size_t scratch = kMaxThreads * sizeof(double);
+    const std::size_t scratch = (kMaxThreads + kChainStageDoubles) * sizeof(double);
     if (sharedNodes_) {
         const std::size_t bytes = 4 * nodes_ * sizeof(double) + scratch;
         hipLaunchKernelGGL(substepKernel<true>, dim3(1), dim3(threads_), bytes, stream_, v, args);
@@ -1625,11 +1907,11 @@ __global__ void mathKernel(int function, const double* a, const double* b, doubl
     switch (function) {
     case 0: r = sqrt(x); break;
     case 1: r = x / y; break;
-    case 2: r = atan2(x, y); break;
+    case 2: r = pm::atan2(x, y); break;
     case 3: r = remainder(x, 6.283185307179586476925286766559); break;
-    case 4: r = sin(x); break;
-    case 5: r = cos(x); break;
-    case 6: r = acos(x); break;
+    case 4: r = pm::sin(x); break;
+    case 5: r = pm::cos(x); break;
+    case 6: r = pm::acos(x); break;
     default: r = 0.0; break;
     }
     out[i] = r;

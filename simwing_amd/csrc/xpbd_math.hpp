@@ -3,8 +3,10 @@
 // Every expression keeps the reference's operation order and association
 // (compiled with -ffp-contract=off, no fast-math), so each projection is
 // bit-identical to the reference's CPU projection given identical inputs:
-// +, -, *, / and sqrt are correctly rounded on both sides. Transcendentals
-// (atan2, remainder, sin, cos, acos) are the device math library's.
+// +, -, *, / and sqrt are correctly rounded on both sides. The
+// transcendentals atan2, sin, cos and acos are the portable double-double
+// functions of portable_math.hpp (correctly rounded in practice, identical
+// bits on host and device); remainder is exact.
 //
 // Vector helpers follow src/softwing/include/softwing/vec3.h (dot and cross
 // left to right, length = sqrt(dot)); quaternion and 3x3 helpers follow
@@ -12,6 +14,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include "portable_math.hpp"
 
 namespace swx {
 namespace dev {
@@ -264,8 +268,10 @@ __device__ __forceinline__ bool rotationIncrement(d3 v, quat& out) {
     const double angle = len(v);
     if (angle < 1.0e-12) return canonical(quat{1.0, 0.5 * v.x, 0.5 * v.y, 0.5 * v.z}, out);
     const double half = 0.5 * angle;
-    const double s = sin(half) / angle;
-    out = quat{cos(half), v.x * s, v.y * s, v.z * s};
+    double sine, cosine;
+    pm::sincos(half, &sine, &cosine);
+    const double s = sine / angle;
+    out = quat{cosine, v.x * s, v.y * s, v.z * s};
     return true;
 }
 

@@ -11,7 +11,9 @@ wings, and writes into tests/golden/:
 * ``<preset>.states.swfx``    -- full solver state after the listed steps;
 * ``export.digest.swfx``      -- for the ~70k-triangle wing: SHA-256 digests
   of every assembly and colouring array, and strided samples plus digests of
-  the state after 240 steps (the full arrays are too large to commit).
+  the state after 1 and 240 steps (the full arrays are too large to commit;
+  dihedral arrays are in the reference's hinge insertion order, which is the
+  build's colour-major order).
 
 Usage: python tests/golden/make_fixtures.py [preset ...]
 """
@@ -35,7 +37,7 @@ WORK = ROOT / "_work" / "fixtures"
 PLAN = {
     "tiny": {"steps": 20, "snapshots": [1, 5, 20], "workers": 12},
     "small": {"steps": 120, "snapshots": [1, 120], "workers": 12},
-    "export": {"steps": 240, "snapshots": [240], "workers": 8},
+    "export": {"steps": 240, "snapshots": [1, 240], "workers": 8},
 }
 STRIDE = 101
 
@@ -66,19 +68,20 @@ def run(preset: str) -> None:
     for name, value in {**assembly, **colouring}.items():
         out[f"digest/{name}"] = digest(value)
         out[f"size/{name}"] = np.array([value.size], np.uint64)
-    tag = f"step{plan['snapshots'][-1]}"
-    for key in ("position", "previous", "velocity", "membrane_residual", "membrane_lambda", "dihedral_lambda",
-                "constraint_lambda"):
-        value = states[f"{tag}/{key}"]
-        out[f"{tag}/{key}/digest"] = digest(value)
-        width = 3 if key in ("position", "previous", "velocity", "membrane_lambda") else 1
-        rows = value.reshape(-1, width)
-        picked = np.arange(0, rows.shape[0], STRIDE)
-        out[f"{tag}/{key}/rows"] = picked.astype(np.uint64)
-        out[f"{tag}/{key}/samples"] = rows[picked].reshape(-1).copy()
-        out[f"{tag}/{key}/maxabs"] = np.array([np.abs(value).max()])
-    for key in ("segment_lambda", "payload", "suspension_residual"):
-        out[f"{tag}/{key}"] = states[f"{tag}/{key}"]
+    for step in plan["snapshots"]:
+        tag = f"step{step}"
+        for key in ("position", "previous", "velocity", "membrane_residual", "membrane_lambda", "dihedral_lambda",
+                    "constraint_lambda"):
+            value = states[f"{tag}/{key}"]
+            out[f"{tag}/{key}/digest"] = digest(value)
+            width = 3 if key in ("position", "previous", "velocity", "membrane_lambda") else 1
+            rows = value.reshape(-1, width)
+            picked = np.arange(0, rows.shape[0], STRIDE)
+            out[f"{tag}/{key}/rows"] = picked.astype(np.uint64)
+            out[f"{tag}/{key}/samples"] = rows[picked].reshape(-1).copy()
+            out[f"{tag}/{key}/maxabs"] = np.array([np.abs(value).max()])
+        for key in ("segment_lambda", "payload", "suspension_residual"):
+            out[f"{tag}/{key}"] = states[f"{tag}/{key}"]
     out["dihedral_colour_order_digest"] = digest(assembly["dihedral_colour_order"])
     swfx.write(str(HERE / "export.digest.swfx"), out)
 
