@@ -76,7 +76,8 @@ struct StepArgs {
     double reference[3];
     double maxLineResidual;
     int iterations;
-    int dedicatedChain;  // set at launch: the last workgroup runs the serial segment run
+    int neighbourSync;   // set at launch: point-to-point synchronisation of the iterations
+    int dedicatedChain;  // set at launch: the last workgroup runs the serial segment runs
 };
 
 class DeviceWing {
@@ -151,7 +152,20 @@ private:
     ColourOrder distance_, membrane_, hinge_;
     SuspensionLayout layout_;
     std::vector<Phase> iterPhases_, closePhases_;
-    int chainPhase_ = -1;  // program index of the single serial segment run, or -1
+    // Nodes of each item in device slot order (the payload is node nodes_),
+    // kept on the host for the synchronisation analysis.
+    struct ItemNodes {
+        std::vector<int> dist, memb, hinge;  // 2, 3 and 4 per slot
+        std::vector<int> segFrom, segTo;     // < 0: harness point on the payload
+        std::vector<int> sphereNode, sphereCentre;
+    } items_;
+    void itemAccess(int type, int k, std::vector<int>& reads, std::vector<int>& writes) const;
+    // Point-to-point dependency lists of the iteration program for the
+    // current launch shape.
+    void buildSync();
+    bool hasRun_ = false, neighbourSync_ = false, dedicatedChain_ = false;
+    int participants_ = 0, maxParticipants_ = 0;
+    void* depAlloc_[3] = {nullptr, nullptr, nullptr};
     bool timing_ = false;
     std::vector<hipEvent_t> events_;  // start/stop pairs, grown on demand
     std::size_t eventsUsed_ = 0;

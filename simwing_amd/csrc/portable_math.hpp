@@ -12,6 +12,7 @@
 // Method (table-driven reduction in double-double arithmetic, after Tang,
 // "Table-lookup algorithms for elementary functions", ACM TOMS 1990s, and
 // Dekker/Knuth error-free transformations):
+//   Quotients use one division and a Newton-corrected reciprocal.
 //   atan(t), t in [0, 1]:  atan(t) = atan(c) + atan((t - c) / (1 + t c)),
 //       c = k/64; the reduced argument |u| <= 2^-7 goes through the Maclaurin
 //       series with the u^3 term in double-double.
@@ -43,6 +44,8 @@ constexpr double kPiHi = 0x1.921fb54442d18p+1, kPiLo = 0x1.1a62633145c07p-53;
 constexpr double kPio2Hi = 0x1.921fb54442d18p+0, kPio2Lo = 0x1.1a62633145c07p-54;
 constexpr double kPio4 = 0x1.921fb54442d18p-1, k3Pio4 = 0x1.2d97c7f3321d2p+1;
 constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
+constexpr double kThirdHi = 0x1.5555555555555p-2, kThirdLo = 0x1.5555555555555p-56;
+constexpr double kSixthHi = 0x1.5555555555555p-3, kSixthLo = 0x1.5555555555555p-57;
 // pi/2 = kPio2Part1 + kPio2Part2 + kPio2Part3 + kPio2Part3Tail; parts 1 and 2 carry 33
 // significant bits, so n * part is exact for |n| < 2^20.
 constexpr double kPio2Part1 = 0x1.921fb54400000p+0, kPio2Part2 = 0x1.0b4611a600000p-34;
@@ -218,20 +221,14 @@ SWX_PM dd mulD(dd a, double b) {
     p.lo += a.lo * b;
     return fastTwoSum(p.hi, p.lo);
 }
-SWX_PM dd div(dd a, dd b) {
-    const double q1 = a.hi / b.hi;
-    dd r = sub(a, mulD(b, q1));
-    const double q2 = r.hi / b.hi;
-    r = sub(r, mulD(b, q2));
-    const double q3 = r.hi / b.hi;
-    return addD(fastTwoSum(q1, q2), q3);
+// 1 / b to about 2^-104 relative: one division and a Newton correction.
+SWX_PM dd recip(dd b) {
+    const double r = 1.0 / b.hi;
+    const double e = fma(-b.hi, r, 1.0) - b.lo * r;
+    return fastTwoSum(r, r * e);
 }
-SWX_PM dd divD(dd a, double b) {  // b > 0 exact small integer or any double
-    const double q1 = a.hi / b;
-    const double r = fma(-q1, b, a.hi);  // exact remainder
-    const double q2 = (r + a.lo) / b;
-    return fastTwoSum(q1, q2);
-}
+// a / b to about 2^-103 relative.
+SWX_PM dd quotient(dd a, dd b) { return mul(a, recip(b)); }
 SWX_PM dd sqrtDD(dd a) {  // a >= 0
     if (a.hi <= 0.0) return {0.0, 0.0};
     const double s = sqrt(a.hi);
@@ -248,7 +245,7 @@ SWX_PM dd atanUnit(dd t) {
     dd den = twoProd(t.hi, c);
     den.lo += t.lo * c;
     den = addD(fastTwoSum(den.hi, den.lo), 1.0);
-    const dd u = div(num, den);
+    const dd u = quotient(num, den);
     // atan(u) = u + u * (-u^2/3 + u^4/5 - ... + u^16/17), |u| <= 2^-7.
     const dd u2 = mul(u, u);
     const double s = u2.hi;
@@ -256,7 +253,7 @@ SWX_PM dd atanUnit(dd t) {
         s * s *
         (1.0 / 5.0 -
          s * (1.0 / 7.0 - s * (1.0 / 9.0 - s * (1.0 / 11.0 - s * (1.0 / 13.0 - s * (1.0 / 15.0 - s * (1.0 / 17.0)))))));
-    const dd w = addD(neg(divD(u2, 3.0)), tail);
+    const dd w = addD(neg(mul(u2, dd{kThirdHi, kThirdLo})), tail);
     dd uw = twoProd(u.hi, w.hi);
     uw.lo += u.hi * w.lo + u.lo * w.hi;
     const dd series = add(u, fastTwoSum(uw.hi, uw.lo));
@@ -266,7 +263,7 @@ SWX_PM dd atanUnit(dd t) {
 // atan2 of non-negative double-double operands, both finite and not both zero.
 SWX_PM dd atan2Positive(dd y, dd x) {
     const bool swap = y.hi > x.hi;
-    const dd t = swap ? div(x, y) : div(y, x);
+    const dd t = swap ? quotient(x, y) : quotient(y, x);
     dd r = atanUnit(t);
     if (swap) r = sub(dd{kPio2Hi, kPio2Lo}, r);
     return r;
@@ -341,7 +338,7 @@ SWX_PM void sincos(double x, double* sine, double* cosine) {
     const double s = d2.hi;
     const double sinTail =
         s * s * (1.0 / 120.0 - s * (1.0 / 5040.0 - s * (1.0 / 362880.0 - s * (1.0 / 39916800.0))));
-    const dd ws = addD(neg(divD(d2, 6.0)), sinTail);
+    const dd ws = addD(neg(mul(d2, dd{kSixthHi, kSixthLo})), sinTail);
     dd dw = twoProd(d.hi, ws.hi);
     dw.lo += d.hi * ws.lo + d.lo * ws.hi;
     const dd sinD = add(d, fastTwoSum(dw.hi, dw.lo));

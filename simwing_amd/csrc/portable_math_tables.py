@@ -85,6 +85,10 @@ def block() -> str:
     lines.append(f"constexpr double kPio2Hi = {lit(hi)}, kPio2Lo = {lit(lo)};")
     lines.append(f"constexpr double kPio4 = {lit(float(p / 4))}, k3Pio4 = {lit(float(3 * p / 4))};")
     lines.append(f"constexpr double kTwoOverPi = {lit(float(2 / p))};")
+    hi, lo = dd(Decimal(1) / 3)
+    lines.append(f"constexpr double kThirdHi = {lit(hi)}, kThirdLo = {lit(lo)};")
+    hi, lo = dd(Decimal(1) / 6)
+    lines.append(f"constexpr double kSixthHi = {lit(hi)}, kSixthLo = {lit(lo)};")
     p1 = truncate(float(pio2), 33)
     p2 = truncate(float(pio2 - Decimal(p1)), 33)
     p3 = float(pio2 - Decimal(p1) - Decimal(p2))

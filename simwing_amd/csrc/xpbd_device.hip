@@ -71,12 +71,14 @@ struct WingView {
     int closePhaseCount;
     DeviceError* error;
     unsigned* barrier;
-    // Dedicated chain workgroup: program index of the serial segment run,
-    // the first later phase (cyclically) that touches its nodes or the
-    // payload (-1: none), and the handshake counters (runReady at [0],
-    // chainDone at [64]).
-    int chainPhase, chainWait;
-    unsigned* chainSync;
+    // Point-to-point synchronisation of the iteration program: for
+    // participant p and phase ph, entries [depOffsets[p*m+ph],
+    // depOffsets[p*m+ph+1]) name a participant (depPart) whose progress must
+    // have reached step - depLag + 1 before p starts that step; flags[32*q]
+    // is participant q's progress (steps completed in this launch).
+    int *depOffsets, *depPart, *depLag;
+    unsigned* flags;
+    int hasRun;  // the iteration program holds a serial segment run
     double* state;      // contiguous mutable state block
     double* snapshot;   // copy of `state`
     long stateDoubles;
@@ -92,6 +94,25 @@ __device__ __forceinline__ void raise(const WingView& v, int code, int phase, in
         v.error->kind = kind;
         v.error->index = index;
     }
+}
+
+// Bound on the polls of one wait (each poll is a device-scope round trip
+// plus a short sleep, so this is several seconds). A wait that runs out
+// raises SW_DEVICE_ERROR and lets the kernel finish instead of hanging.
+constexpr unsigned kSpinLimit = 1u << 22;
+
+// Spin bookkeeping shared by every wait: true when this wait has run out
+// (it then flags the failure for all others) or another wait already has.
+// The host clears the flag, the barrier words and the progress words before
+// the next launch.
+__device__ __forceinline__ bool syncTimedOut(const WingView& v, unsigned spins) {
+    unsigned* failed = v.barrier + 64;
+    if (spins > kSpinLimit) {
+        __hip_atomic_store(failed, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        raise(v, SW_DEVICE_ERROR, -1, 3, -1);
+        return true;
+    }
+    return (spins & 255u) == 0 && __hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
 // Grid barrier over all resident workgroups of the launch (generation
@@ -118,8 +139,11 @@ __device__ __forceinline__ void gridSync(const WingView& v, int workers) {
             __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(generation, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            while (__hip_atomic_load(generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+            unsigned spins = 0;
+            while (__hip_atomic_load(generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 __builtin_amdgcn_s_sleep(1);
+                if (syncTimedOut(v, ++spins)) break;
+            }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
     }
@@ -366,11 +390,17 @@ __device__ __forceinline__ bool endpointState(const WingView& v, const Store& N,
 }
 
 // generalizedLineEffectiveInverseMass (suspension.cpp:1104-1123).
+// `iinvFinite` is the finiteness of every entry of `iinv`, which the
+// reference checks on each call; callers evaluate it once per inverse.
+__device__ __forceinline__ bool finiteM(const m3& m) {
+    bool ok = true;
+    for (int k = 0; k < 9; ++k) ok = ok && isfinite(m.a[k]);
+    return ok;
+}
 __device__ __forceinline__ bool effectiveInverseMass(const Endpoint& e, bool rigid, d3 g, const m3& iinv,
-                                                     double& out) {
+                                                     bool iinvFinite, double& out) {
     if (!(e.invMass >= 0.0) || !isfinite(e.invMass) || !fin3(g) || !fin3(e.offset)) return false;
-    for (int k = 0; k < 9; ++k)
-        if (!isfinite(iinv.a[k])) return false;
+    if (!iinvFinite) return false;
     double r = e.invMass;
     if (rigid && e.invMass > 0.0) {
         const d3 rot = cross(e.offset, g);
@@ -388,7 +418,7 @@ __device__ __forceinline__ bool effectiveInverseMass(const Endpoint& e, bool rig
 template <class Store>
 __device__ bool projectSegment(const WingView& v, const Store& N, int fe, int te, double rest, double stiffness,
                                double& lambda, int runtime, Payload& P, const PayloadDev& def, const m3& iinv,
-                               const StepArgs& a) {
+                               bool iinvFinite, const StepArgs& a) {
     Endpoint from, to;
     if (!endpointState(v, N, fe, P, def, from) || !endpointState(v, N, te, P, def, to)) {
         raise(v, SW_SUSPENSION_ERROR, 1, 2, runtime);
@@ -411,8 +441,8 @@ __device__ bool projectSegment(const WingView& v, const Store& N, int fe, int te
         return false;
     }
     double effFrom, effTo;
-    if (!effectiveInverseMass(from, from.rigid && !from.fixed, dir, iinv, effFrom) ||
-        !effectiveInverseMass(to, to.rigid && !to.fixed, neg(dir), iinv, effTo)) {
+    if (!effectiveInverseMass(from, from.rigid && !from.fixed, dir, iinv, iinvFinite, effFrom) ||
+        !effectiveInverseMass(to, to.rigid && !to.fixed, neg(dir), iinv, iinvFinite, effTo)) {
         raise(v, SW_SUSPENSION_ERROR, 6, 1, runtime);
         return false;
     }
@@ -490,7 +520,7 @@ constexpr int kChainStageDoubles = 4 * kChainSlots + 3 * kChainItems + 3 * kChai
 // registers, so the serial chain waits on no global memory; the workgroup
 // finally writes nodes, multipliers and the payload back. Identical
 // arithmetic to projecting the segments in place.
-__device__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, PayloadDev& payload,
+__device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, PayloadDev& payload,
                          const StepArgs& a, double* stage) {
     const int slots = P.slotEnd - P.slotBegin;
     const int items = P.end - P.begin;
@@ -499,7 +529,7 @@ __device__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, P
     double* lam = hang + 3 * kChainHang;
     double* rest = lam + kChainItems;
     double* stiff = rest + kChainItems;
-    int* ends = reinterpret_cast<int*>(stiff + kChainItems);  // from, to
+    int* ends = reinterpret_cast<int*>(stiff + kChainItems);  // from, to, runtime index, last-payload flag
     const NodeRefs S{stage, stage + kChainSlots, stage + 2 * kChainSlots, stage + 3 * kChainSlots,
                      stageHang ? hang : v.hangLocal};
     for (int s = threadIdx.x; s < slots; s += blockDim.x) {
@@ -514,8 +544,10 @@ __device__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, P
         lam[j] = v.slam[k];
         rest[j] = v.srest[k];
         stiff[j] = v.sstiff[k];
-        ends[2 * j] = v.sfromSlot[k];
-        ends[2 * j + 1] = v.stoSlot[k];
+        ends[4 * j] = v.sfromSlot[k];
+        ends[4 * j + 1] = v.stoSlot[k];
+        ends[4 * j + 2] = v.sruntime[k];
+        ends[4 * j + 3] = v.slastPayload[k];
     }
     if (stageHang)
         for (int h = threadIdx.x; h < 3 * v.hangCount; h += blockDim.x) hang[h] = v.hangLocal[h];
@@ -524,21 +556,24 @@ __device__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, P
         const PayloadDev def = payload;
         Payload state = loadPayload(def);
         m3 iinv = loadInverseInertia(def);
+        bool iinvFinite = finiteM(iinv);
         bool touched = false;
         for (int j = 0; j < items; ++j) {
-            const int k = P.begin + j;
-            const int fe = ends[2 * j], te = ends[2 * j + 1];
+            const int fe = ends[4 * j], te = ends[4 * j + 1];
             double lambda = lam[j];
-            if (!projectSegment(v, S, fe, te, rest[j], stiff[j], lambda, v.sruntime[k], state, def, iinv, a)) break;
+            if (!projectSegment(v, S, fe, te, rest[j], stiff[j], lambda, ends[4 * j + 2], state, def, iinv,
+                                iinvFinite, a))
+                break;
             lam[j] = lambda;
             if (fe >= 0 && te >= 0) continue;
             touched = true;
-            if (v.slastPayload[k]) {
+            if (ends[4 * j + 3]) {
                 m3 world, inv;
                 if (!worldInertia(state.q, def.inertia, world) || !inverseM(world, inv)) {
                     raise(v, SW_SUSPENSION_ERROR, 1, 2, 0);
                 } else {
                     iinv = inv;
+                    iinvFinite = finiteM(iinv);
                     for (int c = 0; c < 9; ++c) payload.iinv[c] = inv.a[c];
                 }
             }
@@ -555,40 +590,34 @@ __device__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, P
     for (int j = threadIdx.x; j < items; j += blockDim.x) v.slam[P.begin + j] = lam[j];
 }
 
-// Waits until the handshake counter reaches `target`, then acquires.
-__device__ __forceinline__ void waitCounter(unsigned* counter, unsigned target) {
-    if (threadIdx.x == 0) {
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
-            __builtin_amdgcn_s_sleep(1);
+// Waits for the progress words named by dependency entries [e0, e1) of
+// (participant, phase) `index` at `step`, then acquires. Lanes of the first
+// wave poll one entry each.
+__device__ __forceinline__ void waitDeps(const WingView& v, int index, int step) {
+    const int e0 = v.depOffsets[index], e1 = v.depOffsets[index + 1];
+    if (e0 == e1) return;
+    if (threadIdx.x < 64) {
+        for (int e = e0 + static_cast<int>(threadIdx.x); e < e1; e += 64) {
+            const int need = step - v.depLag[e] + 1;
+            if (need <= 0) continue;
+            unsigned* flag = v.flags + 32 * v.depPart[e];
+            unsigned spins = 0;
+            while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < static_cast<unsigned>(need)) {
+                __builtin_amdgcn_s_sleep(1);
+                if (syncTimedOut(v, ++spins)) break;
+            }
+        }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
 
-// Publishes `value` on the handshake counter after releasing this
-// workgroup's earlier writes (call after a workgroup barrier).
+// Publishes `value` on a progress word after releasing this workgroup's
+// earlier writes (call after a workgroup barrier).
 __device__ __forceinline__ void publishCounter(unsigned* counter, unsigned value) {
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_store(counter, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
-// The dedicated chain workgroup: for every iteration, waits until the
-// worker workgroups have finished the phases before the serial segment run,
-// runs it, and publishes its completion. The workers meanwhile continue with
-// the phases that follow the run and wait for it only before the first phase
-// that touches its nodes or the payload, so the serial chain overlaps the
-// canopy sweeps; the data each side reads is the data the in-order program
-// would give it.
-__device__ void chainWorkgroup(const WingView& v, const NodeRefs& N, PayloadDev& payload, const StepArgs& a,
-                               double* stage) {
-    const Phase P = v.iterPhases[v.chainPhase];
-    for (int it = 0; it < a.iterations; ++it) {
-        waitCounter(v.chainSync, static_cast<unsigned>(it + 1));
-        runChain(v, N, P, payload, a, stage);
-        __syncthreads();
-        publishCounter(v.chainSync + 64, static_cast<unsigned>(it + 1));
     }
 }
 
@@ -688,6 +717,49 @@ __device__ __forceinline__ void atomicMaxNonNegative(double* address, double val
     atomicMax(reinterpret_cast<unsigned long long*>(address), static_cast<unsigned long long>(__double_as_longlong(value)));
 }
 
+// Items [first, end) with the given stride of one phase of the iteration
+// program; a serial run (kPhaseSegmentRun) is staged and projected by the
+// calling workgroup as a whole.
+__device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, int first, int end, int stride,
+                         PayloadDev& payload, const StepArgs& a, double* stage) {
+    switch (P.type) {
+    case kPhaseDistance:
+        for (int k = first; k < end; k += stride) projectDistance(v, N, k, a.invDt2);
+        break;
+    case kPhaseMembrane:
+        for (int k = first; k < end; k += stride) projectMembrane(v, N, k, a);
+        break;
+    case kPhaseHinge:
+        for (int k = first; k < end; k += stride) projectHinge(v, N, k, a.invDt2);
+        break;
+    case kPhaseSegments:
+        for (int k = first; k < end; k += stride) {
+            Payload state = loadPayload(payload);
+            const m3 iinv = loadInverseInertia(payload);
+            const int fe = v.sfrom[k], te = v.sto[k];
+            double lambda = v.slam[k];
+            if (!projectSegment(v, N, fe, te, v.srest[k], v.sstiff[k], lambda, v.sruntime[k], state, payload, iinv,
+                                finiteM(iinv), a))
+                break;
+            v.slam[k] = lambda;
+            if (fe < 0 || te < 0) {
+                storePayload(payload, state);
+                if (v.slastPayload[k] && !refreshInverseInertia(payload, state.q))
+                    raise(v, SW_SUSPENSION_ERROR, 1, 2, 0);
+            }
+        }
+        break;
+    case kPhaseSegmentRun:
+        runChain(v, N, P, payload, a, stage);
+        break;
+    case kPhaseSpheres:
+        for (int k = first; k < end; k += stride) projectSphere(v, N, k, a.invDt2);
+        break;
+    default:
+        break;
+    }
+}
+
 template <bool kShared>
 __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArgs a) {
     extern __shared__ double lds[];
@@ -697,9 +769,12 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
     // would strand the others at the grid barrier.
     if (gridDim.x == 1 && __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int n = v.nodeCount;
-    const bool dedicated = !kShared && a.dedicatedChain != 0 && gridDim.x > 1 && v.chainPhase >= 0;
+    const bool dedicated = !kShared && a.neighbourSync != 0 && a.dedicatedChain != 0 && gridDim.x > 1 && v.hasRun;
     const int workers = dedicated ? static_cast<int>(gridDim.x) - 1 : static_cast<int>(gridDim.x);
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    // The chain workgroup takes part in the grid barriers around the
+    // iterations but owns no item of the other passes.
+    const bool chainOnly = dedicated && static_cast<int>(blockIdx.x) == workers;
+    const int gtid = chainOnly ? 0x3fffffff : static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int gstride = workers * blockDim.x;
     NodeRefs N;
     if (kShared) {
@@ -716,10 +791,6 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
     }
     PayloadDev& payload = *v.payload;
     double* stage = lds + (kShared ? 4 * n : 0) + kMaxThreads;  // kChainStageDoubles
-    if (dedicated && static_cast<int>(blockIdx.x) == workers) {
-        chainWorkgroup(v, N, payload, a, stage);
-        return;
-    }
 
     // A: prev = x, multiplier resets, payload prediction.
     for (int i = gtid; i < n; i += gstride) {
@@ -746,7 +817,7 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
             !refreshInverseInertia(payload, quat{payload.q[0], payload.q[1], payload.q[2], payload.q[3]}))
             raise(v, SW_SUSPENSION_ERROR, 5, 2, 0);
     }
-    gridSync(v, workers);
+    gridSync(v, static_cast<int>(gridDim.x));
 
     // B: pressure gather from the start-of-substep positions, then predict
     // (soft_body.cpp:1492-1566).
@@ -777,64 +848,55 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
         v.vz[i] = vel.z;
         N.put(i, add(N.get(i), scale(vel, a.dt)));
     }
-    gridSync(v, workers);
+    gridSync(v, static_cast<int>(gridDim.x));
 
     // Constraint iterations.
-    for (int it = 0; it < a.iterations; ++it) {
-        for (int ph = 0; ph < v.iterPhaseCount; ++ph) {
-            const Phase P = v.iterPhases[ph];
-            if (dedicated) {
-                if (ph == v.chainWait) {
-                    const int needed = ph > v.chainPhase ? it + 1 : it;  // this iteration's run, or the last one
-                    if (needed > 0) waitCounter(v.chainSync + 64, static_cast<unsigned>(needed));
+    if (a.neighbourSync) {
+        // Point-to-point synchronisation: workgroup p owns a contiguous share
+        // of every phase's items (the dedicated chain workgroup owns the
+        // serial runs). Before a phase it waits only for the workgroups whose
+        // earlier accesses to its nodes the in-order program puts first
+        // (host-computed), and after it publishes its progress.
+        const int m = v.iterPhaseCount;
+        const int p = static_cast<int>(blockIdx.x);
+        const bool chainGroup = chainOnly;
+        for (int it = 0; it < a.iterations; ++it) {
+            for (int ph = 0; ph < m; ++ph) {
+                const Phase P = v.iterPhases[ph];
+                int b = P.begin, e = P.begin;
+                if (P.type == kPhaseSegmentRun) {
+                    if (p == (dedicated ? workers : 0)) e = P.end;
+                } else if (!chainGroup) {
+                    const long count = P.end - P.begin;
+                    b = P.begin + static_cast<int>(count * p / workers);
+                    e = P.begin + static_cast<int>(count * (p + 1) / workers);
                 }
-                if (ph == v.chainPhase) {
-                    // Every worker has passed the barrier after the phases
-                    // before the run: hand it to the chain workgroup.
-                    if (blockIdx.x == 0) publishCounter(v.chainSync, static_cast<unsigned>(it + 1));
-                    continue;
-                }
+                if (b >= e) continue;
+                const int step = it * m + ph;
+                waitDeps(v, p * m + ph, step);
+                runPhase(v, N, P, b + static_cast<int>(threadIdx.x), e, static_cast<int>(blockDim.x), payload, a,
+                         stage);
+                __syncthreads();
+                publishCounter(v.flags + 32 * p, static_cast<unsigned>(step + 1));
             }
-            switch (P.type) {
-            case kPhaseDistance:
-                for (int k = P.begin + gtid; k < P.end; k += gstride) projectDistance(v, N, k, a.invDt2);
-                break;
-            case kPhaseMembrane:
-                for (int k = P.begin + gtid; k < P.end; k += gstride) projectMembrane(v, N, k, a);
-                break;
-            case kPhaseHinge:
-                for (int k = P.begin + gtid; k < P.end; k += gstride) projectHinge(v, N, k, a.invDt2);
-                break;
-            case kPhaseSegments:
-                for (int k = P.begin + gtid; k < P.end; k += gstride) {
-                    Payload state = loadPayload(payload);
-                    const m3 iinv = loadInverseInertia(payload);
-                    const int fe = v.sfrom[k], te = v.sto[k];
-                    double lambda = v.slam[k];
-                    if (!projectSegment(v, N, fe, te, v.srest[k], v.sstiff[k], lambda, v.sruntime[k], state, payload,
-                                        iinv, a))
-                        break;
-                    v.slam[k] = lambda;
-                    if (fe < 0 || te < 0) {
-                        storePayload(payload, state);
-                        if (v.slastPayload[k] && !refreshInverseInertia(payload, state.q))
-                            raise(v, SW_SUSPENSION_ERROR, 1, 2, 0);
-                    }
-                }
-                break;
-            case kPhaseSegmentRun:
-                if (blockIdx.x == 0) runChain(v, N, P, payload, a, stage);
-                break;
-            case kPhaseSpheres:
-                for (int k = P.begin + gtid; k < P.end; k += gstride) projectSphere(v, N, k, a.invDt2);
-                break;
-            default:
-                break;
-            }
-            gridSync(v, workers);
         }
+        gridSync(v, static_cast<int>(gridDim.x));
+        // Every participant has left the loop: nobody polls the progress
+        // words any more, so each resets its own for the next launch.
+        if (threadIdx.x == 0) __hip_atomic_store(v.flags + 32 * p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (chainGroup) return;
+    } else {
+        for (int it = 0; it < a.iterations; ++it)
+            for (int ph = 0; ph < v.iterPhaseCount; ++ph) {
+                const Phase P = v.iterPhases[ph];
+                if (P.type == kPhaseSegmentRun) {
+                    if (blockIdx.x == 0) runPhase(v, N, P, P.begin, P.end, 1, payload, a, stage);
+                } else {
+                    runPhase(v, N, P, P.begin + gtid, P.end, gstride, payload, a, stage);
+                }
+                gridSync(v, workers);
+            }
     }
-    if (dedicated) waitCounter(v.chainSync + 64, static_cast<unsigned>(a.iterations));
     // Closing seam-stitch pass (soft_body.cpp:1389-1405).
     for (int ph = 0; ph < v.closePhaseCount; ++ph) {
         const Phase P = v.closePhases[ph];
@@ -912,12 +974,6 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
         if (threadIdx.x == 0) atomicMaxNonNegative(&payload.maxResidual, scratch[0]);
     }
     gridSync(v, workers);
-    if (dedicated && gtid == 0) {
-        // The chain workgroup has finished (every worker waited for its last
-        // run before the closing pass): reset the handshake for the next launch.
-        __hip_atomic_store(v.chainSync, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(v.chainSync + 64, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     if (gtid == 0 && v.segCount > 0) {
         if (!finishPayload(payload, a)) raise(v, SW_SUSPENSION_ERROR, 8, 2, 0);
         else if (payload.maxResidual > a.maxLineResidual) raise(v, SW_SUSPENSION_ERROR, 8, 3, 0);
@@ -1081,6 +1137,8 @@ DeviceWing::~DeviceWing() {
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : allocations_) (void)hipFree(p);
+    for (void* p : depAlloc_)
+        if (p) (void)hipFree(p);
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -1129,10 +1187,18 @@ void DeviceWing::build(const Definition& d) {
             if (damping != 0.0)
                 throw std::invalid_argument("axial line damping is not supported by the device suspension sweep");
 
-    std::vector<int> segFromHost, segToHost, sphereNodeHost, sphereCentreHost;  // device slot order
     distance_ = colourDistance(d);
     membrane_ = colourMembranes(d);
     hinge_ = colourDihedrals(d);
+    items_ = ItemNodes{};
+    for (std::size_t k = 0; k < distance_.order.size(); ++k)
+        for (int c = 0; c < 2; ++c) items_.dist.push_back(static_cast<int>(d.constraintNodes[2 * distance_.order[k] + c]));
+    for (std::size_t k = 0; k < membrane_.order.size(); ++k) {
+        const std::size_t t = d.membraneTriangle[membrane_.order[k]];
+        for (int c = 0; c < 3; ++c) items_.memb.push_back(static_cast<int>(d.triangleNodes[3 * t + c]));
+    }
+    for (std::size_t k = 0; k < hinge_.order.size(); ++k)
+        for (int c = 0; c < 4; ++c) items_.hinge.push_back(static_cast<int>(d.dihedralNodes[4 * hinge_.order[k] + c]));
     layout_ = suspensionLayout(d);
     segments_ = layout_.segmentOrder.size();
 
@@ -1443,8 +1509,8 @@ void DeviceWing::build(const Definition& d) {
                 s = e;
             }
         }
-        segFromHost = from;
-        segToHost = to;
+        items_.segFrom = from;
+        items_.segTo = to;
         v.sfromSlot = deviceCopy(allocations_, fromSlot);
         v.stoSlot = deviceCopy(allocations_, toSlot);
         v.runNodes = deviceCopy(allocations_, runNodes);
@@ -1477,10 +1543,10 @@ void DeviceWing::build(const Definition& d) {
         sphereDevicePosition_.assign(slots.size(), 0);
         for (std::size_t s = 0; s < slots.size(); ++s) sphereDevicePosition_[slots[s]] = static_cast<std::uint32_t>(s);
         v.pairCount = static_cast<int>(slots.size());
-        sphereNodeHost = permute(node, slots);
-        sphereCentreHost = permute(centre, slots);
-        v.spNode = deviceCopy(allocations_, sphereNodeHost);
-        v.spCentre = deviceCopy(allocations_, sphereCentreHost);
+        items_.sphereNode = permute(node, slots);
+        items_.sphereCentre = permute(centre, slots);
+        v.spNode = deviceCopy(allocations_, items_.sphereNode);
+        v.spCentre = deviceCopy(allocations_, items_.sphereCentre);
         v.spRadius = deviceCopy(allocations_, permute(radius, slots));
         v.spComp = deviceCopy(allocations_, permute(comp, slots));
         std::size_t s = 0;
@@ -1520,75 +1586,9 @@ void DeviceWing::build(const Definition& d) {
             s = e;
         }
     }
-    // Dedicated chain workgroup: with exactly one serial segment run per
-    // iteration, the run may execute on its own workgroup while the others go
-    // on with the phases after it, up to the first phase (cyclically) that
-    // touches one of its nodes or the payload.
-    chainPhase_ = -1;
-    v.chainPhase = -1;
-    v.chainWait = -1;
-    {
-        int runs = 0, runIndex = -1;
-        for (std::size_t q = 0; q < iterPhases_.size(); ++q)
-            if (iterPhases_[q].type == kPhaseSegmentRun) {
-                ++runs;
-                runIndex = static_cast<int>(q);
-            }
-        if (runs == 1) {
-            std::vector<char> mark(nodes_, 0);
-            const Phase& run = iterPhases_[runIndex];
-            for (int k = run.begin; k < run.end; ++k)
-                for (int e : {segFromHost[k], segToHost[k]})
-                    if (e >= 0) mark[e] = 1;
-            const auto hit = [&](std::uint64_t node) { return mark[node] != 0; };
-            const auto touches = [&](const Phase& P) {
-                for (int k = P.begin; k < P.end; ++k) {
-                    switch (P.type) {
-                    case kPhaseDistance: {
-                        const std::size_t c = distance_.order[k];
-                        if (hit(d.constraintNodes[2 * c]) || hit(d.constraintNodes[2 * c + 1])) return true;
-                        break;
-                    }
-                    case kPhaseMembrane: {
-                        const std::size_t t = d.membraneTriangle[membrane_.order[k]];
-                        for (int c = 0; c < 3; ++c)
-                            if (hit(d.triangleNodes[3 * t + c])) return true;
-                        break;
-                    }
-                    case kPhaseHinge: {
-                        const std::size_t h = hinge_.order[k];
-                        for (int c = 0; c < 4; ++c)
-                            if (hit(d.dihedralNodes[4 * h + c])) return true;
-                        break;
-                    }
-                    case kPhaseSegments:
-                    case kPhaseSegmentRun:
-                        for (int e : {segFromHost[k], segToHost[k]})
-                            if (e < 0 || hit(static_cast<std::uint64_t>(e))) return true;
-                        break;
-                    case kPhaseSpheres:
-                        if (hit(sphereNodeHost[k]) || hit(sphereCentreHost[k])) return true;
-                        break;
-                    default:
-                        return true;
-                    }
-                }
-                return false;
-            };
-            int wait = -1;
-            for (std::size_t step = 1; step < iterPhases_.size(); ++step) {
-                const std::size_t q = (runIndex + step) % iterPhases_.size();
-                if (touches(iterPhases_[q])) {
-                    wait = static_cast<int>(q);
-                    break;
-                }
-            }
-            chainPhase_ = runIndex;
-            v.chainPhase = runIndex;
-            v.chainWait = wait;
-        }
-        v.chainSync = deviceCopy(allocations_, std::vector<unsigned>(128, 0u));
-    }
+    hasRun_ = false;
+    for (const Phase& P : iterPhases_) hasRun_ = hasRun_ || P.type == kPhaseSegmentRun;
+    v.hasRun = hasRun_ ? 1 : 0;
     v.iterPhases = deviceCopy(allocations_, iterPhases_);
     v.iterPhaseCount = static_cast<int>(iterPhases_.size());
     v.closePhases = deviceCopy(allocations_, closePhases_);
@@ -1627,7 +1627,7 @@ void DeviceWing::build(const Definition& d) {
         DeviceError e{};
         std::vector<DeviceError> errors(1, e);
         v.error = deviceCopy(allocations_, errors);
-        std::vector<unsigned> barrier(64, 0);
+        std::vector<unsigned> barrier(128, 0);  // count, generation, sync-failure flag
         v.barrier = deviceCopy(allocations_, barrier);
     }
     // Launch shape: one workgroup with LDS-resident nodes when they fit.
@@ -1649,6 +1649,13 @@ void DeviceWing::build(const Definition& d) {
         check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
         workgroups_ = std::max(1, std::min(cus, 64));
     }
+    {
+        int cus = 0;
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
+        maxParticipants_ = std::max(cus, 1);
+        v.flags = deviceCopy(allocations_, std::vector<unsigned>(32 * static_cast<std::size_t>(maxParticipants_), 0u));
+    }
+    buildSync();
     check(hipMalloc(&partials_, 64 * sizeof(Partial)), "diagnostics partials");
     allocations_.push_back(partials_);
     check(hipStreamSynchronize(stream_), "sync build");
@@ -1672,6 +1679,7 @@ void DeviceWing::setLaunch(int workgroups, int threads) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ldsNeed)),
               "shared memory attribute");
     }
+    buildSync();
 }
 
 void DeviceWing::setExternalForces(const std::vector<double>& forces) {
@@ -1696,7 +1704,8 @@ void DeviceWing::setFacePressure(const std::vector<double>& pressure) {
 void DeviceWing::launchSubstep(const StepArgs& stepArgs) {
     const WingView& v = *view_;
     StepArgs args = stepArgs;
-    args.dedicatedChain = (!sharedNodes_ && workgroups_ > 1 && chainPhase_ >= 0) ? 1 : 0;
+    args.neighbourSync = neighbourSync_ ? 1 : 0;
+    args.dedicatedChain = dedicatedChain_ ? 1 : 0;
     if (timing_) {
         while (events_.size() < eventsUsed_ + 2) {
             hipEvent_t e;
@@ -1736,7 +1745,153 @@ void DeviceWing::restoreSnapshot() {
 }
 
 void DeviceWing::clearError() {
+    // Also resets the synchronisation words (they are zero after every
+    // completed launch, but not after a launch whose waits timed out).
     check(hipMemsetAsync(view_->error, 0, sizeof(DeviceError), stream_), "clear error");
+    check(hipMemsetAsync(view_->barrier, 0, 128 * sizeof(unsigned), stream_), "clear barrier");
+    check(hipMemsetAsync(view_->flags, 0, 32 * sizeof(unsigned) * maxParticipants_, stream_), "clear progress");
+}
+
+void DeviceWing::itemAccess(int type, int k, std::vector<int>& reads, std::vector<int>& writes) const {
+    reads.clear();
+    writes.clear();
+    const int payloadNode = static_cast<int>(nodes_);
+    switch (type) {
+    case kPhaseDistance:
+        writes.assign(items_.dist.begin() + 2 * k, items_.dist.begin() + 2 * k + 2);
+        break;
+    case kPhaseMembrane:
+        writes.assign(items_.memb.begin() + 3 * k, items_.memb.begin() + 3 * k + 3);
+        break;
+    case kPhaseHinge:
+        writes.assign(items_.hinge.begin() + 4 * k, items_.hinge.begin() + 4 * k + 4);
+        break;
+    case kPhaseSegments:
+    case kPhaseSegmentRun:
+        for (int e : {items_.segFrom[k], items_.segTo[k]}) writes.push_back(e >= 0 ? e : payloadNode);
+        break;
+    case kPhaseSpheres:
+        writes.push_back(items_.sphereNode[k]);
+        reads.push_back(items_.sphereCentre[k]);
+        break;
+    default:
+        throw std::logic_error("phase type outside the iteration program");
+    }
+}
+
+// Dependency lists of the point-to-point synchronisation. Workgroup p owns
+// items [n p / W, n (p + 1) / W) of every phase of n items (W workers); the
+// serial runs belong to the dedicated chain workgroup (participant W) when
+// there is one, else to workgroup 0. Replaying two iterations of the program
+// on the host, every access of p at step s to node x (the payload counts as a
+// node) records the last other participant that wrote x and, for a write,
+// every other participant that read x since; p must see their progress
+// reach that step before it starts. The second iteration gives the periodic
+// lists, stored as lags (steps back); lags reaching before the first step
+// refer to the prologue, which a grid barrier already orders.
+void DeviceWing::buildSync() {
+    WingView& v = *view_;
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipStreamSynchronize(stream_), "synchronise before rebuilding the schedule");
+    const int m = static_cast<int>(iterPhases_.size());
+    neighbourSync_ = !sharedNodes_ && workgroups_ > 1 && m > 0 && workgroups_ <= maxParticipants_;
+    dedicatedChain_ = neighbourSync_ && hasRun_ && workgroups_ > 2;
+    for (void*& p : depAlloc_) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    v.depOffsets = v.depPart = v.depLag = nullptr;
+    participants_ = 0;
+    if (!neighbourSync_) return;
+    const int parts = workgroups_;
+    const int workers = dedicatedChain_ ? parts - 1 : parts;
+    const int runOwner = dedicatedChain_ ? workers : 0;
+    struct Access {
+        int p, s;
+    };
+    const std::size_t slots = nodes_ + 1;
+    std::vector<Access> lastWrite(slots, Access{-1, -1});
+    std::vector<std::vector<Access>> readers(slots);
+    std::vector<int> needStep(static_cast<std::size_t>(parts) * m * parts, -1);
+    std::vector<int> reads, writes;
+    struct Pending {
+        int node, p;
+        bool write;
+    };
+    std::vector<Pending> pending;
+    for (int s = 0; s < 2 * m; ++s) {
+        const int ph = s % m;
+        const Phase& P = iterPhases_[ph];
+        pending.clear();
+        for (int p = 0; p < parts; ++p) {
+            int b = P.begin, e = P.begin;
+            if (P.type == kPhaseSegmentRun) {
+                if (p == runOwner) e = P.end;
+            } else if (p < workers) {
+                const long long count = P.end - P.begin;
+                b = P.begin + static_cast<int>(count * p / workers);
+                e = P.begin + static_cast<int>(count * (p + 1) / workers);
+            }
+            int* need = needStep.data() + (static_cast<std::size_t>(p) * m + ph) * parts;
+            const auto depend = [&](const Access& a) {
+                if (s >= m && a.p >= 0 && a.p != p) need[a.p] = std::max(need[a.p], a.s);
+            };
+            for (int k = b; k < e; ++k) {
+                itemAccess(P.type, k, reads, writes);
+                for (int x : writes) {
+                    depend(lastWrite[x]);
+                    for (const Access& r : readers[x]) depend(r);
+                    pending.push_back({x, p, true});
+                }
+                for (int x : reads) {
+                    depend(lastWrite[x]);
+                    pending.push_back({x, p, false});
+                }
+            }
+        }
+        for (const Pending& a : pending) {
+            if (a.write) {
+                lastWrite[a.node] = Access{a.p, s};
+                readers[a.node].clear();
+            }
+        }
+        for (const Pending& a : pending) {
+            if (a.write) continue;
+            auto& list = readers[a.node];
+            bool found = false;
+            for (Access& r : list)
+                if (r.p == a.p) {
+                    r.s = s;
+                    found = true;
+                }
+            if (!found) list.push_back(Access{a.p, s});
+        }
+    }
+    std::vector<int> offsets(static_cast<std::size_t>(parts) * m + 1, 0), part, lag;
+    for (int p = 0; p < parts; ++p)
+        for (int ph = 0; ph < m; ++ph) {
+            const int* need = needStep.data() + (static_cast<std::size_t>(p) * m + ph) * parts;
+            for (int q = 0; q < parts; ++q)
+                if (need[q] >= 0) {
+                    const int l = (m + ph) - need[q];
+                    if (l < 1 || l > 2 * m) throw std::logic_error("synchronisation analysis produced an invalid lag");
+                    part.push_back(q);
+                    lag.push_back(l);
+                }
+            offsets[static_cast<std::size_t>(p) * m + ph + 1] = static_cast<int>(part.size());
+        }
+    const auto upload = [&](const std::vector<int>& host, int index) {
+        void* ptr = nullptr;
+        const std::size_t bytes = std::max<std::size_t>(host.size(), 1) * sizeof(int);
+        check(hipMalloc(&ptr, bytes), "hipMalloc");
+        depAlloc_[index] = ptr;
+        if (!host.empty()) check(hipMemcpy(ptr, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice), "upload");
+        return static_cast<int*>(ptr);
+    };
+    v.depOffsets = upload(offsets, 0);
+    v.depPart = upload(part, 1);
+    v.depLag = upload(lag, 2);
+    participants_ = parts;
 }
 
 DeviceError DeviceWing::readError(double* maxResidual) {
