@@ -113,6 +113,19 @@ def tune(structure, loads, settings, cus: int) -> tuple[tuple[int, int], dict]:
     return (wg, threads), results
 
 
+def measured_traffic(workload: str, launch: str) -> int | None:
+    """HBM bytes per substep-kernel launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (scaled by a copy kernel of known size),
+    when they were taken on this workload and launch shape."""
+    path = ROOT / "profiles" / f"substepKernel_{workload}_counters.json"
+    if not path.exists():
+        return None
+    record = json.loads(path.read_text())
+    if record.get("launch") != launch:
+        return None
+    return int(record["traffic_bytes_per_launch"])
+
+
 def cpu_share() -> int:
     try:
         cores = len(os.sched_getaffinity(0))
@@ -183,6 +196,8 @@ def main() -> int:
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-tune", action="store_true")
     parser.add_argument("--launch", default=None, help="fixed launch shape WGxTHREADS")
+    parser.add_argument("--plain-launch", action="store_true",
+                        help="plain instead of cooperative launches (for profilers that do not follow the latter)")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the structural substep of one wing runs on a single MI355X "
@@ -198,6 +213,8 @@ def main() -> int:
           file=sys.stderr, flush=True)
     structure = sw.Structure(definition)
     structure.set_face_pressure(pressure)
+    if args.plain_launch:
+        structure.set_cooperative(False)
     settings = sw.flight_step_settings(args.iterations)
     cus = 256  # MI355X compute units; shapes beyond the device's limits are skipped by the tuner
 
@@ -268,7 +285,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, f"{shape[0]}x{shape[1]}"),
             "kernel": "substepKernel",
             "kernel_ms": avg_kernel_ms,
             "algorithmic_bytes": algorithmic,
@@ -276,6 +293,7 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    structure.close()  # release device memory before interpreter teardown (profilers hook process exit)
     return 0
 
 

@@ -254,6 +254,13 @@ sw_status sw_device_math(int device, int function, const double* a, const double
 sw_status sw_structure_set_launch(sw_structure* s, int workgroups, int threads);
 sw_status sw_structure_get_launch(const sw_structure* s, int* workgroups, int* threads);
 
+/* Multi-workgroup launches use hipLaunchCooperativeKernel (default, 1) or a
+ * plain launch (0; resident in practice on an idle device, for tools that
+ * do not follow cooperative launches). Every in-kernel wait is bounded and
+ * reports SW_DEVICE_ERROR instead of hanging. No counterpart in the
+ * reference (launch configuration). */
+sw_status sw_structure_set_cooperative(sw_structure* s, int cooperative);
+
 /* Substep-kernel timing (no counterpart in the reference; measurement only).
  * While enabled, each substep kernel launch is bracketed by HIP events on the
  * structure's own stream. sw_structure_kernel_time waits for the stream and

@@ -430,6 +430,9 @@ class Structure:
     def set_launch(self, workgroups: int, threads: int) -> None:
         check(native.library().sw_structure_set_launch(self._handle, workgroups, threads))
 
+    def set_cooperative(self, cooperative: bool) -> None:
+        check(native.library().sw_structure_set_cooperative(self._handle, int(cooperative)))
+
     def set_kernel_timing(self, enabled: bool) -> None:
         check(native.library().sw_structure_set_kernel_timing(self._handle, int(enabled)))
 

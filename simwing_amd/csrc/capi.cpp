@@ -497,6 +497,13 @@ sw_status sw_structure_set_launch(sw_structure* s, int workgroups, int threads) 
     });
 }
 
+sw_status sw_structure_set_cooperative(sw_structure* s, int cooperative) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        s->wing->setCooperative(cooperative != 0);
+    });
+}
+
 sw_status sw_structure_set_kernel_timing(sw_structure* s, int enabled) {
     return guarded([&] {
         requirePointer(s, "structure");

@@ -135,6 +135,12 @@ public:
     void kernelTime(double* milliseconds, long long* launches);
 
     void setLaunch(int workgroups, int threads);
+    // Multi-workgroup launches go through hipLaunchCooperativeKernel (the
+    // runtime checks that every workgroup is resident at once) unless this is
+    // off; then a plain launch is used, which is resident in practice on an
+    // otherwise idle device and lets tools that do not follow cooperative
+    // launches observe the kernel. Every wait in the kernel is bounded.
+    void setCooperative(bool cooperative) { cooperative_ = cooperative; }
     int workgroups() const { return workgroups_; }
     int threads() const { return threads_; }
     bool sharedNodes() const { return sharedNodes_; }
@@ -163,7 +169,7 @@ private:
     // Point-to-point dependency lists of the iteration program for the
     // current launch shape.
     void buildSync();
-    bool hasRun_ = false, neighbourSync_ = false, dedicatedChain_ = false;
+    bool hasRun_ = false, neighbourSync_ = false, dedicatedChain_ = false, cooperative_ = true;
     int participants_ = 0, maxParticipants_ = 0;
     void* depAlloc_[3] = {nullptr, nullptr, nullptr};
     bool timing_ = false;

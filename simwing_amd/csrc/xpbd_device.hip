@@ -1720,6 +1720,8 @@ void DeviceWing::launchSubstep(const StepArgs& stepArgs) {
         hipLaunchKernelGGL(substepKernel<true>, dim3(1), dim3(threads_), bytes, stream_, v, args);
     } else if (workgroups_ == 1) {
         hipLaunchKernelGGL(substepKernel<false>, dim3(1), dim3(threads_), scratch, stream_, v, args);
+    } else if (!cooperative_) {
+        hipLaunchKernelGGL(substepKernel<false>, dim3(workgroups_), dim3(threads_), scratch, stream_, v, args);
     } else {
         void* params[] = {const_cast<WingView*>(&v), const_cast<StepArgs*>(&args)};
         check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&substepKernel<false>), dim3(workgroups_),

@@ -204,6 +204,7 @@ _SIGNATURES = {
     "sw_structure_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "sw_structure_get_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sw_structure_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "sw_structure_set_cooperative": (C.c_int, [C.c_void_p, C.c_int]),
     "sw_structure_kernel_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "sw_scene_generate": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "sw_scene_destroy": (None, [C.c_void_p]),
