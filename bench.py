@@ -178,6 +178,59 @@ def cpu_baseline(workload: str, counts_no_spheres: dict, iterations: int, budget
     }
 
 
+def batched_leg(count: int, steps: int, seed: int = 0x5EED0002) -> dict:
+    """64 wings (node counts log-uniform 2k-40k, span 6-12 m, per-wing
+    compliance, pressure 20-100 Pa and line rest scale 0.97-1.03 from the
+    seed) stepped by one persistent launch, against the same wings stepped
+    one after another with their own launches (one LDS-resident workgroup
+    for wings whose nodes fit, else 64 workgroups)."""
+    import simwing_amd as sw
+    settings = sw.flight_step_settings()
+    t0 = time.perf_counter()
+    cases = [make_workload(f"batch:{seed}:{i}:{count}") for i in range(count)]
+    batch = sw.Batch([c[0] for c in cases])
+    for w, (_, pressure, loads) in enumerate(cases):
+        batch.set_face_pressure(w, pressure)
+        batch.set_external_forces(w, loads)
+    setup = time.perf_counter() - t0
+    batch.step(1, settings)  # first launch
+    for w, (_, _, loads) in enumerate(cases):
+        batch.set_external_forces(w, loads)
+    start = time.perf_counter()
+    batch.step(steps, settings)  # the loads apply to every step of the call
+    batched_s = time.perf_counter() - start
+    blocks = batch.workgroups()
+    # Per-wing launches: each wing alone with its own solo launch shape.
+    solo_steps = 3
+    solo_s = 0.0
+    for w, (d, _, loads) in enumerate(cases):
+        s = batch.structure(w)
+        s.set_launch(1, 256) if blocks[w] == 1 else s.set_launch(64, 256)
+        s.set_external_forces(loads)
+        s.step_many(1, settings)
+        s.set_external_forces(loads)
+        start = time.perf_counter()
+        s.step_many(solo_steps, settings)
+        solo_s += (time.perf_counter() - start) / solo_steps
+    batch.set_launch(8, 256)
+    nodes = [c[0].node_count for c in cases]
+    batched_rate = count * steps / batched_s
+    solo_rate = count / solo_s
+    batch.close()
+    return {
+        "metric": "batched wings/s",
+        "value": batched_rate,
+        "unit": "wing-steps/s",
+        "per_wing_launches": solo_rate,
+        "speedup": batched_rate / solo_rate,
+        "ms_per_batched_step": 1e3 * batched_s / steps,
+        "steps": steps,
+        "workload": (f"{count} wings, {min(nodes)}-{max(nodes)} nodes (total {sum(nodes)}), seed {seed:#x}, "
+                     f"dt 1/120 s, 48 iterations; {int(blocks.sum())} workgroups "
+                     f"({int((blocks == 1).sum())} LDS-resident wings); setup {setup:.1f} s"),
+    }
+
+
 def make_workload(name: str):
     import simwing_amd as sw
     scene = sw.generate_scene(name)
@@ -195,6 +248,9 @@ def main() -> int:
     parser.add_argument("--cpu-seconds", type=float, default=15.0)
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-tune", action="store_true")
+    parser.add_argument("--no-batch", action="store_true")
+    parser.add_argument("--batch-wings", type=int, default=64)
+    parser.add_argument("--batch-steps", type=int, default=120)
     parser.add_argument("--launch", default=None, help="fixed launch shape WGxTHREADS")
     parser.add_argument("--plain-launch", action="store_true",
                         help="plain instead of cooperative launches (for profilers that do not follow the latter)")
@@ -253,6 +309,9 @@ def main() -> int:
     algorithmic = bytes_per_substep(counts, args.iterations, membrane_damped=False)
     achieved = algorithmic / (avg_kernel_ms * 1e-3) / 1e9
 
+    structure.close()
+    batched = None if args.no_batch else batched_leg(args.batch_wings, args.batch_steps)
+
     cpu = None
     if not args.no_cpu:
         no_spheres = dict(counts, sphere_pairs=0)
@@ -291,9 +350,9 @@ def main() -> int:
             "algorithmic_bytes": algorithmic,
         },
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
-    structure.close()  # release device memory before interpreter teardown (profilers hook process exit)
     return 0
 
 

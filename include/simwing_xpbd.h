@@ -311,16 +311,33 @@ sw_status sw_assembly_flight_loads(const sw_assembly* assembly, double* forces);
 
 typedef struct sw_batch sw_batch;
 
-/* Many wings of uneven size stepped together in one persistent launch
- * over concatenated tables with per-wing offsets. */
+/* Many wings of uneven size stepped together by one persistent launch (no
+ * counterpart in the reference, which steps one Structure at a time; each
+ * wing behaves exactly as a Structure built from the same definition and
+ * returns the same bits). Small wings run one LDS-resident workgroup each;
+ * large wings share the remaining compute units in proportion to their
+ * items. A wing whose step fails is restored to its pre-step state and the
+ * call reports the first failing wing's error; the other wings keep their
+ * accepted steps. */
 sw_status sw_batch_create(const sw_definition* const* definitions, size_t count, int device, sw_batch** out);
 void sw_batch_destroy(sw_batch* batch);
 sw_status sw_batch_set_face_pressure(sw_batch* b, size_t wing, const double* pressure, size_t count);
 sw_status sw_batch_set_external_forces(sw_batch* b, size_t wing, const double* forces, size_t node_count);
-/* Steps every wing `count` times; loads persist across these steps. */
+/* Steps every wing `count` times with the loads set before the call (as
+ * Structure::step consumes them after an accepted step). */
 sw_status sw_batch_step(sw_batch* b, const sw_step_settings* settings, int count);
 sw_status sw_batch_node_states(const sw_batch* b, size_t wing, double* position, double* velocity);
+/* Re-plans the launch: at most `workgroups_per_wing_cap` workgroups per
+ * wing, `threads` lanes per workgroup; reports the chosen workgroups. */
 sw_status sw_batch_set_launch(sw_batch* b, int workgroups_per_wing_cap, int threads);
+sw_status sw_batch_workgroups(const sw_batch* b, int* per_wing, size_t count);
+/* Cooperative (default) or plain launches, as sw_structure_set_cooperative. */
+sw_status sw_batch_set_cooperative(sw_batch* b, int cooperative);
+/* The wing's own structure handle, owned by the batch (valid until
+ * sw_batch_destroy): every sw_structure_* call applies to it, including
+ * stepping it alone with its own launch shape; sw_batch_set_launch restores
+ * the batched plan afterwards. */
+sw_status sw_batch_structure(sw_batch* b, size_t wing, sw_structure** out);
 
 #ifdef __cplusplus
 }

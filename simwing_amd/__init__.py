@@ -325,11 +325,24 @@ class Structure:
         handle = C.c_void_p()
         check(lib.sw_structure_create(C.byref(view), device, C.byref(handle)))
         self._handle = handle
+        self._owned = True
         self.node_count = definition.node_count
+
+    @classmethod
+    def _borrow(cls, handle, definition: DefinitionArrays, owner) -> "Structure":
+        """A wing owned by a batch (kept alive by `owner`)."""
+        s = cls.__new__(cls)
+        s._definition = definition
+        s._handle = handle
+        s._owned = False
+        s._owner = owner
+        s.node_count = definition.node_count
+        return s
 
     def close(self) -> None:
         if getattr(self, "_handle", None):
-            native.library().sw_structure_destroy(self._handle)
+            if getattr(self, "_owned", True):
+                native.library().sw_structure_destroy(self._handle)
             self._handle = None
 
     def __del__(self):
@@ -503,3 +516,18 @@ class Batch:
 
     def set_launch(self, workgroups_cap: int, threads: int) -> None:
         check(native.library().sw_batch_set_launch(self._handle, workgroups_cap, threads))
+
+    def workgroups(self) -> np.ndarray:
+        """Workgroups each wing runs on in the current launch plan."""
+        out = (C.c_int * len(self._definitions))()
+        check(native.library().sw_batch_workgroups(self._handle, out, len(self._definitions)))
+        return np.array(out[:], dtype=np.int64)
+
+    def set_cooperative(self, cooperative: bool) -> None:
+        check(native.library().sw_batch_set_cooperative(self._handle, int(cooperative)))
+
+    def structure(self, wing: int) -> Structure:
+        """The wing's own structure (owned by the batch)."""
+        handle = C.c_void_p()
+        check(native.library().sw_batch_structure(self._handle, wing, C.byref(handle)))
+        return Structure._borrow(handle, self._definitions[wing], self)

@@ -55,6 +55,21 @@ sw_status guarded(F&& body) {
     }
 }
 
+// Rethrows the failure a nested C entry point reported, as the exception
+// that produced its status.
+void check(sw_status status) {
+    if (status == SW_OK) return;
+    const std::string message = lastError;
+    switch (status) {
+    case SW_SUSPENSION_ERROR: throw swx::SuspensionFailure(lastPhase, lastEntity, message);
+    case SW_OUT_OF_RANGE: throw std::out_of_range(message);
+    case SW_INVALID_ARGUMENT: throw std::invalid_argument(message);
+    case SW_BAD_ALLOC: throw std::bad_alloc();
+    case SW_DEVICE_ERROR: throw std::runtime_error(message);
+    default: throw std::logic_error(message);
+    }
+}
+
 void requirePointer(const void* p, const char* what) {
     if (!p) throw std::invalid_argument(std::string(what) + " must not be null");
 }
@@ -133,6 +148,8 @@ swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st) {
     if (e.code == SW_INVALID_ARGUMENT)
         throw std::invalid_argument("SymmetricMatrix3 is singular or ill-conditioned; membrane element slot=" +
                                     std::to_string(e.index));
+    if (e.code == SW_DEVICE_ERROR)
+        throw std::runtime_error("device synchronisation timed out; the step was rolled back");
     throw std::logic_error("device step failed with code " + std::to_string(e.code));
 }
 
@@ -646,36 +663,147 @@ sw_status sw_assembly_flight_loads(const sw_assembly* a, double* forces) {
 
 // ---- batched wings ---------------------------------------------------------
 
+// Wings of uneven size stepped by one persistent launch. Each wing is a full
+// structure (definition, validation, pending loads, accounting, checkpoint
+// state); the batch shares only the launch. A wing whose step fails is
+// restored to its own pre-step state; the others keep their accepted steps.
 struct sw_batch {
-    int placeholder = 0;
+    std::vector<std::unique_ptr<sw_structure>> wings;
+    std::unique_ptr<swx::DeviceBatch> device;
 };
+
+namespace {
+
+sw_structure& batchWing(const sw_batch* b, size_t wing) {
+    requirePointer(b, "batch");
+    if (wing >= b->wings.size()) throw std::out_of_range("wing index is outside the batch");
+    return *b->wings[wing];
+}
+
+}  // namespace
 
 extern "C" {
 
 sw_status sw_batch_create(const sw_definition* const* definitions, size_t count, int device, sw_batch** out) {
     return guarded([&] {
-        (void)definitions;
-        (void)count;
-        (void)device;
-        (void)out;
-        throw std::logic_error("batched stepping is not built yet");
+        requirePointer(definitions, "definitions");
+        requirePointer(out, "out");
+        *out = nullptr;
+        if (count == 0) throw std::invalid_argument("a batch needs at least one wing");
+        auto b = std::make_unique<sw_batch>();
+        std::vector<swx::DeviceWing*> devices;
+        for (size_t w = 0; w < count; ++w) {
+            sw_structure* s = nullptr;
+            const sw_status status = sw_structure_create(definitions[w], device, &s);
+            if (status != SW_OK) throw std::invalid_argument("wing " + std::to_string(w) + ": " + lastError);
+            b->wings.emplace_back(s);
+            devices.push_back(s->wing.get());
+        }
+        b->device = std::make_unique<swx::DeviceBatch>(devices, device);
+        b->device->plan(8, 256);
+        *out = b.release();
     });
 }
+
 void sw_batch_destroy(sw_batch* batch) { delete batch; }
-sw_status sw_batch_set_face_pressure(sw_batch*, size_t, const double*, size_t) {
-    return guarded([&] { throw std::logic_error("batched stepping is not built yet"); });
+
+sw_status sw_batch_set_face_pressure(sw_batch* b, size_t wing, const double* pressure, size_t count) {
+    return guarded([&] { check(sw_structure_set_face_pressure(&batchWing(b, wing), pressure, count)); });
 }
-sw_status sw_batch_set_external_forces(sw_batch*, size_t, const double*, size_t) {
-    return guarded([&] { throw std::logic_error("batched stepping is not built yet"); });
+
+sw_status sw_batch_set_external_forces(sw_batch* b, size_t wing, const double* forces, size_t node_count) {
+    return guarded([&] { check(sw_structure_set_external_forces(&batchWing(b, wing), forces, node_count)); });
 }
-sw_status sw_batch_step(sw_batch*, const sw_step_settings*, int) {
-    return guarded([&] { throw std::logic_error("batched stepping is not built yet"); });
+
+sw_status sw_batch_step(sw_batch* b, const sw_step_settings* settings, int count) {
+    return guarded([&] {
+        requirePointer(b, "batch");
+        requirePointer(settings, "settings");
+        if (count < 1) throw std::invalid_argument("step count must be positive");
+        const size_t n = b->wings.size();
+        std::vector<swx::StepArgs> args(n);
+        for (size_t w = 0; w < n; ++w) args[w] = prepare(*b->wings[w], *settings);
+        b->device->ensurePlan();
+        for (auto& s : b->wings) {
+            if (s->pendingDirty) {
+                s->wing->setExternalForces(s->pending);
+                s->pendingDirty = false;
+            }
+            s->wing->clearError();
+            s->wing->synchronize();
+        }
+        for (int step = 0; step < count; ++step) {
+            b->device->snapshot();
+            for (int sub = 0; sub < settings->substeps; ++sub) b->device->launchSubstep(args);
+        }
+        b->device->synchronize();
+        std::vector<unsigned char> failed(n, 0);
+        std::vector<swx::DeviceError> errors(n);
+        std::vector<double> residual(n, 0.0);
+        bool any = false;
+        for (size_t w = 0; w < n; ++w) {
+            errors[w] = b->wings[w]->wing->readError(&residual[w]);
+            failed[w] = errors[w].code != 0;
+            any = any || failed[w];
+        }
+        if (any) {
+            b->device->restore(failed);
+            for (size_t w = 0; w < n; ++w)
+                if (failed[w]) b->wings[w]->wing->clearError();
+        }
+        for (size_t w = 0; w < n; ++w) {
+            if (failed[w]) continue;
+            sw_structure& s = *b->wings[w];
+            double applied[3] = {0.0, 0.0, 0.0};
+            for (size_t i = 0; i < s.pending.size() / 3; ++i)
+                for (int k = 0; k < 3; ++k) applied[k] += s.pending[3 * i + k];
+            for (int k = 0; k < 3; ++k) s.lastApplied[k] = applied[k];
+            std::fill(s.pending.begin(), s.pending.end(), 0.0);
+            s.pendingDirty = true;
+            s.acceptedSteps += static_cast<std::uint64_t>(count);
+            for (int step = 0; step < count; ++step) s.time += settings->time_step;
+            s.maxResidual = residual[w];
+        }
+        for (size_t w = 0; w < n; ++w)
+            if (failed[w]) throwDeviceError(*b->wings[w], errors[w]);
+    });
 }
-sw_status sw_batch_node_states(const sw_batch*, size_t, double*, double*) {
-    return guarded([&] { throw std::logic_error("batched stepping is not built yet"); });
+
+sw_status sw_batch_node_states(const sw_batch* b, size_t wing, double* position, double* velocity) {
+    return guarded([&] {
+        sw_structure& s = batchWing(b, wing);
+        check(sw_structure_node_states(&s, position, nullptr, velocity));
+    });
 }
-sw_status sw_batch_set_launch(sw_batch*, int, int) {
-    return guarded([&] { throw std::logic_error("batched stepping is not built yet"); });
+
+sw_status sw_batch_set_launch(sw_batch* b, int workgroups_per_wing_cap, int threads) {
+    return guarded([&] {
+        requirePointer(b, "batch");
+        b->device->plan(workgroups_per_wing_cap, threads);
+    });
+}
+
+sw_status sw_batch_set_cooperative(sw_batch* b, int cooperative) {
+    return guarded([&] {
+        requirePointer(b, "batch");
+        b->device->setCooperative(cooperative != 0);
+    });
+}
+
+sw_status sw_batch_structure(sw_batch* b, size_t wing, sw_structure** out) {
+    return guarded([&] {
+        requirePointer(out, "out");
+        *out = &batchWing(b, wing);
+    });
+}
+
+sw_status sw_batch_workgroups(const sw_batch* b, int* per_wing, size_t count) {
+    return guarded([&] {
+        requirePointer(b, "batch");
+        const auto& blocks = b->device->blocks();
+        if (count != blocks.size()) throw std::invalid_argument("per_wing must hold one entry per wing");
+        for (size_t w = 0; w < count; ++w) per_wing[w] = blocks[w];
+    });
 }
 
 }  // extern "C"

@@ -76,9 +76,9 @@ struct StepArgs {
     double reference[3];
     double maxLineResidual;
     int iterations;
-    int neighbourSync;   // set at launch: point-to-point synchronisation of the iterations
-    int dedicatedChain;  // set at launch: the last workgroup runs the serial segment runs
 };
+
+class DeviceBatch;
 
 class DeviceWing {
 public:
@@ -135,6 +135,8 @@ public:
     void kernelTime(double* milliseconds, long long* launches);
 
     void setLaunch(int workgroups, int threads);
+    // Incremented by every setLaunch (batches re-plan when a wing changed).
+    unsigned long long launchGeneration() const { return launchGeneration_; }
     // Multi-workgroup launches go through hipLaunchCooperativeKernel (the
     // runtime checks that every workgroup is resident at once) unless this is
     // off; then a plain launch is used, which is resident in practice on an
@@ -146,10 +148,16 @@ public:
     bool sharedNodes() const { return sharedNodes_; }
     hipStream_t stream() const { return stream_; }
 
-    // Device buffer base pointers exposed for the batched driver.
-    const void* viewBytes() const;
+    // The wing's device view with its current launch modes (for batched
+    // launches), the dynamic LDS its substep needs, and its item count.
+    WingView launchView() const;
+    std::size_t dynamicLdsBytes() const;
+    std::size_t itemCount() const { return membranes_ + hinges_ + constraints_ + segments_ + pairs_; }
+    std::size_t nodeCount() const { return nodes_; }
+    bool fitsInLds() const;
 
 private:
+    friend class DeviceBatch;
     void build(const Definition& d);
     int device_;
     hipStream_t stream_ = nullptr;
@@ -169,8 +177,10 @@ private:
     // Point-to-point dependency lists of the iteration program for the
     // current launch shape.
     void buildSync();
+    std::size_t sharedNodeBudget() const;
     bool hasRun_ = false, neighbourSync_ = false, dedicatedChain_ = false, cooperative_ = true;
     int participants_ = 0, maxParticipants_ = 0;
+    unsigned long long launchGeneration_ = 0;
     void* depAlloc_[3] = {nullptr, nullptr, nullptr};
     bool timing_ = false;
     std::vector<hipEvent_t> events_;  // start/stop pairs, grown on demand
@@ -190,5 +200,52 @@ private:
 
 // Device math conformance probe (function codes as sw_device_math).
 void deviceMath(int device, int function, const double* a, const double* b, double* out, std::size_t count);
+
+}  // namespace swx
+
+namespace swx {
+
+// Many wings stepped by one persistent launch: workgroup b runs workgroup
+// blockLocal[b] of wing blockWing[b]. Each wing keeps its own tables, state,
+// snapshot, error word and synchronisation words; only the launch is shared.
+class DeviceBatch {
+public:
+    DeviceBatch(std::vector<DeviceWing*> wings, int device);
+    ~DeviceBatch();
+    DeviceBatch(const DeviceBatch&) = delete;
+    DeviceBatch& operator=(const DeviceBatch&) = delete;
+
+    // Chooses each wing's workgroups: one LDS-resident workgroup when its
+    // nodes fit, else a share of the compute units in proportion to its items
+    // (at least 3: workers and the chain workgroup; at most `cap`), and
+    // uploads the launch tables. The total never exceeds the compute units,
+    // so every workgroup of the launch is resident at once.
+    void plan(int cap, int threads);
+    // Re-plans with the last cap and threads when a wing's launch shape was
+    // changed through its own handle since the plan.
+    void ensurePlan();
+    const std::vector<int>& blocks() const { return blocks_; }
+    int totalBlocks() const { return total_; }
+
+    void snapshot();                                 // wings without a pending error
+    void restore(const std::vector<unsigned char>& which);
+    void launchSubstep(const std::vector<StepArgs>& args);
+    void synchronize();
+    void setCooperative(bool cooperative) { cooperative_ = cooperative; }
+
+private:
+    void release();
+    std::vector<DeviceWing*> wings_;
+    int device_;
+    hipStream_t stream_ = nullptr;
+    void *views_ = nullptr, *args_ = nullptr, *blockWing_ = nullptr, *blockLocal_ = nullptr, *mask_ = nullptr;
+    std::vector<int> blocks_;
+    std::vector<unsigned long long> generations_;
+    int cap_ = 8;
+    int total_ = 0;
+    int threads_ = 256;
+    std::size_t lds_ = 0;
+    bool cooperative_ = true;
+};
 
 }  // namespace swx

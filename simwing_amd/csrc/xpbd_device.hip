@@ -79,6 +79,10 @@ struct WingView {
     int *depOffsets, *depPart, *depLag;
     unsigned* flags;
     int hasRun;  // the iteration program holds a serial segment run
+    // Launch modes of this wing (set on the host before each launch):
+    // point-to-point synchronisation, dedicated chain workgroup, nodes in
+    // LDS, and the wing's workgroup count in a batched launch.
+    int neighbourSync, dedicatedChain, sharedNodes, launchBlocks;
     double* state;      // contiguous mutable state block
     double* snapshot;   // copy of `state`
     long stateDoubles;
@@ -508,66 +512,78 @@ __device__ __forceinline__ m3 loadInverseInertia(const PayloadDev& p) {
     return r;
 }
 
-// LDS staging of one serial segment run, in doubles.
-constexpr int kChainHang = 32;  // harness points staged with the run
-constexpr int kChainStageDoubles = 4 * kChainSlots + 3 * kChainItems + 3 * kChainHang + 4 * kChainItems / 2;
+constexpr int kChainHang = 32;  // harness points staged with a serial run
+
+// Staging of a serial run: distinct LDS objects addressed directly, so the
+// serial loop reads and writes them with LDS instructions and its loads of
+// item values never wait on its stores of node slots.
+__shared__ double chainX[kChainSlots], chainY[kChainSlots], chainZ[kChainSlots], chainW[kChainSlots];
+__shared__ double chainHang[3 * kChainHang];
+__shared__ double chainLambda[kChainItems], chainRest[kChainItems], chainStiffness[kChainItems];
+__shared__ int chainItem[4 * kChainItems];  // from, to, runtime index, last-payload flag
+
+struct SlotRefs {
+    __device__ __forceinline__ d3 get(int i) const { return d3{chainX[i], chainY[i], chainZ[i]}; }
+    __device__ __forceinline__ double invMass(int i) const { return chainW[i]; }
+    __device__ __forceinline__ d3 hang(int h) const {
+        return d3{chainHang[3 * h], chainHang[3 * h + 1], chainHang[3 * h + 2]};
+    }
+    __device__ __forceinline__ void put(int i, d3 p) const {
+        chainX[i] = p.x;
+        chainY[i] = p.y;
+        chainZ[i] = p.z;
+    }
+};
 
 // A serial segment run: consecutive single-item dependency levels, i.e.
 // segments that share the payload or a node, in the reference's string-id
-// order (suspension.cpp:2197-2274). The workgroup stages the run's nodes and
-// per-segment values in LDS; lane 0 then projects the segments one after
-// another with the payload pose, its definition and the inverse inertia in
-// registers, so the serial chain waits on no global memory; the workgroup
-// finally writes nodes, multipliers and the payload back. Identical
-// arithmetic to projecting the segments in place.
+// order (suspension.cpp:2197-2274). The workgroup stages the run's nodes,
+// per-segment values and the harness points in LDS; lane 0 then projects the
+// segments one after another with the payload pose, its definition and the
+// inverse inertia in registers, so the serial chain waits on no global
+// memory; the workgroup finally writes nodes, multipliers and the payload
+// back. Identical arithmetic to projecting the segments in place.
 __device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, PayloadDev& payload,
-                         const StepArgs& a, double* stage) {
+                                      const StepArgs& a) {
     const int slots = P.slotEnd - P.slotBegin;
     const int items = P.end - P.begin;
-    const bool stageHang = v.hangCount <= kChainHang;
-    double* hang = stage + 4 * kChainSlots;
-    double* lam = hang + 3 * kChainHang;
-    double* rest = lam + kChainItems;
-    double* stiff = rest + kChainItems;
-    int* ends = reinterpret_cast<int*>(stiff + kChainItems);  // from, to, runtime index, last-payload flag
-    const NodeRefs S{stage, stage + kChainSlots, stage + 2 * kChainSlots, stage + 3 * kChainSlots,
-                     stageHang ? hang : v.hangLocal};
     for (int s = threadIdx.x; s < slots; s += blockDim.x) {
         const int node = v.runNodes[P.slotBegin + s];
-        S.x[s] = N.x[node];
-        S.y[s] = N.y[node];
-        S.z[s] = N.z[node];
-        S.w[s] = N.w[node];
+        chainX[s] = N.x[node];
+        chainY[s] = N.y[node];
+        chainZ[s] = N.z[node];
+        chainW[s] = N.w[node];
     }
     for (int j = threadIdx.x; j < items; j += blockDim.x) {
         const int k = P.begin + j;
-        lam[j] = v.slam[k];
-        rest[j] = v.srest[k];
-        stiff[j] = v.sstiff[k];
-        ends[4 * j] = v.sfromSlot[k];
-        ends[4 * j + 1] = v.stoSlot[k];
-        ends[4 * j + 2] = v.sruntime[k];
-        ends[4 * j + 3] = v.slastPayload[k];
+        chainLambda[j] = v.slam[k];
+        chainRest[j] = v.srest[k];
+        chainStiffness[j] = v.sstiff[k];
+        chainItem[4 * j] = v.sfromSlot[k];
+        chainItem[4 * j + 1] = v.stoSlot[k];
+        chainItem[4 * j + 2] = v.sruntime[k];
+        chainItem[4 * j + 3] = v.slastPayload[k];
     }
-    if (stageHang)
-        for (int h = threadIdx.x; h < 3 * v.hangCount; h += blockDim.x) hang[h] = v.hangLocal[h];
+    for (int h = threadIdx.x; h < 3 * v.hangCount; h += blockDim.x) chainHang[h] = v.hangLocal[h];
     __syncthreads();
     if (threadIdx.x == 0) {
+        const SlotRefs S{};
+        const StepArgs args = a;  // by value: kept in registers across the loop
         const PayloadDev def = payload;
         Payload state = loadPayload(def);
         m3 iinv = loadInverseInertia(def);
         bool iinvFinite = finiteM(iinv);
         bool touched = false;
         for (int j = 0; j < items; ++j) {
-            const int fe = ends[4 * j], te = ends[4 * j + 1];
-            double lambda = lam[j];
-            if (!projectSegment(v, S, fe, te, rest[j], stiff[j], lambda, ends[4 * j + 2], state, def, iinv,
-                                iinvFinite, a))
+            const int fe = chainItem[4 * j], te = chainItem[4 * j + 1];
+            double lambda = chainLambda[j];
+            if (!projectSegment(v, S, fe, te, chainRest[j], chainStiffness[j], lambda, chainItem[4 * j + 2], state,
+                                def, iinv, iinvFinite, args))
                 break;
-            lam[j] = lambda;
+            chainLambda[j] = lambda;
             if (fe >= 0 && te >= 0) continue;
             touched = true;
-            if (ends[4 * j + 3]) {
+            if (chainItem[4 * j + 3]) {
                 m3 world, inv;
                 if (!worldInertia(state.q, def.inertia, world) || !inverseM(world, inv)) {
                     raise(v, SW_SUSPENSION_ERROR, 1, 2, 0);
@@ -583,11 +599,11 @@ __device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, cons
     __syncthreads();
     for (int s = threadIdx.x; s < slots; s += blockDim.x) {
         const int node = v.runNodes[P.slotBegin + s];
-        N.x[node] = S.x[s];
-        N.y[node] = S.y[s];
-        N.z[node] = S.z[s];
+        N.x[node] = chainX[s];
+        N.y[node] = chainY[s];
+        N.z[node] = chainZ[s];
     }
-    for (int j = threadIdx.x; j < items; j += blockDim.x) v.slam[P.begin + j] = lam[j];
+    for (int j = threadIdx.x; j < items; j += blockDim.x) v.slam[P.begin + j] = chainLambda[j];
 }
 
 // Waits for the progress words named by dependency entries [e0, e1) of
@@ -721,7 +737,7 @@ __device__ __forceinline__ void atomicMaxNonNegative(double* address, double val
 // program; a serial run (kPhaseSegmentRun) is staged and projected by the
 // calling workgroup as a whole.
 __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, int first, int end, int stride,
-                         PayloadDev& payload, const StepArgs& a, double* stage) {
+                         PayloadDev& payload, const StepArgs& a) {
     switch (P.type) {
     case kPhaseDistance:
         for (int k = first; k < end; k += stride) projectDistance(v, N, k, a.invDt2);
@@ -750,7 +766,7 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
         }
         break;
     case kPhaseSegmentRun:
-        runChain(v, N, P, payload, a, stage);
+        runChain(v, N, P, payload, a);
         break;
     case kPhaseSpheres:
         for (int k = first; k < end; k += stride) projectSphere(v, N, k, a.invDt2);
@@ -760,24 +776,28 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
     }
 }
 
-template <bool kShared>
-__global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArgs a) {
+// One substep of one wing, run by `blocks` workgroups of which this is
+// `block`. kMode: 0 nodes in HBM, 1 nodes resident in LDS, 2 either, as the
+// wing's view says (batched launches mix both kinds of wing).
+template <int kMode>
+__device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a, int block, int blocks) {
     extern __shared__ double lds[];
     // A pending failure stops later substeps of a queued batch. Only a
-    // single-workgroup launch may leave early: with several workgroups the
+    // single-workgroup wing may leave early: with several workgroups the
     // error word can change between their starts, and a workgroup that left
     // would strand the others at the grid barrier.
-    if (gridDim.x == 1 && __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    if (blocks == 1 && __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int n = v.nodeCount;
-    const bool dedicated = !kShared && a.neighbourSync != 0 && a.dedicatedChain != 0 && gridDim.x > 1 && v.hasRun;
-    const int workers = dedicated ? static_cast<int>(gridDim.x) - 1 : static_cast<int>(gridDim.x);
+    const bool shared = kMode == 1 || (kMode == 2 && v.sharedNodes != 0);
+    const bool dedicated = !shared && v.neighbourSync != 0 && v.dedicatedChain != 0 && blocks > 1 && v.hasRun;
+    const int workers = dedicated ? blocks - 1 : blocks;
     // The chain workgroup takes part in the grid barriers around the
     // iterations but owns no item of the other passes.
-    const bool chainOnly = dedicated && static_cast<int>(blockIdx.x) == workers;
-    const int gtid = chainOnly ? 0x3fffffff : static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const bool chainOnly = dedicated && block == workers;
+    const int gtid = chainOnly ? 0x3fffffff : static_cast<int>(block * blockDim.x + threadIdx.x);
     const int gstride = workers * blockDim.x;
     NodeRefs N;
-    if (kShared) {
+    if (shared) {
         N = NodeRefs{lds, lds + n, lds + 2 * n, lds + 3 * n, v.hangLocal};
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             N.x[i] = v.px[i];
@@ -790,7 +810,6 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
         N = NodeRefs{v.px, v.py, v.pz, v.w, v.hangLocal};
     }
     PayloadDev& payload = *v.payload;
-    double* stage = lds + (kShared ? 4 * n : 0) + kMaxThreads;  // kChainStageDoubles
 
     // A: prev = x, multiplier resets, payload prediction.
     for (int i = gtid; i < n; i += gstride) {
@@ -817,7 +836,7 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
             !refreshInverseInertia(payload, quat{payload.q[0], payload.q[1], payload.q[2], payload.q[3]}))
             raise(v, SW_SUSPENSION_ERROR, 5, 2, 0);
     }
-    gridSync(v, static_cast<int>(gridDim.x));
+    gridSync(v, blocks);
 
     // B: pressure gather from the start-of-substep positions, then predict
     // (soft_body.cpp:1492-1566).
@@ -848,17 +867,17 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
         v.vz[i] = vel.z;
         N.put(i, add(N.get(i), scale(vel, a.dt)));
     }
-    gridSync(v, static_cast<int>(gridDim.x));
+    gridSync(v, blocks);
 
     // Constraint iterations.
-    if (a.neighbourSync) {
+    if (v.neighbourSync && !shared && blocks > 1) {
         // Point-to-point synchronisation: workgroup p owns a contiguous share
         // of every phase's items (the dedicated chain workgroup owns the
         // serial runs). Before a phase it waits only for the workgroups whose
         // earlier accesses to its nodes the in-order program puts first
         // (host-computed), and after it publishes its progress.
         const int m = v.iterPhaseCount;
-        const int p = static_cast<int>(blockIdx.x);
+        const int p = block;
         const bool chainGroup = chainOnly;
         for (int it = 0; it < a.iterations; ++it) {
             for (int ph = 0; ph < m; ++ph) {
@@ -874,13 +893,12 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
                 if (b >= e) continue;
                 const int step = it * m + ph;
                 waitDeps(v, p * m + ph, step);
-                runPhase(v, N, P, b + static_cast<int>(threadIdx.x), e, static_cast<int>(blockDim.x), payload, a,
-                         stage);
+                runPhase(v, N, P, b + static_cast<int>(threadIdx.x), e, static_cast<int>(blockDim.x), payload, a);
                 __syncthreads();
                 publishCounter(v.flags + 32 * p, static_cast<unsigned>(step + 1));
             }
         }
-        gridSync(v, static_cast<int>(gridDim.x));
+        gridSync(v, blocks);
         // Every participant has left the loop: nobody polls the progress
         // words any more, so each resets its own for the next launch.
         if (threadIdx.x == 0) __hip_atomic_store(v.flags + 32 * p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -890,9 +908,9 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
             for (int ph = 0; ph < v.iterPhaseCount; ++ph) {
                 const Phase P = v.iterPhases[ph];
                 if (P.type == kPhaseSegmentRun) {
-                    if (blockIdx.x == 0) runPhase(v, N, P, P.begin, P.end, 1, payload, a, stage);
+                    if (block == 0) runPhase(v, N, P, P.begin, P.end, 1, payload, a);
                 } else {
-                    runPhase(v, N, P, P.begin + gtid, P.end, gstride, payload, a, stage);
+                    runPhase(v, N, P, P.begin + gtid, P.end, gstride, payload, a);
                 }
                 gridSync(v, workers);
             }
@@ -964,7 +982,7 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
             const double compliant = v.slam[k] / (v.sstiff[k] * a.dt * a.dt);
             worst = stdmax(worst, fabs(stretch - compliant));
         }
-        double* scratch = kShared ? lds + 4 * n : lds;  // kMaxThreads doubles
+        double* scratch = shared ? lds + 4 * n : lds;  // kMaxThreads doubles
         scratch[threadIdx.x] = worst;
         __syncthreads();
         for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -978,7 +996,7 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
         if (!finishPayload(payload, a)) raise(v, SW_SUSPENSION_ERROR, 8, 2, 0);
         else if (payload.maxResidual > a.maxLineResidual) raise(v, SW_SUSPENSION_ERROR, 8, 3, 0);
     }
-    if (kShared) {
+    if (shared) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             v.px[i] = N.x[i];
             v.py[i] = N.y[i];
@@ -987,10 +1005,38 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
     }
 }
 
+template <bool kShared>
+__global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArgs a) {
+    substepBody<kShared ? 1 : 0>(v, a, static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+}
+
+// Batched substep: workgroup b runs workgroup blockLocal[b] of wing
+// blockWing[b]; each wing's view holds its own workgroup count and modes.
+__global__ void __launch_bounds__(kMaxThreads) batchSubstepKernel(const WingView* views, const StepArgs* args,
+                                                                   const int* blockWing, const int* blockLocal) {
+    const int wing = blockWing[blockIdx.x];
+    const WingView& v = views[wing];
+    substepBody<2>(v, args[wing], blockLocal[blockIdx.x], v.launchBlocks);
+}
+
 // Copies the mutable state block into the snapshot while no error is
 // pending (so a failing step keeps its own pre-step snapshot).
 __global__ void snapshotKernel(WingView v, int restore) {
     if (!restore && __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < v.stateDoubles; i += stride) {
+        if (restore) v.state[i] = v.snapshot[i];
+        else v.snapshot[i] = v.state[i];
+    }
+}
+
+// Snapshot (restore = 0) of every wing without a pending error, or restore
+// (restore = 1) of the wings with mask[wing] set; blockIdx.y is the wing.
+__global__ void batchSnapshotKernel(const WingView* views, const unsigned char* mask, int restore) {
+    const WingView& v = views[blockIdx.y];
+    if (restore ? mask[blockIdx.y] == 0
+                : __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        return;
     const long stride = static_cast<long>(gridDim.x) * blockDim.x;
     for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < v.stateDoubles; i += stride) {
         if (restore) v.state[i] = v.snapshot[i];
@@ -1514,6 +1560,8 @@ void DeviceWing::build(const Definition& d) {
         v.sfromSlot = deviceCopy(allocations_, fromSlot);
         v.stoSlot = deviceCopy(allocations_, toSlot);
         v.runNodes = deviceCopy(allocations_, runNodes);
+        if (layout_.hangOrder.size() > static_cast<std::size_t>(kChainHang))
+            throw std::invalid_argument("the device suspension sweep supports at most 32 harness points");
         std::vector<double> hang(3 * layout_.hangOrder.size());
         for (std::size_t r = 0; r < layout_.hangOrder.size(); ++r)
             for (int c = 0; c < 3; ++c) hang[3 * r + c] = d.suspension->harnessLocal[3 * layout_.hangOrder[r] + c];
@@ -1630,16 +1678,15 @@ void DeviceWing::build(const Definition& d) {
         std::vector<unsigned> barrier(128, 0);  // count, generation, sync-failure flag
         v.barrier = deviceCopy(allocations_, barrier);
     }
-    // Launch shape: one workgroup with LDS-resident nodes when they fit.
-    int maxShared = 0;
-    check(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device_), "attribute");
-    const std::size_t ldsNeed = (4 * nodes_ + kMaxThreads + kChainStageDoubles) * sizeof(double);
-    sharedNodes_ = ldsNeed <= static_cast<std::size_t>(maxShared);
+    // Launch shape: one workgroup with LDS-resident nodes when they fit
+    // beside the kernel's static LDS (the serial-run staging).
+    const std::size_t dynamicLds = (4 * nodes_ + kMaxThreads) * sizeof(double);
+    sharedNodes_ = dynamicLds <= sharedNodeBudget();
     workgroups_ = 1;
     threads_ = kMaxThreads;
     if (sharedNodes_) {
         check(hipFuncSetAttribute(reinterpret_cast<const void*>(&substepKernel<true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ldsNeed)),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamicLds)),
               "shared memory attribute");
     } else {
         // Nodes stay in HBM: a cooperative grid of one workgroup per compute
@@ -1662,7 +1709,19 @@ void DeviceWing::build(const Definition& d) {
     check(hipDeviceSynchronize(), "sync build");
 }
 
+// Dynamic LDS available to the LDS-resident kernel: the per-workgroup
+// maximum less the kernel's static LDS.
+std::size_t DeviceWing::sharedNodeBudget() const {
+    int maxShared = 0;
+    check(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device_), "attribute");
+    hipFuncAttributes attributes{};
+    check(hipFuncGetAttributes(&attributes, reinterpret_cast<const void*>(&substepKernel<true>)), "kernel attributes");
+    const std::size_t total = static_cast<std::size_t>(maxShared);
+    return total > attributes.sharedSizeBytes ? total - attributes.sharedSizeBytes : 0;
+}
+
 void DeviceWing::setLaunch(int workgroups, int threads) {
+    ++launchGeneration_;
     if (workgroups < 1 || threads < 64 || threads > kMaxThreads || (threads & (threads - 1)) != 0)
         throw std::invalid_argument("launch shape must be >= 1 workgroup of 64..256 (power of two) threads");
     int cus = 0;
@@ -1670,13 +1729,11 @@ void DeviceWing::setLaunch(int workgroups, int threads) {
     if (workgroups > cus) throw std::invalid_argument("launch shape exceeds one workgroup per compute unit");
     workgroups_ = workgroups;
     threads_ = threads;
-    int maxShared = 0;
-    check(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device_), "attribute");
-    const std::size_t ldsNeed = (4 * nodes_ + kMaxThreads + kChainStageDoubles) * sizeof(double);
-    sharedNodes_ = workgroups == 1 && ldsNeed <= static_cast<std::size_t>(maxShared);
+    const std::size_t dynamicLds = (4 * nodes_ + kMaxThreads) * sizeof(double);
+    sharedNodes_ = workgroups == 1 && dynamicLds <= sharedNodeBudget();
     if (sharedNodes_) {
         check(hipFuncSetAttribute(reinterpret_cast<const void*>(&substepKernel<true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ldsNeed)),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamicLds)),
               "shared memory attribute");
     }
     buildSync();
@@ -1702,10 +1759,12 @@ void DeviceWing::setFacePressure(const std::vector<double>& pressure) {
 }
 
 void DeviceWing::launchSubstep(const StepArgs& stepArgs) {
-    const WingView& v = *view_;
+    WingView v = *view_;
+    v.neighbourSync = neighbourSync_ ? 1 : 0;
+    v.dedicatedChain = dedicatedChain_ ? 1 : 0;
+    v.sharedNodes = sharedNodes_ ? 1 : 0;
+    v.launchBlocks = sharedNodes_ ? 1 : workgroups_;
     StepArgs args = stepArgs;
-    args.neighbourSync = neighbourSync_ ? 1 : 0;
-    args.dedicatedChain = dedicatedChain_ ? 1 : 0;
     if (timing_) {
         while (events_.size() < eventsUsed_ + 2) {
             hipEvent_t e;
@@ -1714,7 +1773,7 @@ void DeviceWing::launchSubstep(const StepArgs& stepArgs) {
         }
         check(hipEventRecord(events_[eventsUsed_], stream_), "hipEventRecord");
     }
-    const std::size_t scratch = (kMaxThreads + kChainStageDoubles) * sizeof(double);
+    const std::size_t scratch = kMaxThreads * sizeof(double);
     if (sharedNodes_) {
         const std::size_t bytes = 4 * nodes_ * sizeof(double) + scratch;
         hipLaunchKernelGGL(substepKernel<true>, dim3(1), dim3(threads_), bytes, stream_, v, args);
@@ -1723,7 +1782,7 @@ void DeviceWing::launchSubstep(const StepArgs& stepArgs) {
     } else if (!cooperative_) {
         hipLaunchKernelGGL(substepKernel<false>, dim3(workgroups_), dim3(threads_), scratch, stream_, v, args);
     } else {
-        void* params[] = {const_cast<WingView*>(&v), const_cast<StepArgs*>(&args)};
+        void* params[] = {&v, &args};
         check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&substepKernel<false>), dim3(workgroups_),
                                          dim3(threads_), params, scratch, stream_),
               "cooperative launch");
@@ -2046,7 +2105,156 @@ DeviceWing::Reductions DeviceWing::reduce() {
     return r;
 }
 
-const void* DeviceWing::viewBytes() const { return view_.get(); }
+WingView DeviceWing::launchView() const {
+    WingView v = *view_;
+    v.neighbourSync = neighbourSync_ ? 1 : 0;
+    v.dedicatedChain = dedicatedChain_ ? 1 : 0;
+    v.sharedNodes = sharedNodes_ ? 1 : 0;
+    v.launchBlocks = sharedNodes_ ? 1 : workgroups_;
+    return v;
+}
+
+std::size_t DeviceWing::dynamicLdsBytes() const {
+    return (sharedNodes_ ? 4 * nodes_ * sizeof(double) : 0) + kMaxThreads * sizeof(double);
+}
+
+bool DeviceWing::fitsInLds() const { return (4 * nodes_ + kMaxThreads) * sizeof(double) <= sharedNodeBudget(); }
+
+// ---- batched wings ----------------------------------------------------------
+
+DeviceBatch::DeviceBatch(std::vector<DeviceWing*> wings, int device) : wings_(std::move(wings)), device_(device) {
+    if (wings_.empty()) throw std::invalid_argument("a batch needs at least one wing");
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+DeviceBatch::~DeviceBatch() {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    release();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void DeviceBatch::release() {
+    for (void** p : {&views_, &args_, &blockWing_, &blockLocal_, &mask_}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+}
+
+void DeviceBatch::plan(int cap, int threads) {
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipStreamSynchronize(stream_), "synchronise before planning");
+    if (threads < 64 || threads > kMaxThreads || (threads & (threads - 1)) != 0)
+        throw std::invalid_argument("batch threads must be a power of two in 64..256");
+    int cus = 0;
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
+    const int count = static_cast<int>(wings_.size());
+    if (count > cus) throw std::invalid_argument("a batch holds at most one wing per compute unit");
+    cap = std::max(cap, 1);
+    // One LDS-resident workgroup per small wing; the remaining compute units
+    // are shared by the large wings in proportion to their items.
+    blocks_.assign(count, 1);
+    std::vector<int> large;
+    double largeItems = 0.0;
+    for (int w = 0; w < count; ++w)
+        if (!wings_[w]->fitsInLds() || cap == 1) {
+            large.push_back(w);
+            largeItems += static_cast<double>(wings_[w]->itemCount());
+        }
+    const int floor = std::min(3, cap);
+    int spare = cus - count;
+    for (int w : large) {
+        const int extra = std::min(floor, cap) - 1;
+        blocks_[w] += std::max(0, std::min(extra, spare));
+        spare -= std::max(0, std::min(extra, spare));
+    }
+    if (spare > 0 && largeItems > 0.0) {
+        const int pool = spare;
+        for (int w : large) {
+            const int want = static_cast<int>(pool * (wings_[w]->itemCount() / largeItems));
+            const int add = std::max(0, std::min({want, cap - blocks_[w], spare}));
+            blocks_[w] += add;
+            spare -= add;
+        }
+    }
+    lds_ = 0;
+    std::vector<WingView> views(count);
+    std::vector<int> blockWing, blockLocal;
+    for (int w = 0; w < count; ++w) {
+        wings_[w]->setLaunch(blocks_[w], threads);
+        views[w] = wings_[w]->launchView();
+        lds_ = std::max(lds_, wings_[w]->dynamicLdsBytes());
+        for (int b = 0; b < views[w].launchBlocks; ++b) {
+            blockWing.push_back(w);
+            blockLocal.push_back(b);
+        }
+    }
+    if (lds_ > kMaxThreads * sizeof(double))
+        check(hipFuncSetAttribute(reinterpret_cast<const void*>(&batchSubstepKernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_)),
+              "shared memory attribute");
+    total_ = static_cast<int>(blockWing.size());
+    threads_ = threads;
+    cap_ = cap;
+    generations_.clear();
+    for (DeviceWing* w : wings_) generations_.push_back(w->launchGeneration());
+    release();
+    check(hipMalloc(&views_, views.size() * sizeof(WingView)), "hipMalloc");
+    check(hipMemcpy(views_, views.data(), views.size() * sizeof(WingView), hipMemcpyHostToDevice), "upload");
+    check(hipMalloc(&args_, count * sizeof(StepArgs)), "hipMalloc");
+    check(hipMalloc(&blockWing_, blockWing.size() * sizeof(int)), "hipMalloc");
+    check(hipMemcpy(blockWing_, blockWing.data(), blockWing.size() * sizeof(int), hipMemcpyHostToDevice), "upload");
+    check(hipMalloc(&blockLocal_, blockLocal.size() * sizeof(int)), "hipMalloc");
+    check(hipMemcpy(blockLocal_, blockLocal.data(), blockLocal.size() * sizeof(int), hipMemcpyHostToDevice), "upload");
+    check(hipMalloc(&mask_, count), "hipMalloc");
+}
+
+void DeviceBatch::ensurePlan() {
+    for (std::size_t w = 0; w < wings_.size(); ++w)
+        if (generations_.size() != wings_.size() || wings_[w]->launchGeneration() != generations_[w]) {
+            plan(cap_, threads_);
+            return;
+        }
+}
+
+void DeviceBatch::snapshot() {
+    const dim3 grid(16, static_cast<unsigned>(wings_.size()));
+    hipLaunchKernelGGL(batchSnapshotKernel, grid, dim3(256), 0, stream_, static_cast<const WingView*>(views_),
+                       static_cast<const unsigned char*>(mask_), 0);
+    check(hipGetLastError(), "batch snapshot");
+}
+
+void DeviceBatch::restore(const std::vector<unsigned char>& which) {
+    check(hipMemcpyAsync(mask_, which.data(), which.size(), hipMemcpyHostToDevice, stream_), "restore mask");
+    const dim3 grid(16, static_cast<unsigned>(wings_.size()));
+    hipLaunchKernelGGL(batchSnapshotKernel, grid, dim3(256), 0, stream_, static_cast<const WingView*>(views_),
+                       static_cast<const unsigned char*>(mask_), 1);
+    check(hipGetLastError(), "batch restore");
+    check(hipStreamSynchronize(stream_), "batch restore");
+}
+
+void DeviceBatch::launchSubstep(const std::vector<StepArgs>& args) {
+    if (!views_) throw std::logic_error("the batch has no launch plan");
+    check(hipMemcpyAsync(args_, args.data(), args.size() * sizeof(StepArgs), hipMemcpyHostToDevice, stream_),
+          "batch arguments");
+    const WingView* views = static_cast<const WingView*>(views_);
+    const StepArgs* stepArgs = static_cast<const StepArgs*>(args_);
+    const int* blockWing = static_cast<const int*>(blockWing_);
+    const int* blockLocal = static_cast<const int*>(blockLocal_);
+    if (!cooperative_) {
+        hipLaunchKernelGGL(batchSubstepKernel, dim3(total_), dim3(threads_), lds_, stream_, views, stepArgs, blockWing,
+                           blockLocal);
+    } else {
+        void* params[] = {&views, &stepArgs, &blockWing, &blockLocal};
+        check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&batchSubstepKernel), dim3(total_),
+                                         dim3(threads_), params, lds_, stream_),
+              "cooperative batch launch");
+    }
+    check(hipGetLastError(), "batch substep launch");
+}
+
+void DeviceBatch::synchronize() { check(hipStreamSynchronize(stream_), "batch synchronize"); }
 
 }  // namespace swx
 

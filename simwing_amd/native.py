@@ -222,6 +222,9 @@ _SIGNATURES = {
     "sw_batch_step": (C.c_int, [C.c_void_p, C.POINTER(StepSettings), C.c_int]),
     "sw_batch_node_states": (C.c_int, [C.c_void_p, C.c_size_t, _F64P, _F64P]),
     "sw_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "sw_batch_workgroups": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_size_t]),
+    "sw_batch_set_cooperative": (C.c_int, [C.c_void_p, C.c_int]),
+    "sw_batch_structure": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
 }
 
 _lib = None
