@@ -231,6 +231,52 @@ def batched_leg(count: int, steps: int, seed: int = 0x5EED0002) -> dict:
     }
 
 
+def adjoint_leg(wing: str, steps: int, interval: int, seed: int = 0x5EED0003) -> dict:
+    """Shape-loss gradient (membrane log-compliance, line rest length,
+    triangle pressure) over `steps` steps of `wing`, checkpoints every
+    `interval` steps, against the forward pass alone. The target is a hidden
+    run with compliance x U(0.8, 1.2) and rest lengths x U(0.98, 1.02)."""
+    import simwing_amd as sw
+    definition, pressure, loads = make_workload(wing)
+    s = sw.Structure(definition)
+    s.set_face_pressure(pressure)
+    s.set_external_forces(loads)
+    adj = sw.Adjoint(s, steps, interval)
+    rng = np.random.default_rng(seed)
+    rest = np.asarray(definition.suspension["segment_rest"], dtype=np.float64)
+    hidden_theta = np.log(rng.uniform(0.8, 1.2, adj.membrane_count))
+    hidden_rest = rest * rng.uniform(0.98, 1.02, rest.size)
+    x0 = np.asarray(definition.arrays["node_position"]).reshape(-1, 3)
+    target = adj.run(x0, hidden_theta, hidden_rest, gradients=False)["final_position"]
+    start = time.perf_counter()
+    forward = adj.run(target, gradients=False)
+    forward_s = time.perf_counter() - start
+    forward_ms, _, _ = adj.timing()
+    start = time.perf_counter()
+    result = adj.run(target)
+    gradient_s = time.perf_counter() - start
+    fwd_ms, rev_ms, device_bytes = adj.timing()
+    finite = all(np.all(np.isfinite(result[k])) for k in ("grad_log_compliance", "grad_rest_length", "grad_pressure"))
+    if not finite or result["loss"] != forward["loss"]:
+        raise RuntimeError("adjoint run is not finite or not reproducible")
+    adj.close()
+    s.close()
+    return {
+        "metric": "adjoint/fwd cost",
+        "value": gradient_s / forward_s,
+        "unit": "x (gradient run / forward run, wall clock)",
+        "higher_is_better": False,
+        "device_ratio": (fwd_ms + rev_ms) / forward_ms,
+        "forward_ms": forward_ms,
+        "gradient_forward_ms": fwd_ms,
+        "gradient_reverse_ms": rev_ms,
+        "device_bytes": device_bytes,
+        "workload": (f"{wing} wing: {definition.node_count} nodes, {adj.membrane_count} membranes, "
+                     f"{rest.size} line segments; {steps} steps x 48 iterations, checkpoints every {interval} steps; "
+                     f"{adj.membrane_count + rest.size + adj.triangle_count} parameters"),
+    }
+
+
 def make_workload(name: str):
     import simwing_amd as sw
     scene = sw.generate_scene(name)
@@ -251,6 +297,8 @@ def main() -> int:
     parser.add_argument("--no-batch", action="store_true")
     parser.add_argument("--batch-wings", type=int, default=64)
     parser.add_argument("--batch-steps", type=int, default=120)
+    parser.add_argument("--no-adjoint", action="store_true")
+    parser.add_argument("--adjoint-steps", type=int, default=120)
     parser.add_argument("--launch", default=None, help="fixed launch shape WGxTHREADS")
     parser.add_argument("--plain-launch", action="store_true",
                         help="plain instead of cooperative launches (for profilers that do not follow the latter)")
@@ -311,6 +359,7 @@ def main() -> int:
 
     structure.close()
     batched = None if args.no_batch else batched_leg(args.batch_wings, args.batch_steps)
+    adjoint = None if args.no_adjoint else [adjoint_leg(w, args.adjoint_steps, 8) for w in ("small", "variant10k")]
 
     cpu = None
     if not args.no_cpu:
@@ -351,6 +400,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -339,6 +339,46 @@ sw_status sw_batch_set_cooperative(sw_batch* b, int cooperative);
  * the batched plan afterwards. */
 sw_status sw_batch_structure(sw_batch* b, size_t wing, sw_structure** out);
 
+/* ---- Adjoint ------------------------------------------------------------ */
+
+typedef struct sw_adjoint sw_adjoint;
+
+/* Reverse-mode gradients of the shape loss
+ *     L = sum over nodes of |x(end) - x_target|^2
+ * after `steps` steps of `settings` from the structure's current state, with
+ * the loads pending now applied at every step (no counterpart in the
+ * reference, which has no differentiation). Parameters:
+ *   - log-compliance per membrane element (definition order): the element's
+ *     compliance matrix becomes C e^theta (its stiffness is scaled by
+ *     e^-theta before the cofactor inverse of membrane.cpp:268-271);
+ *   - rest length per suspension segment (definition order; a line's trim is
+ *     the sum over its segments);
+ *   - pressure per triangle.
+ * The forward pass keeps a state checkpoint every `checkpoint_interval`
+ * steps; the reverse pass replays each interval from its checkpoint with
+ * the trajectory recorded and runs the reverse kernels substep by substep
+ * in FP64, carrying the Lagrange-multiplier adjoints through every colour
+ * sweep and iteration. The structure must outlive the adjoint; its state,
+ * step count and parameters are unchanged by every run. Membrane damping is
+ * rejected with SW_INVALID_ARGUMENT. */
+sw_status sw_adjoint_configure(sw_structure* structure, const sw_step_settings* settings, uint64_t steps,
+                               uint32_t checkpoint_interval, sw_adjoint** out);
+void sw_adjoint_destroy(sw_adjoint* adjoint);
+/* One forward + reverse run at the given parameters (null: the
+ * definition's values; arrays of membrane_count, segment_count and
+ * triangle_count). `target` holds 3 per node. `final_position` (3 per node)
+ * and every gradient output are optional; with no gradient output only the
+ * forward pass and the loss run. A failing step restores the structure and
+ * reports the step's error. */
+sw_status sw_adjoint_run(sw_adjoint* adjoint, const double* log_compliance, const double* rest_length,
+                         const double* pressure, const double* target, double* loss, double* final_position,
+                         double* grad_log_compliance, double* grad_rest_length, double* grad_pressure);
+/* Workgroups of the reverse kernel (default: enough for the widest colour). */
+sw_status sw_adjoint_set_workgroups(sw_adjoint* adjoint, int workgroups);
+/* Device milliseconds of the last run: checkpointed forward pass, and
+ * recorded replay + reverse kernels; device bytes the adjoint holds. */
+sw_status sw_adjoint_timing(const sw_adjoint* adjoint, double* forward_ms, double* reverse_ms, size_t* device_bytes);
+
 #ifdef __cplusplus
 }
 #endif

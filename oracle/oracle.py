@@ -9,15 +9,23 @@ from pathlib import Path
 import numpy as np
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "liboracle_xpbd.so"
-_lib = None
+# The same restatement computed in long double (x87 extended precision):
+# its central differences are the low-noise derivative check of the adjoint.
+EXTENDED_LIBRARY_PATH = Path(__file__).resolve().parent / "liboracle_xpbd_ext.so"
+# ... and with forward-mode tangent pairs (exact directional derivatives).
+TANGENT_LIBRARY_PATH = Path(__file__).resolve().parent / "liboracle_xpbd_tan.so"
+_libs = {}
+_PATHS = {False: LIBRARY_PATH, True: EXTENDED_LIBRARY_PATH, "tangent": TANGENT_LIBRARY_PATH}
 
 
-def library():
-    global _lib
-    if _lib is None:
-        if not LIBRARY_PATH.exists():
-            raise RuntimeError(f"{LIBRARY_PATH} is missing: run make -C oracle")
-        lib = C.CDLL(str(LIBRARY_PATH))
+def library(extended=False):
+    """The double build, the long-double build (extended=True) or the
+    tangent build (extended="tangent")."""
+    if extended not in _libs:
+        path = _PATHS[extended]
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: run make -C oracle")
+        lib = C.CDLL(str(path))
         lib.oracle_last_error.restype = C.c_char_p
         lib.oracle_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
         lib.oracle_destroy.argtypes = [C.c_void_p]
@@ -37,8 +45,10 @@ def library():
             getattr(lib, name).restype = None
         lib.oracle_quad_math.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
         lib.oracle_quad_math.restype = None
-        _lib = lib
-    return _lib
+        lib.oracle_set_tangent.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.oracle_tangent_state.argtypes = [C.c_void_p, C.c_void_p]
+        _libs[extended] = lib
+    return _libs[extended]
 
 
 MATH_FUNCTIONS = {"atan2": 2, "sin": 4, "cos": 5, "acos": 6}
@@ -69,9 +79,9 @@ class OracleError(RuntimeError):
         self.status = status
 
 
-def _check(status):
+def _check(status, lib=None):
     if status != 0:
-        raise OracleError(status, library().oracle_last_error().decode())
+        raise OracleError(status, (lib or library()).oracle_last_error().decode())
 
 
 def _p(a):
@@ -79,14 +89,16 @@ def _p(a):
 
 
 class OracleBody:
-    """CPU XPBD body in the reference's coloured sweep order."""
+    """CPU XPBD body in the reference's coloured sweep order (`extended`:
+    True computes in long double, "tangent" with tangent pairs)."""
 
-    def __init__(self, definition):
-        lib = library()
+    def __init__(self, definition, extended=False):
+        lib = library(extended)
+        self._lib = lib
         view = definition.to_ctypes()
         self._definition = definition
         handle = C.c_void_p()
-        _check(lib.oracle_create(C.byref(view), C.byref(handle)))
+        _check(lib.oracle_create(C.byref(view), C.byref(handle)), lib)
         self._handle = handle
         a = definition.arrays
         self.n = a["node_mass"].size
@@ -97,26 +109,39 @@ class OracleBody:
 
     def __del__(self):
         if getattr(self, "_handle", None):
-            library().oracle_destroy(self._handle)
+            self._lib.oracle_destroy(self._handle)
             self._handle = None
 
     def set_face_pressure(self, pressure):
         p = np.ascontiguousarray(pressure, dtype=np.float64)
-        _check(library().oracle_set_face_pressure(self._handle, _p(p), p.size))
+        _check(self._lib.oracle_set_face_pressure(self._handle, _p(p), p.size), self._lib)
 
     def set_external_forces(self, forces):
         f = np.ascontiguousarray(forces, dtype=np.float64).reshape(-1)
-        _check(library().oracle_set_external_forces(self._handle, _p(f), f.size // 3))
+        _check(self._lib.oracle_set_external_forces(self._handle, _p(f), f.size // 3), self._lib)
 
     def step(self, settings):
-        _check(library().oracle_step(self._handle, C.byref(settings)))
+        _check(self._lib.oracle_step(self._handle, C.byref(settings)), self._lib)
 
     def use_portable_math(self, enabled: bool = True):
         """Steps use the portable transcendentals of the device kernels."""
-        library().oracle_set_portable_math(self._handle, int(enabled))
+        self._lib.oracle_set_portable_math(self._handle, int(enabled))
+
+    def set_tangent(self, d_log_compliance=None, d_rest=None, d_pressure=None):
+        """Tangent build: seeds one parameter direction (after the face
+        pressure is set; definition order)."""
+        arrays = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                  for a in (d_log_compliance, d_rest, d_pressure)]
+        self._seeds = arrays
+        _check(self._lib.oracle_set_tangent(self._handle, *[_p(a) for a in arrays]), self._lib)
+
+    def tangent_positions(self) -> np.ndarray:
+        out = np.zeros(3 * self.n)
+        _check(self._lib.oracle_tangent_state(self._handle, _p(out)), self._lib)
+        return out
 
     def perturb_angles(self, ulps: int):
-        library().oracle_set_angle_perturbation(self._handle, ulps)
+        self._lib.oracle_set_angle_perturbation(self._handle, ulps)
 
     def state(self) -> dict:
         out = {
@@ -126,13 +151,13 @@ class OracleBody:
             "dihedral_lambda": np.zeros(self.h), "segment_lambda": np.zeros(self.s), "payload": np.zeros(13),
             "suspension_residual": np.zeros(1),
         }
-        _check(library().oracle_state(self._handle, *[_p(out[k]) for k in (
+        _check(self._lib.oracle_state(self._handle, *[_p(out[k]) for k in (
             "position", "previous", "velocity", "constraint_lambda", "membrane_lambda", "membrane_resultant",
-            "membrane_residual", "dihedral_lambda", "segment_lambda", "payload", "suspension_residual")]))
+            "membrane_residual", "dihedral_lambda", "segment_lambda", "payload", "suspension_residual")]), self._lib)
         return out
 
     def colouring(self, family):
-        lib = library()
+        lib = self._lib
         nc, no = C.c_size_t(), C.c_size_t()
         _check(lib.oracle_colouring(self._handle, family, None, C.byref(nc), None, C.byref(no)))
         order = np.zeros(nc.value, np.uint64)

@@ -28,29 +28,150 @@
 #include <queue>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
+
+// Scalar of every computed quantity: double (the restatement itself), long
+// double (the extended build, ORACLE_REAL), or a forward-mode tangent pair
+// (the tangent build, ORACLE_TANGENT) that carries one directional
+// derivative through the identical operation sequence; the extended build's
+// central differences and the tangent build's derivatives are the checks of
+// the adjoint.
+namespace oracle {
+struct Tangent {
+    double v = 0.0, d = 0.0;
+    Tangent() = default;
+    Tangent(double value) : v(value) {}  // constants carry no derivative
+    Tangent(double value, double derivative) : v(value), d(derivative) {}
+    explicit operator double() const { return v; }
+    Tangent& operator+=(const Tangent& o) { d += o.d; v += o.v; return *this; }
+    Tangent& operator-=(const Tangent& o) { d -= o.d; v -= o.v; return *this; }
+    Tangent& operator*=(const Tangent& o) { d = d * o.v + v * o.d; v *= o.v; return *this; }
+    Tangent& operator/=(const Tangent& o) {
+        const double q = v / o.v;
+        d = (d - q * o.d) / o.v;
+        v = q;
+        return *this;
+    }
+};
+inline Tangent operator+(Tangent a, const Tangent& b) { return a += b; }
+inline Tangent operator-(Tangent a, const Tangent& b) { return a -= b; }
+inline Tangent operator*(Tangent a, const Tangent& b) { return a *= b; }
+inline Tangent operator/(Tangent a, const Tangent& b) { return a /= b; }
+inline Tangent operator+(Tangent a, double b) { return a += Tangent(b); }
+inline Tangent operator-(Tangent a, double b) { return a -= Tangent(b); }
+inline Tangent operator*(Tangent a, double b) { return {a.v * b, a.d * b}; }
+inline Tangent operator/(Tangent a, double b) { return {a.v / b, a.d / b}; }
+inline Tangent operator+(double a, const Tangent& b) { return Tangent(a) += b; }
+inline Tangent operator-(double a, const Tangent& b) { return Tangent(a) -= b; }
+inline Tangent operator*(double a, const Tangent& b) { return {a * b.v, a * b.d}; }
+inline Tangent operator/(double a, const Tangent& b) { return Tangent(a) /= b; }
+inline Tangent operator-(const Tangent& a) { return {-a.v, -a.d}; }
+inline bool operator<(const Tangent& a, const Tangent& b) { return a.v < b.v; }
+inline bool operator>(const Tangent& a, const Tangent& b) { return a.v > b.v; }
+inline bool operator<=(const Tangent& a, const Tangent& b) { return a.v <= b.v; }
+inline bool operator>=(const Tangent& a, const Tangent& b) { return a.v >= b.v; }
+inline bool operator==(const Tangent& a, const Tangent& b) { return a.v == b.v; }
+inline bool operator!=(const Tangent& a, const Tangent& b) { return a.v != b.v; }
+inline bool operator<(const Tangent& a, double b) { return a.v < b; }
+inline bool operator>(const Tangent& a, double b) { return a.v > b; }
+inline bool operator<=(const Tangent& a, double b) { return a.v <= b; }
+inline bool operator>=(const Tangent& a, double b) { return a.v >= b; }
+inline bool operator==(const Tangent& a, double b) { return a.v == b; }
+inline bool operator!=(const Tangent& a, double b) { return a.v != b; }
+inline bool operator<(double a, const Tangent& b) { return a < b.v; }
+inline bool operator>(double a, const Tangent& b) { return a > b.v; }
+inline Tangent sqrt(const Tangent& a) {
+    const double r = std::sqrt(a.v);
+    return {r, r > 0.0 ? a.d / (2.0 * r) : 0.0};
+}
+inline Tangent abs(const Tangent& a) { return a.v < 0.0 ? -a : a; }
+inline bool isfinite(const Tangent& a) { return std::isfinite(a.v); }
+inline Tangent remainder(const Tangent& a, double b) { return {std::remainder(a.v, b), a.d}; }
+inline Tangent atan2(const Tangent& y, const Tangent& x) {
+    const double r2 = x.v * x.v + y.v * y.v;
+    return {std::atan2(y.v, x.v), r2 > 0.0 ? (x.v * y.d - y.v * x.d) / r2 : 0.0};
+}
+inline Tangent sin(const Tangent& a) { return {std::sin(a.v), std::cos(a.v) * a.d}; }
+inline Tangent cos(const Tangent& a) { return {std::cos(a.v), -std::sin(a.v) * a.d}; }
+inline Tangent acos(const Tangent& a) {
+    const double r = 1.0 - a.v * a.v;
+    return {std::acos(a.v), r > 0.0 ? -a.d / std::sqrt(r) : 0.0};
+}
+inline Tangent exp(const Tangent& a) {
+    const double e = std::exp(a.v);
+    return {e, e * a.d};
+}
+inline Tangent nextafter(const Tangent& a, double to) { return {std::nextafter(a.v, to), a.d}; }
+inline double value(const Tangent& a) { return a.v; }
+inline double tangentOf(const Tangent& a) { return a.d; }
+inline double value(double a) { return a; }
+inline double value(long double a) { return static_cast<double>(a); }
+inline double tangentOf(double) { return 0.0; }
+inline double tangentOf(long double) { return 0.0; }
+inline void seedTangent(Tangent& a, double d) { a.d = d; }
+inline void seedTangent(double&, double) {}
+inline void seedTangent(long double&, double) {}
+}  // namespace oracle
+
+#if defined(ORACLE_TANGENT)
+using Real = oracle::Tangent;
+#elif defined(ORACLE_REAL)
+using Real = ORACLE_REAL;
+#else
+using Real = double;
+#endif
+constexpr bool kTangentBuild = std::is_same_v<Real, oracle::Tangent>;
+
+using oracle::seedTangent;
+using oracle::tangentOf;
+using oracle::value;
+using std::abs;
+using std::acos;
+using std::atan2;
+using std::cos;
+using std::exp;
+using std::isfinite;
+using std::nextafter;
+using std::remainder;
+using std::sin;
+using std::sqrt;
 
 namespace {
 
 thread_local std::string lastError;
 
 // Transcendentals: the C library's, as the reference calls them (default),
-// or the portable double-double functions that the device kernels use
+// or the portable Real-Real functions that the device kernels use
 // (oracle_set_portable_math), which return the same bits on every platform.
+// In the extended and tangent builds the C library's functions are used
+// (the portable ones are off).
 thread_local bool portableMath = false;
-double mAtan2(double y, double x) { return portableMath ? swx::pm::atan2(y, x) : std::atan2(y, x); }
-double mSin(double x) { return portableMath ? swx::pm::sin(x) : std::sin(x); }
-double mCos(double x) { return portableMath ? swx::pm::cos(x) : std::cos(x); }
-double mAcos(double x) { return portableMath ? swx::pm::acos(x) : std::acos(x); }
+Real mAtan2(Real y, Real x) {
+    if constexpr (std::is_same_v<Real, double>) return portableMath ? Real(swx::pm::atan2(value(y), value(x))) : atan2(y, x);
+    else return atan2(y, x);
+}
+Real mSin(Real x) {
+    if constexpr (std::is_same_v<Real, double>) return portableMath ? Real(swx::pm::sin(value(x))) : sin(x);
+    else return sin(x);
+}
+Real mCos(Real x) {
+    if constexpr (std::is_same_v<Real, double>) return portableMath ? Real(swx::pm::cos(value(x))) : cos(x);
+    else return cos(x);
+}
+Real mAcos(Real x) {
+    if constexpr (std::is_same_v<Real, double>) return portableMath ? Real(swx::pm::acos(value(x))) : acos(x);
+    else return acos(x);
+}
 
 struct V {
-    double x = 0.0, y = 0.0, z = 0.0;
+    Real x = 0.0, y = 0.0, z = 0.0;
 };
 V operator+(V a, const V& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 V operator-(V a, const V& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-V operator*(V a, double s) { return {a.x * s, a.y * s, a.z * s}; }
-V operator*(double s, V a) { return {a.x * s, a.y * s, a.z * s}; }
-V operator/(V a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+V operator*(V a, Real s) { return {a.x * s, a.y * s, a.z * s}; }
+V operator*(Real s, V a) { return {a.x * s, a.y * s, a.z * s}; }
+V operator/(V a, Real s) { return {a.x / s, a.y / s, a.z / s}; }
 V operator-(V a) { return {-a.x, -a.y, -a.z}; }
 V& operator+=(V& a, const V& b) {
     a.x += b.x;
@@ -64,37 +185,37 @@ V& operator-=(V& a, const V& b) {
     a.z -= b.z;
     return a;
 }
-double dot(const V& a, const V& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+Real dot(const V& a, const V& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 V cross(const V& a, const V& b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-double length(const V& a) { return std::sqrt(dot(a, a)); }
-bool finite(const V& a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
+Real length(const V& a) { return sqrt(dot(a, a)); }
+bool finite(const V& a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
 
 struct Sym {
-    double xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
+    Real xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
 };
 V operator*(const Sym& m, const V& v) {
     return {m.xx * v.x + m.xy * v.y + m.xz * v.z, m.xy * v.x + m.yy * v.y + m.yz * v.z,
             m.xz * v.x + m.yz * v.y + m.zz * v.z};
 }
-Sym operator*(double s, const Sym& m) { return {s * m.xx, s * m.yy, s * m.zz, s * m.xy, s * m.xz, s * m.yz}; }
+Sym operator*(Real s, const Sym& m) { return {s * m.xx, s * m.yy, s * m.zz, s * m.xy, s * m.xz, s * m.yz}; }
 Sym operator+(const Sym& a, const Sym& b) {
     return {a.xx + b.xx, a.yy + b.yy, a.zz + b.zz, a.xy + b.xy, a.xz + b.xz, a.yz + b.yz};
 }
 bool finite(const Sym& m) {
-    return std::isfinite(m.xx) && std::isfinite(m.yy) && std::isfinite(m.zz) && std::isfinite(m.xy) &&
-           std::isfinite(m.xz) && std::isfinite(m.yz);
+    return isfinite(m.xx) && isfinite(m.yy) && isfinite(m.zz) && isfinite(m.xy) &&
+           isfinite(m.xz) && isfinite(m.yz);
 }
 
 // membrane.cpp:208-235
 bool symInverse(const Sym& m, Sym& out) {
     if (!finite(m)) return false;
-    const double scale = std::max({std::abs(m.xx), std::abs(m.yy), std::abs(m.zz), std::abs(m.xy),
-                                   std::abs(m.xz), std::abs(m.yz)});
-    const double det = m.xx * (m.yy * m.zz - m.yz * m.yz) - m.xy * (m.xy * m.zz - m.yz * m.xz) +
+    const Real scale = std::max({abs(m.xx), abs(m.yy), abs(m.zz), abs(m.xy),
+                                   abs(m.xz), abs(m.yz)});
+    const Real det = m.xx * (m.yy * m.zz - m.yz * m.yz) - m.xy * (m.xy * m.zz - m.yz * m.xz) +
                        m.xz * (m.xy * m.yz - m.yy * m.xz);
-    const double tol = 128.0 * std::numeric_limits<double>::epsilon() * scale * scale * scale;
-    if (!(scale > 0.0) || !std::isfinite(det) || std::abs(det) <= tol) return false;
-    const double inv = 1.0 / det;
+    const Real tol = 128.0 * std::numeric_limits<double>::epsilon() * scale * scale * scale;
+    if (!(scale > 0.0) || !isfinite(det) || abs(det) <= tol) return false;
+    const Real inv = 1.0 / det;
     out = {(m.yy * m.zz - m.yz * m.yz) * inv, (m.xx * m.zz - m.xz * m.xz) * inv,
            (m.xx * m.yy - m.xy * m.xy) * inv, (m.xz * m.yz - m.xy * m.zz) * inv,
            (m.xy * m.yz - m.xz * m.yy) * inv, (m.xy * m.xz - m.xx * m.yz) * inv};
@@ -104,49 +225,49 @@ bool symInverse(const Sym& m, Sym& out) {
 // membrane.cpp:186-206
 bool symPositiveDefinite(const Sym& m) {
     if (!finite(m) || !(m.xx > 0.0) || !(m.yy > 0.0) || !(m.zz > 0.0)) return false;
-    const double sx = std::sqrt(m.xx), sy = std::sqrt(m.yy), sz = std::sqrt(m.zz);
-    const double l10 = m.xy / sx / sy, l20 = m.xz / sx / sz;
-    const double p1 = 1.0 - l10 * l10;
-    if (!std::isfinite(p1) || !(p1 > 0.0)) return false;
-    const double l11 = std::sqrt(p1);
-    const double l21 = (m.yz / sy / sz - l20 * l10) / l11;
-    const double p2 = 1.0 - l20 * l20 - l21 * l21;
-    return std::isfinite(p2) && p2 > 0.0;
+    const Real sx = sqrt(m.xx), sy = sqrt(m.yy), sz = sqrt(m.zz);
+    const Real l10 = m.xy / sx / sy, l20 = m.xz / sx / sz;
+    const Real p1 = 1.0 - l10 * l10;
+    if (!isfinite(p1) || !(p1 > 0.0)) return false;
+    const Real l11 = sqrt(p1);
+    const Real l21 = (m.yz / sy / sz - l20 * l10) / l11;
+    const Real p2 = 1.0 - l20 * l20 - l21 * l21;
+    return isfinite(p2) && p2 > 0.0;
 }
 
 // membrane.cpp:31-127
 V scaledSolve(const Sym& m, const V& b) {
     const auto fail = [] { throw std::invalid_argument("SymmetricMatrix3 is singular or ill-conditioned"); };
-    const double sx = std::sqrt(m.xx), sy = std::sqrt(m.yy), sz = std::sqrt(m.zz);
-    if (!(sx > 0.0) || !(sy > 0.0) || !(sz > 0.0) || !std::isfinite(sx) || !std::isfinite(sy) || !std::isfinite(sz))
+    const Real sx = sqrt(m.xx), sy = sqrt(m.yy), sz = sqrt(m.zz);
+    if (!(sx > 0.0) || !(sy > 0.0) || !(sz > 0.0) || !isfinite(sx) || !isfinite(sy) || !isfinite(sz))
         fail();
-    const double l10 = m.xy / sx / sy, l20 = m.xz / sx / sz;
-    const double p1 = 1.0 - l10 * l10;
-    const double minPivot = 4096.0 * std::numeric_limits<double>::epsilon();
-    if (!std::isfinite(p1) || !(p1 > minPivot)) fail();
-    const double l11 = std::sqrt(p1);
-    const double l21 = (m.yz / sy / sz - l20 * l10) / l11;
-    const double p2 = 1.0 - l20 * l20 - l21 * l21;
-    if (!std::isfinite(l21) || !std::isfinite(p2) || !(p2 > minPivot)) fail();
-    const double l22 = std::sqrt(p2);
+    const Real l10 = m.xy / sx / sy, l20 = m.xz / sx / sz;
+    const Real p1 = 1.0 - l10 * l10;
+    const Real minPivot = 4096.0 * std::numeric_limits<double>::epsilon();
+    if (!isfinite(p1) || !(p1 > minPivot)) fail();
+    const Real l11 = sqrt(p1);
+    const Real l21 = (m.yz / sy / sz - l20 * l10) / l11;
+    const Real p2 = 1.0 - l20 * l20 - l21 * l21;
+    if (!isfinite(l21) || !isfinite(p2) || !(p2 > minPivot)) fail();
+    const Real l22 = sqrt(p2);
     const V s{b.x / sx, b.y / sy, b.z / sz};
-    const double f0 = s.x;
-    const double f1 = (s.y - l10 * f0) / l11;
-    const double f2 = (s.z - l20 * f0 - l21 * f1) / l22;
-    const double rz = f2 / l22;
-    const double ry = (f1 - l21 * rz) / l11;
-    const double rx = f0 - l10 * ry - l20 * rz;
+    const Real f0 = s.x;
+    const Real f1 = (s.y - l10 * f0) / l11;
+    const Real f2 = (s.z - l20 * f0 - l21 * f1) / l22;
+    const Real rz = f2 / l22;
+    const Real ry = (f1 - l21 * rz) / l11;
+    const Real rx = f0 - l10 * ry - l20 * rz;
     const V r{rx / sx, ry / sy, rz / sz};
     if (!finite(r)) throw std::invalid_argument("SymmetricMatrix3 solve result is non-finite");
     const V res = m * r - b;
-    const double ax = std::abs(m.xx * r.x) + std::abs(m.xy * r.y) + std::abs(m.xz * r.z) + std::abs(b.x);
-    const double ay = std::abs(m.xy * r.x) + std::abs(m.yy * r.y) + std::abs(m.yz * r.z) + std::abs(b.y);
-    const double az = std::abs(m.xz * r.x) + std::abs(m.yz * r.y) + std::abs(m.zz * r.z) + std::abs(b.z);
-    const double rel = 4096.0 * std::numeric_limits<double>::epsilon();
-    if (!finite(res) || !std::isfinite(ax) || !std::isfinite(ay) || !std::isfinite(az) ||
-        (ax == 0.0 ? res.x != 0.0 : std::abs(res.x) > rel * ax) ||
-        (ay == 0.0 ? res.y != 0.0 : std::abs(res.y) > rel * ay) ||
-        (az == 0.0 ? res.z != 0.0 : std::abs(res.z) > rel * az))
+    const Real ax = abs(m.xx * r.x) + abs(m.xy * r.y) + abs(m.xz * r.z) + abs(b.x);
+    const Real ay = abs(m.xy * r.x) + abs(m.yy * r.y) + abs(m.yz * r.z) + abs(b.y);
+    const Real az = abs(m.xz * r.x) + abs(m.yz * r.y) + abs(m.zz * r.z) + abs(b.z);
+    const Real rel = 4096.0 * std::numeric_limits<double>::epsilon();
+    if (!finite(res) || !isfinite(ax) || !isfinite(ay) || !isfinite(az) ||
+        (ax == 0.0 ? res.x != 0.0 : abs(res.x) > rel * ax) ||
+        (ay == 0.0 ? res.y != 0.0 : abs(res.y) > rel * ay) ||
+        (az == 0.0 ? res.z != 0.0 : abs(res.z) > rel * az))
         fail();
     return r;
 }
@@ -161,12 +282,12 @@ V checkedSolve(const Sym& m, const V& b) {
 }
 
 struct Q {
-    double w = 1, x = 0, y = 0, z = 0;
+    Real w = 1, x = 0, y = 0, z = 0;
 };
 struct M3 {
-    double a[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    double& operator()(int r, int c) { return a[r * 3 + c]; }
-    double operator()(int r, int c) const { return a[r * 3 + c]; }
+    Real a[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    Real& operator()(int r, int c) { return a[r * 3 + c]; }
+    Real operator()(int r, int c) const { return a[r * 3 + c]; }
 };
 
 struct SuspensionFail : std::runtime_error {
@@ -177,7 +298,7 @@ struct SuspensionFail : std::runtime_error {
 [[noreturn]] void fail(int phase, const std::string& entity, const std::string& message) {
     throw SuspensionFail(phase, entity, message);
 }
-bool finite(const Q& q) { return std::isfinite(q.w) && std::isfinite(q.x) && std::isfinite(q.y) && std::isfinite(q.z); }
+bool finite(const Q& q) { return isfinite(q.w) && isfinite(q.x) && isfinite(q.y) && isfinite(q.z); }
 
 // suspension.cpp:553-684
 M3 transpose(const M3& m) {
@@ -186,19 +307,19 @@ M3 transpose(const M3& m) {
         for (int j = 0; j < 3; ++j) r(i, j) = m(j, i);
     return r;
 }
-double det3(const M3& m) {
+Real det3(const M3& m) {
     return m(0, 0) * (m(1, 1) * m(2, 2) - m(1, 2) * m(2, 1)) - m(0, 1) * (m(1, 0) * m(2, 2) - m(1, 2) * m(2, 0)) +
            m(0, 2) * (m(1, 0) * m(2, 1) - m(1, 1) * m(2, 0));
 }
 M3 inverse3(const M3& m) {
-    double scale = std::abs(m.a[0]);
+    Real scale = abs(m.a[0]);
     for (int k = 1; k < 9; ++k)
-        if (scale < std::abs(m.a[k])) scale = std::abs(m.a[k]);
-    if (!(scale > 0.0) || !std::isfinite(scale)) fail(1, "inertia", "singular matrix");
+        if (scale < abs(m.a[k])) scale = abs(m.a[k]);
+    if (!(scale > 0.0) || !isfinite(scale)) fail(1, "inertia", "singular matrix");
     M3 n = m;
-    for (double& v : n.a) v /= scale;
-    const double det = det3(n);
-    if (!std::isfinite(det) || std::abs(det) <= 1.0e-14) fail(1, "inertia", "singular matrix");
+    for (Real& v : n.a) v /= scale;
+    const Real det = det3(n);
+    if (!isfinite(det) || abs(det) <= 1.0e-14) fail(1, "inertia", "singular matrix");
     M3 r;
     r.a[0] = n(1, 1) * n(2, 2) - n(1, 2) * n(2, 1);
     r.a[1] = n(0, 2) * n(2, 1) - n(0, 1) * n(2, 2);
@@ -209,7 +330,7 @@ M3 inverse3(const M3& m) {
     r.a[6] = n(1, 0) * n(2, 1) - n(1, 1) * n(2, 0);
     r.a[7] = n(0, 1) * n(2, 0) - n(0, 0) * n(2, 1);
     r.a[8] = n(0, 0) * n(1, 1) - n(0, 1) * n(1, 0);
-    for (double& v : r.a) v /= det * scale;
+    for (Real& v : r.a) v /= det * scale;
     return r;
 }
 V operator*(const M3& m, const V& v) {
@@ -218,16 +339,16 @@ V operator*(const M3& m, const V& v) {
 }
 M3 operator*(const M3& a, const M3& b) {
     M3 r;
-    for (double& v : r.a) v = 0.0;
+    for (Real& v : r.a) v = 0.0;
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             for (int k = 0; k < 3; ++k) r(i, j) += a(i, k) * b(k, j);
     return r;
 }
 Q canonical(const Q& q) {
-    const double n2 = q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z;
+    const Real n2 = q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z;
     if (!finite(q) || !(n2 > 1.0e-28)) fail(1, "orientation", "quaternion is non-finite or near zero");
-    const double inv = 1.0 / std::sqrt(n2);
+    const Real inv = 1.0 / sqrt(n2);
     Q r{q.w * inv, q.x * inv, q.y * inv, q.z * inv};
     if (r.w < 0.0 || (r.w == 0.0 && (r.x < 0.0 || (r.x == 0.0 && (r.y < 0.0 || (r.y == 0.0 && r.z < 0.0)))))) {
         r = {-r.w, -r.x, -r.y, -r.z};
@@ -240,10 +361,10 @@ Q operator*(const Q& a, const Q& b) {
 }
 M3 rotation(const Q& orientation) {
     const Q q = canonical(orientation);
-    const double xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z, xy = q.x * q.y, xz = q.x * q.z, yz = q.y * q.z;
-    const double wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
+    const Real xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z, xy = q.x * q.y, xz = q.x * q.z, yz = q.y * q.z;
+    const Real wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
     M3 r;
-    const double v[9] = {1.0 - 2.0 * (yy + zz), 2.0 * (xy - wz),       2.0 * (xz + wy),
+    const Real v[9] = {1.0 - 2.0 * (yy + zz), 2.0 * (xy - wz),       2.0 * (xz + wy),
                          2.0 * (xy + wz),       1.0 - 2.0 * (xx + zz), 2.0 * (yz - wx),
                          2.0 * (xz - wy),       2.0 * (yz + wx),       1.0 - 2.0 * (xx + yy)};
     std::memcpy(r.a, v, sizeof(v));
@@ -251,50 +372,51 @@ M3 rotation(const Q& orientation) {
 }
 Q increment(const V& v) {
     if (!finite(v)) fail(5, "rotation-increment", "non-finite rotation vector");
-    const double angle = length(v);
+    const Real angle = length(v);
     if (angle < 1.0e-12) return canonical({1.0, 0.5 * v.x, 0.5 * v.y, 0.5 * v.z});
-    const double half = 0.5 * angle;
-    const double s = mSin(half) / angle;
+    const Real half = 0.5 * angle;
+    const Real s = mSin(half) / angle;
     return {mCos(half), v.x * s, v.y * s, v.z * s};
 }
 
 struct Node {
     V position, previous, velocity, force;
-    double w = 0.0;
+    Real w = 0.0;
 };
 struct Constraint {
     std::size_t a, b;
-    double rest, compliance, lambda = 0.0;
+    Real rest, compliance, lambda = 0.0;
     int kind;
 };
 struct Membrane {
     std::size_t a, b, c;
-    double m00, m01, m10, m11, area;
+    Real m00, m01, m10, m11, area;
     Sym compliance;
-    double preW, preT, dampingTime;
+    Real preW, preT, dampingTime;
     V lambda, resultant;
-    double residual = 0.0;
+    Real residual = 0.0;
 };
 struct Hinge {
     std::size_t a, b, c, d;
-    double rest, compliance, lambda = 0.0;
+    Real rest, compliance, lambda = 0.0;
 };
 struct Segment {
     std::string id;
     long from, to;  // node index, or -(hang + 1)
-    double rest, stiffness, lambda = 0.0;
+    Real rest, stiffness, lambda = 0.0;
+    std::size_t definition = 0;  // index in the definition's segment arrays
 };
 struct Pair {
     std::size_t node, centre;
-    double radius, compliance, lambda = 0.0;
+    Real radius, compliance, lambda = 0.0;
 };
 struct Payload {
     V com, v, w, comLocal, prevCom;
     Q q, prevQ;
-    double mass = 0.0;
+    Real mass = 0.0;
     M3 inertia;
     std::vector<V> hang;
-    double minLength = 1e-10, maxResidual = 2e-4, certified = 0.0;
+    Real minLength = 1e-10, maxResidual = 2e-4, certified = 0.0;
     int iterations = 12;
 };
 
@@ -303,7 +425,7 @@ struct Payload {
 struct oracle_body {
     std::vector<Node> nodes;
     std::vector<std::array<std::size_t, 3>> triangles;
-    std::vector<double> pressure;
+    std::vector<Real> pressure;
     std::vector<Constraint> constraints;
     std::vector<Membrane> membranes;
     std::vector<Hinge> hinges;              // colour-major insertion order
@@ -323,28 +445,28 @@ struct oracle_body {
 namespace {
 
 // soft_body.cpp:1598-1640
-void projectDistance(oracle_body& b, Constraint& c, double invDt2) {
+void projectDistance(oracle_body& b, Constraint& c, Real invDt2) {
     V& pa = b.nodes[c.a].position;
     V& pb = b.nodes[c.b].position;
-    const double wa = b.nodes[c.a].w, wb = b.nodes[c.b].w;
+    const Real wa = b.nodes[c.a].w, wb = b.nodes[c.b].w;
     const V d = pb - pa;
-    const double L = length(d);
+    const Real L = length(d);
     if (L <= 1.0e-12) return;
-    const double value = L - c.rest;
+    const Real value = L - c.rest;
     if (c.kind == SW_CONSTRAINT_CABLE && value <= 0.0 && c.lambda == 0.0) return;
-    const double wsum = wa + wb;
-    const double alpha = c.compliance * invDt2;
+    const Real wsum = wa + wb;
+    const Real alpha = c.compliance * invDt2;
     if (wsum + alpha <= 0.0) return;
-    const double dl = (-value - alpha * c.lambda) / (wsum + alpha);
-    double applied = dl;
+    const Real dl = (-value - alpha * c.lambda) / (wsum + alpha);
+    Real applied = dl;
     if (c.kind == SW_CONSTRAINT_CABLE) {
-        const double next = std::min(0.0, c.lambda + dl);
+        const Real next = std::min<Real>(0.0, c.lambda + dl);
         applied = next - c.lambda;
         c.lambda = next;
     } else {
         c.lambda += dl;
     }
-    const double s = applied / L;
+    const Real s = applied / L;
     pa -= (wa * s) * d;
     pb += (wb * s) * d;
 }
@@ -375,16 +497,16 @@ MemEval evaluate(const oracle_body& b, const Membrane& m) {
 }
 
 // soft_body.cpp:1822-1929
-void projectMembrane(oracle_body& b, Membrane& m, double dt) {
+void projectMembrane(oracle_body& b, Membrane& m, Real dt) {
     const MemEval e = evaluate(b, m);
     const std::size_t idx[3] = {m.a, m.b, m.c};
-    const double s = std::sqrt(m.area);
+    const Real s = sqrt(m.area);
     V J[3][3];
     for (int k = 0; k < 3; ++k)
         for (int c = 0; c < 3; ++c) J[k][c] = e.g[k][c] * s;
     Sym M;
     for (int c = 0; c < 3; ++c) {
-        const double w = b.nodes[idx[c]].w;
+        const Real w = b.nodes[idx[c]].w;
         M.xx += w * dot(J[0][c], J[0][c]);
         M.yy += w * dot(J[1][c], J[1][c]);
         M.zz += w * dot(J[2][c], J[2][c]);
@@ -399,9 +521,9 @@ void projectMembrane(oracle_body& b, Membrane& m, double dt) {
         dm.y += dot(J[1][c], disp);
         dm.z += dot(J[2][c], disp);
     }
-    const double invDt2 = 1.0 / (dt * dt);
+    const Real invDt2 = 1.0 / (dt * dt);
     const Sym at = invDt2 * m.compliance;
-    const double gamma = m.dampingTime / dt;
+    const Real gamma = m.dampingTime / dt;
     const Sym S = (1.0 + gamma) * M + at;
     const V rhs = -(s * e.strain) - at * m.lambda - gamma * dm;
     const V dl = checkedSolve(S, rhs);
@@ -412,24 +534,24 @@ void projectMembrane(oracle_body& b, Membrane& m, double dt) {
     for (int c = 0; c < 3; ++c) b.nodes[idx[c]].position += corr[c];
 }
 
-double perturbed(double v, int ulps) {
-    for (int k = 0; k < ulps; ++k) v = std::nextafter(v, 1e300);
-    for (int k = 0; k < -ulps; ++k) v = std::nextafter(v, -1e300);
+Real perturbed(Real v, int ulps) {
+    for (int k = 0; k < ulps; ++k) v = nextafter(v, 1e300);
+    for (int k = 0; k < -ulps; ++k) v = nextafter(v, -1e300);
     return v;
 }
 
 // soft_body.cpp:1648-1711 and 1754-1792
-void projectHinge(oracle_body& b, Hinge& h, double invDt2) {
+void projectHinge(oracle_body& b, Hinge& h, Real invDt2) {
     const std::size_t idx[4] = {h.a, h.b, h.c, h.d};
     const V a = b.nodes[h.a].position, bb = b.nodes[h.b].position, c = b.nodes[h.c].position,
             d = b.nodes[h.d].position;
     const V edge = bb - a, toC = c - a, toD = d - a;
     const V area1 = cross(edge, toC), area2 = cross(toD, edge);
-    const double el = length(edge), l1 = length(area1), l2 = length(area2);
+    const Real el = length(edge), l1 = length(area1), l2 = length(area2);
     if (!(el > 1.0e-12) || !(l1 > 1.0e-12 * el) || !(l2 > 1.0e-12 * el)) return;
     const V eu = edge / el, n1 = area1 / l1, n2 = area2 / l2;
-    const double cosine = std::clamp(dot(n1, n2), -1.0, 1.0);
-    const double sine = dot(eu, cross(n1, n2));
+    const Real cosine = std::clamp<Real>(dot(n1, n2), -1.0, 1.0);
+    const Real sine = dot(eu, cross(n1, n2));
     const V n1a = cosine * cross(n2, eu) - sine * n2;
     const V n2a = cosine * cross(eu, n1) - sine * n1;
     const V eua = cosine * cross(n1, n2);
@@ -440,20 +562,20 @@ void projectHinge(oracle_body& b, Hinge& h, double invDt2) {
     const V ca = cross(a1a, edge);
     ea += cross(a2a, toD);
     const V da = cross(edge, a2a);
-    double angle = mAtan2(sine, cosine);
+    Real angle = mAtan2(sine, cosine);
     if (b.perturbUlps) angle = perturbed(angle, b.perturbUlps);
     const V g[4] = {-1.0 * (ea + ca + da), ea, ca, da};
-    if (!std::isfinite(angle)) return;
+    if (!isfinite(angle)) return;
     for (const V& v : g)
         if (!finite(v)) return;
-    double W = 0.0;
+    Real W = 0.0;
     for (int k = 0; k < 4; ++k) W += b.nodes[idx[k]].w * dot(g[k], g[k]);
-    const double alpha = h.compliance * invDt2;
-    const double den = W + alpha;
-    if (!(den > 0.0) || !std::isfinite(den)) return;
-    const double value = std::remainder(angle - h.rest, 6.283185307179586476925286766559);
-    const double dl = (-value - alpha * h.lambda) / den;
-    if (!std::isfinite(dl)) return;
+    const Real alpha = h.compliance * invDt2;
+    const Real den = W + alpha;
+    if (!(den > 0.0) || !isfinite(den)) return;
+    const Real value = remainder(angle - h.rest, 6.283185307179586476925286766559);
+    const Real dl = (-value - alpha * h.lambda) / den;
+    if (!isfinite(dl)) return;
     for (int k = 0; k < 4; ++k) b.nodes[idx[k]].position += b.nodes[idx[k]].w * dl * g[k];
     h.lambda += dl;
 }
@@ -466,12 +588,12 @@ M3 worldInertia(const Payload& p, const Q& q) {
 V hangPosition(const Payload& p, std::size_t h) { return p.com + rotation(p.q) * (p.hang[h] - p.comLocal); }
 
 // suspension.cpp:1075-1102 within beginSubstep (suspension.cpp:2072-2182)
-void beginPayload(Payload& p, double dt, const V& gravity) {
+void beginPayload(Payload& p, Real dt, const V& gravity) {
     p.prevCom = p.com;
     p.prevQ = p.q;
     const V force = V{} + p.mass * gravity;
     const V moment{};
-    if (!(dt > 0.0) || !std::isfinite(dt) || !finite(force) || !finite(moment))
+    if (!(dt > 0.0) || !isfinite(dt) || !finite(force) || !finite(moment))
         fail(5, "payload-step", "invalid payload integration input");
     p.q = canonical(p.q);
     const M3 inertia = worldInertia(p, p.q);
@@ -488,26 +610,26 @@ void beginPayload(Payload& p, double dt, const V& gravity) {
 
 struct EndState {
     V position, offset;
-    double w = 0.0;
+    Real w = 0.0;
     bool rigid = false, fixed = false;
 };
 
 // suspension.cpp:1104-1123
-double effective(const EndState& e, bool rigid, const V& g, const M3& iinv) {
-    bool ok = e.w >= 0.0 && std::isfinite(e.w) && finite(g) && finite(e.offset);
-    for (double v : iinv.a) ok = ok && std::isfinite(v);
+Real effective(const EndState& e, bool rigid, const V& g, const M3& iinv) {
+    bool ok = e.w >= 0.0 && isfinite(e.w) && finite(g) && finite(e.offset);
+    for (Real v : iinv.a) ok = ok && isfinite(v);
     if (!ok) fail(6, "effective-mass", "non-finite generalized endpoint");
-    double r = e.w;
+    Real r = e.w;
     if (rigid && e.w > 0.0) {
         const V rot = cross(e.offset, g);
         r += dot(rot, iinv * rot);
     }
-    if (!(r >= 0.0) || !std::isfinite(r)) fail(6, "effective-mass", "invalid generalized effective inverse mass");
+    if (!(r >= 0.0) || !isfinite(r)) fail(6, "effective-mass", "invalid generalized effective inverse mass");
     return r;
 }
 
 // suspension.cpp:2197-2274 with projectUnilateralLine (1125-1163)
-void lineIteration(oracle_body& b, double dt) {
+void lineIteration(oracle_body& b, Real dt) {
     Payload& p = b.payload;
     const M3 iinv = inverse3(worldInertia(p, p.q));
     const auto state = [&](long e) {
@@ -525,7 +647,7 @@ void lineIteration(oracle_body& b, double dt) {
         }
         return s;
     };
-    const auto apply = [&](long e, const EndState& s, const V& g, double inc) {
+    const auto apply = [&](long e, const EndState& s, const V& g, Real inc) {
         if (s.fixed) return;
         if (e >= 0) {
             b.nodes[e].position += s.w * inc * g;
@@ -538,21 +660,21 @@ void lineIteration(oracle_body& b, double dt) {
     for (Segment& seg : b.segments) {
         const EndState from = state(seg.from), to = state(seg.to);
         const V diff = to.position - from.position;
-        const double L = length(diff);
-        if (!std::isfinite(L) || L <= p.minLength) fail(6, seg.id, "line geometry is non-finite or scale-degenerate");
+        const Real L = length(diff);
+        if (!isfinite(L) || L <= p.minLength) fail(6, seg.id, "line geometry is non-finite or scale-degenerate");
         const V dir = diff / L;
-        if (!(L > 0.0) || !std::isfinite(L) || !(seg.rest > 0.0) || !std::isfinite(seg.rest) ||
-            !(seg.stiffness > 0.0) || !std::isfinite(seg.stiffness) || !(dt > 0.0) || !std::isfinite(dt) ||
-            !(seg.lambda >= 0.0) || !std::isfinite(seg.lambda) || std::abs(length(dir) - 1.0) > 1.0e-10)
+        if (!(L > 0.0) || !isfinite(L) || !(seg.rest > 0.0) || !isfinite(seg.rest) ||
+            !(seg.stiffness > 0.0) || !isfinite(seg.stiffness) || !(dt > 0.0) || !isfinite(dt) ||
+            !(seg.lambda >= 0.0) || !isfinite(seg.lambda) || abs(length(dir) - 1.0) > 1.0e-10)
             fail(6, "line-projection", "invalid unilateral line projection input");
-        const double eff = effective(from, from.rigid && !from.fixed, dir, iinv) +
+        const Real eff = effective(from, from.rigid && !from.fixed, dir, iinv) +
                            effective(to, to.rigid && !to.fixed, -dir, iinv);
-        const double at = 1.0 / (seg.stiffness * dt * dt);
+        const Real at = 1.0 / (seg.stiffness * dt * dt);
         if (!(eff + at > 0.0)) fail(6, "line-projection", "zero generalized effective mass");
-        const double gap = seg.rest - L;
-        const double candidate = seg.lambda + (-gap - at * seg.lambda) / (eff + at);
-        const double next = std::max(0.0, candidate);
-        const double inc = next - seg.lambda;
+        const Real gap = seg.rest - L;
+        const Real candidate = seg.lambda + (-gap - at * seg.lambda) / (eff + at);
+        const Real next = std::max<Real>(0.0, candidate);
+        const Real inc = next - seg.lambda;
         seg.lambda = next;
         apply(seg.from, from, dir, inc);
         apply(seg.to, to, -dir, inc);
@@ -560,22 +682,22 @@ void lineIteration(oracle_body& b, double dt) {
 }
 
 // Build-defined contact spheres: node held off the (unmoved) centre node.
-void projectPair(oracle_body& b, Pair& pr, double invDt2) {
+void projectPair(oracle_body& b, Pair& pr, Real invDt2) {
     const V c = b.nodes[pr.centre].position;
     V& x = b.nodes[pr.node].position;
     const V d = x - c;
-    const double L = length(d);
+    const Real L = length(d);
     if (L <= 1.0e-12) return;
-    const double gap = L - pr.radius;
+    const Real gap = L - pr.radius;
     if (gap >= 0.0 && pr.lambda == 0.0) return;
-    const double w = b.nodes[pr.node].w;
-    const double alpha = pr.compliance * invDt2;
+    const Real w = b.nodes[pr.node].w;
+    const Real alpha = pr.compliance * invDt2;
     if (w + alpha <= 0.0) return;
-    const double dl = (-gap - alpha * pr.lambda) / (w + alpha);
-    const double next = std::max(0.0, pr.lambda + dl);
-    const double applied = next - pr.lambda;
+    const Real dl = (-gap - alpha * pr.lambda) / (w + alpha);
+    const Real next = std::max<Real>(0.0, pr.lambda + dl);
+    const Real applied = next - pr.lambda;
     pr.lambda = next;
-    const double s = applied / L;
+    const Real s = applied / L;
     x += (w * s) * d;
 }
 
@@ -621,7 +743,7 @@ std::vector<std::size_t> loadPathOrder(const oracle_body& b) {
     visit();
     for (;;) {
         std::size_t root = n;
-        double smallest = std::numeric_limits<double>::infinity();
+        Real smallest = std::numeric_limits<double>::infinity();
         for (std::size_t i = 0; i < n; ++i)
             if (depth[i] == unknown && !adj[i].empty() && b.nodes[i].w < smallest) {
                 root = i;
@@ -640,7 +762,7 @@ std::vector<std::size_t> loadPathOrder(const oracle_body& b) {
 }
 
 // soft_body.cpp:1175-1460 (coupled, workerThreads >= 1) for one substep.
-void substep(oracle_body& b, double dt, const sw_step_settings& st) {
+void substep(oracle_body& b, Real dt, const sw_step_settings& st) {
     // Pressure gather in ascending triangle order (soft_body.cpp:1492-1534).
     {
         std::vector<V> total(b.nodes.size());
@@ -660,7 +782,7 @@ void substep(oracle_body& b, double dt, const sw_step_settings& st) {
     }
     const V gravity{st.gravity[0], st.gravity[1], st.gravity[2]};
     if (b.hasSuspension) beginPayload(b.payload, dt, gravity);
-    const double damping = std::exp(-st.velocity_damping * dt);
+    const Real damping = exp(-st.velocity_damping * dt);
     const V ref{st.damping_reference_velocity[0], st.damping_reference_velocity[1], st.damping_reference_velocity[2]};
     for (Node& n : b.nodes) {
         n.previous = n.position;
@@ -687,7 +809,7 @@ void substep(oracle_body& b, double dt, const sw_step_settings& st) {
     for (auto& s : b.segments) s.lambda = 0.0;
     for (auto& p : b.pairs) p.lambda = 0.0;
 
-    const double invDt2 = 1.0 / (dt * dt);
+    const Real invDt2 = 1.0 / (dt * dt);
     const auto cablePair = [&] {
         for (auto it = b.loadPath.rbegin(); it != b.loadPath.rend(); ++it) projectDistance(b, b.constraints[*it], invDt2);
         for (auto k : b.loadPath) projectDistance(b, b.constraints[k], invDt2);
@@ -717,9 +839,9 @@ void substep(oracle_body& b, double dt, const sw_step_settings& st) {
     for (Membrane& m : b.membranes) {
         const MemEval e = evaluate(b, m);
         const std::size_t idx[3] = {m.a, m.b, m.c};
-        const double s = std::sqrt(m.area);
+        const Real s = sqrt(m.area);
         const Sym at = invDt2 * m.compliance;
-        const double gamma = m.dampingTime / dt;
+        const Real gamma = m.dampingTime / dt;
         V dm;
         for (int c = 0; c < 3; ++c) {
             const V disp = b.nodes[idx[c]].position - b.nodes[idx[c]].previous;
@@ -731,7 +853,7 @@ void substep(oracle_body& b, double dt, const sw_step_settings& st) {
         m.residual = length(residual) / s;
         m.resultant = -m.lambda * (invDt2 / s);
     }
-    const double invDt = 1.0 / dt;
+    const Real invDt = 1.0 / dt;
     for (Node& n : b.nodes) n.velocity = n.w == 0.0 ? V{} : (n.position - n.previous) * invDt;
     if (b.hasSuspension) {
         Payload& p = b.payload;
@@ -739,23 +861,23 @@ void substep(oracle_body& b, double dt, const sw_step_settings& st) {
         p.v = (p.com - p.prevCom) / dt;
         Q delta = p.q * Q{p.prevQ.w, -p.prevQ.x, -p.prevQ.y, -p.prevQ.z};
         delta = canonical(delta);
-        const double half = mAcos(std::clamp(delta.w, -1.0, 1.0));
-        const double sine = mSin(half);
-        if (std::abs(sine) < 1.0e-12) p.w = V{delta.x, delta.y, delta.z} * (2.0 / dt);
+        const Real half = mAcos(std::clamp<Real>(delta.w, -1.0, 1.0));
+        const Real sine = mSin(half);
+        if (abs(sine) < 1.0e-12) p.w = V{delta.x, delta.y, delta.z} * (2.0 / dt);
         else p.w = V{delta.x / sine, delta.y / sine, delta.z / sine} * (2.0 * half / dt);
         // suspension.cpp:2324-2384
-        const double n2 = p.q.w * p.q.w + p.q.x * p.q.x + p.q.y * p.q.y + p.q.z * p.q.z;
-        if (!finite(p.com) || !finite(p.q) || !finite(p.v) || !finite(p.w) || std::abs(n2 - 1.0) > 1.0e-12)
+        const Real n2 = p.q.w * p.q.w + p.q.x * p.q.x + p.q.y * p.q.y + p.q.z * p.q.z;
+        if (!finite(p.com) || !finite(p.q) || !finite(p.v) || !finite(p.w) || abs(n2 - 1.0) > 1.0e-12)
             fail(8, "payload", "payload trial state is non-finite or unnormalized");
-        double worst = 0.0;
+        Real worst = 0.0;
         for (const Segment& s : b.segments) {
             const V from = s.from >= 0 ? b.nodes[s.from].position : hangPosition(p, -s.from - 1);
             const V to = s.to >= 0 ? b.nodes[s.to].position : hangPosition(p, -s.to - 1);
-            const double L = length(to - from);
-            if (!std::isfinite(L)) fail(8, s.id, "line trial length is non-finite");
-            const double stretch = std::max(0.0, L - s.rest);
-            const double compliant = s.lambda / (s.stiffness * dt * dt);
-            worst = std::max(worst, std::abs(stretch - compliant));
+            const Real L = length(to - from);
+            if (!isfinite(L)) fail(8, s.id, "line trial length is non-finite");
+            const Real stretch = std::max<Real>(0.0, L - s.rest);
+            const Real compliant = s.lambda / (s.stiffness * dt * dt);
+            worst = std::max<Real>(worst, abs(stretch - compliant));
         }
         p.certified = worst;
         if (worst > p.maxResidual) fail(8, "line-residual", "maximum line residual exceeds configured bound");
@@ -818,14 +940,14 @@ int oracle_create(const sw_definition* d, oracle_body** out) {
             const std::size_t t = d->membrane_triangle[m];
             const double* chart = &d->membrane_chart[6 * m];
             // soft_body.cpp:663-690: reference matrix, area, inverse, compliance
-            const double r00 = chart[2] - chart[0], r01 = chart[4] - chart[0];
-            const double r10 = chart[3] - chart[1], r11 = chart[5] - chart[1];
-            const double det = r00 * r11 - r01 * r10;
-            const double scale = std::max({std::abs(r00), std::abs(r01), std::abs(r10), std::abs(r11)});
-            if (!(scale > 0.0) || !std::isfinite(det) ||
-                std::abs(det) <= 64.0 * std::numeric_limits<double>::epsilon() * scale * scale)
+            const Real r00 = chart[2] - chart[0], r01 = chart[4] - chart[0];
+            const Real r10 = chart[3] - chart[1], r11 = chart[5] - chart[1];
+            const Real det = r00 * r11 - r01 * r10;
+            const Real scale = std::max({abs(r00), abs(r01), abs(r10), abs(r11)});
+            if (!(scale > 0.0) || !isfinite(det) ||
+                abs(det) <= 64.0 * std::numeric_limits<double>::epsilon() * scale * scale)
                 throw std::invalid_argument("Matrix2 is singular or ill-conditioned");
-            const double inv = 1.0 / det;
+            const Real inv = 1.0 / det;
             const double* mat = &d->membrane_material[8 * m];
             if (mat[7] != 1.0) throw std::invalid_argument("oracle restates compressionStiffnessRatio == 1 only");
             Sym C;
@@ -872,7 +994,7 @@ int oracle_create(const sw_definition* d, oracle_body** out) {
             p.q = canonical({s.pilot_orientation[0], s.pilot_orientation[1], s.pilot_orientation[2], s.pilot_orientation[3]});
             p.prevCom = p.com;
             p.prevQ = p.q;
-            double in[9] = {s.pilot_inertia[0], 0, 0, 0, s.pilot_inertia[1], 0, 0, 0, s.pilot_inertia[2]};
+            Real in[9] = {s.pilot_inertia[0], 0, 0, 0, s.pilot_inertia[1], 0, 0, 0, s.pilot_inertia[2]};
             std::memcpy(p.inertia.a, in, sizeof(in));
             p.minLength = s.minimum_line_length;
             p.maxResidual = s.maximum_line_residual;
@@ -903,7 +1025,7 @@ int oracle_create(const sw_definition* d, oracle_body** out) {
                 b->segments.push_back({"segment:" + std::to_string(s.segment_ids[k]),
                                        endpoint.at(name(s.segment_from_kind[k], s.segment_from_id[k])),
                                        endpoint.at(name(s.segment_to_kind[k], s.segment_to_id[k])), s.segment_rest[k],
-                                       s.segment_stiffness[k], 0.0});
+                                       s.segment_stiffness[k], 0.0, k});
             std::sort(b->segments.begin(), b->segments.end(),
                       [](const Segment& l, const Segment& r) { return l.id < r.id; });
         }
@@ -931,16 +1053,47 @@ int oracle_set_external_forces(oracle_body* b, const double* forces, size_t coun
 
 void oracle_set_angle_perturbation(oracle_body* b, int ulps) { b->perturbUlps = ulps; }
 
+int oracle_set_tangent(oracle_body* b, const double* dLogCompliance, const double* dRest, const double* dPressure) {
+    return guard([&] {
+        if constexpr (!kTangentBuild) {
+            throw std::logic_error("tangent seeds need the tangent build of the oracle");
+        } else {
+            for (std::size_t m = 0; m < b->membranes.size(); ++m) {
+                const double t = dLogCompliance ? dLogCompliance[m] : 0.0;
+                Sym& c = b->membranes[m].compliance;
+                for (Real* e : {&c.xx, &c.yy, &c.zz, &c.xy, &c.xz, &c.yz}) seedTangent(*e, value(*e) * t);  // d(C e^theta)
+            }
+            for (Segment& seg : b->segments) seedTangent(seg.rest, dRest ? dRest[seg.definition] : 0.0);
+            for (std::size_t t = 0; t < b->pressure.size(); ++t)
+                seedTangent(b->pressure[t], dPressure ? dPressure[t] : 0.0);
+        }
+    });
+}
+
+int oracle_tangent_state(const oracle_body* b, double* dPosition) {
+    return guard([&] {
+        if constexpr (!kTangentBuild) {
+            throw std::logic_error("tangents need the tangent build of the oracle");
+        } else {
+            for (std::size_t i = 0; i < b->nodes.size(); ++i) {
+                dPosition[3 * i] = tangentOf(b->nodes[i].position.x);
+                dPosition[3 * i + 1] = tangentOf(b->nodes[i].position.y);
+                dPosition[3 * i + 2] = tangentOf(b->nodes[i].position.z);
+            }
+        }
+    });
+}
+
 void oracle_set_portable_math(oracle_body* b, int portable) { b->portableMath = portable != 0; }
 
 void oracle_math(int portable, int function, const double* a, const double* b, double* out, size_t count) {
     portableMath = portable != 0;
     for (size_t i = 0; i < count; ++i) {
         switch (function) {
-        case 2: out[i] = mAtan2(a[i], b[i]); break;
-        case 4: out[i] = mSin(a[i]); break;
-        case 5: out[i] = mCos(a[i]); break;
-        case 6: out[i] = mAcos(a[i]); break;
+        case 2: out[i] = value(mAtan2(a[i], b[i])); break;
+        case 4: out[i] = value(mSin(a[i])); break;
+        case 5: out[i] = value(mCos(a[i])); break;
+        case 6: out[i] = value(mAcos(a[i])); break;
         default: out[i] = std::nan(""); break;
         }
     }
@@ -974,7 +1127,7 @@ int oracle_step(oracle_body* b, const sw_step_settings* st) {
         // failing substep restores the pre-step state.
         const oracle_body saved = *b;
         try {
-            const double dt = st->time_step / st->substeps;
+            const Real dt = static_cast<Real>(st->time_step) / st->substeps;
             for (int k = 0; k < st->substeps; ++k) {
                 for (std::size_t i = 0; i < b->nodes.size(); ++i) b->nodes[i].force = b->external[i];
                 substep(*b, dt, *st);
@@ -994,48 +1147,48 @@ int oracle_state(const oracle_body* b, double* position, double* previous, doubl
         for (std::size_t i = 0; i < b->nodes.size(); ++i) {
             const Node& n = b->nodes[i];
             if (position) {
-                position[3 * i] = n.position.x;
-                position[3 * i + 1] = n.position.y;
-                position[3 * i + 2] = n.position.z;
+                position[3 * i] = value(n.position.x);
+                position[3 * i + 1] = value(n.position.y);
+                position[3 * i + 2] = value(n.position.z);
             }
             if (previous) {
-                previous[3 * i] = n.previous.x;
-                previous[3 * i + 1] = n.previous.y;
-                previous[3 * i + 2] = n.previous.z;
+                previous[3 * i] = value(n.previous.x);
+                previous[3 * i + 1] = value(n.previous.y);
+                previous[3 * i + 2] = value(n.previous.z);
             }
             if (velocity) {
-                velocity[3 * i] = n.velocity.x;
-                velocity[3 * i + 1] = n.velocity.y;
-                velocity[3 * i + 2] = n.velocity.z;
+                velocity[3 * i] = value(n.velocity.x);
+                velocity[3 * i + 1] = value(n.velocity.y);
+                velocity[3 * i + 2] = value(n.velocity.z);
             }
         }
         if (constraintLambda)
-            for (std::size_t k = 0; k < b->constraints.size(); ++k) constraintLambda[k] = b->constraints[k].lambda;
+            for (std::size_t k = 0; k < b->constraints.size(); ++k) constraintLambda[k] = value(b->constraints[k].lambda);
         for (std::size_t k = 0; k < b->membranes.size(); ++k) {
             const Membrane& m = b->membranes[k];
             if (membraneLambda) {
-                membraneLambda[3 * k] = m.lambda.x;
-                membraneLambda[3 * k + 1] = m.lambda.y;
-                membraneLambda[3 * k + 2] = m.lambda.z;
+                membraneLambda[3 * k] = value(m.lambda.x);
+                membraneLambda[3 * k + 1] = value(m.lambda.y);
+                membraneLambda[3 * k + 2] = value(m.lambda.z);
             }
             if (membraneResultant) {
-                membraneResultant[3 * k] = m.resultant.x;
-                membraneResultant[3 * k + 1] = m.resultant.y;
-                membraneResultant[3 * k + 2] = m.resultant.z;
+                membraneResultant[3 * k] = value(m.resultant.x);
+                membraneResultant[3 * k + 1] = value(m.resultant.y);
+                membraneResultant[3 * k + 2] = value(m.resultant.z);
             }
-            if (membraneResidual) membraneResidual[k] = m.residual;
+            if (membraneResidual) membraneResidual[k] = value(m.residual);
         }
         if (dihedralLambda)
-            for (std::size_t s = 0; s < b->hinges.size(); ++s) dihedralLambda[b->hingeOrder[s]] = b->hinges[s].lambda;
+            for (std::size_t s = 0; s < b->hinges.size(); ++s) dihedralLambda[b->hingeOrder[s]] = value(b->hinges[s].lambda);
         if (segmentLambda)
-            for (std::size_t k = 0; k < b->segments.size(); ++k) segmentLambda[k] = b->segments[k].lambda;
+            for (std::size_t k = 0; k < b->segments.size(); ++k) segmentLambda[k] = value(b->segments[k].lambda);
         if (payload && b->hasSuspension) {
             const Payload& p = b->payload;
-            const double v[13] = {p.com.x, p.com.y, p.com.z, p.q.w, p.q.x, p.q.y, p.q.z,
+            const Real v[13] = {p.com.x, p.com.y, p.com.z, p.q.w, p.q.x, p.q.y, p.q.z,
                                   p.v.x,   p.v.y,   p.v.z,   p.w.x, p.w.y, p.w.z};
-            std::copy(v, v + 13, payload);
+            for (int k = 0; k < 13; ++k) payload[k] = value(v[k]);
         }
-        if (suspensionResidual) *suspensionResidual = b->payload.certified;
+        if (suspensionResidual) *suspensionResidual = value(b->payload.certified);
     });
 }
 

@@ -47,6 +47,15 @@ void oracle_set_portable_math(oracle_body* body, int portable);
 void oracle_math(int portable, int function, const double* a, const double* b, double* out, size_t count);
 void oracle_quad_math(int function, const double* a, const double* b, double* out, size_t count);
 
+/* Tangent build only (liboracle_xpbd_tan.so; other builds report
+ * SW_LOGIC_ERROR): seeds one direction of the parameters (per membrane
+ * log-compliance, compliance C e^theta; per segment rest length, definition
+ * order; per triangle pressure; NULL = zero) after the face pressure is set,
+ * and reads the tangent of every node position (3 per node) after steps. */
+int oracle_set_tangent(oracle_body* body, const double* d_log_compliance, const double* d_rest,
+                       const double* d_pressure);
+int oracle_tangent_state(const oracle_body* body, double* d_position);
+
 #ifdef __cplusplus
 }
 #endif

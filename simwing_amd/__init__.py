@@ -26,6 +26,7 @@ __all__ = [
     "Assembly",
     "Structure",
     "Batch",
+    "Adjoint",
     "generate_scene",
     "flight_step_settings",
     "flight_assembly_settings",
@@ -326,6 +327,7 @@ class Structure:
         check(lib.sw_structure_create(C.byref(view), device, C.byref(handle)))
         self._handle = handle
         self._owned = True
+        self._dependents = []  # adjoints borrowing this structure: closed first
         self.node_count = definition.node_count
 
     @classmethod
@@ -336,10 +338,13 @@ class Structure:
         s._handle = handle
         s._owned = False
         s._owner = owner
+        s._dependents = []
         s.node_count = definition.node_count
         return s
 
     def close(self) -> None:
+        for dependent in getattr(self, "_dependents", []):
+            dependent.close()
         if getattr(self, "_handle", None):
             if getattr(self, "_owned", True):
                 native.library().sw_structure_destroy(self._handle)
@@ -531,3 +536,76 @@ class Batch:
         handle = C.c_void_p()
         check(native.library().sw_batch_structure(self._handle, wing, C.byref(handle)))
         return Structure._borrow(handle, self._definitions[wing], self)
+
+
+class Adjoint:
+    """Reverse-mode gradients of the shape loss sum |x(end) - target|^2 over
+    `steps` steps from the structure's current state (loads pending now are
+    applied at every step), with respect to per-membrane log-compliance,
+    per-segment rest length and per-triangle pressure (sw_adjoint_*)."""
+
+    def __init__(self, structure: Structure, steps: int, checkpoint_interval: int = 8,
+                 settings: native.StepSettings | None = None):
+        self._structure = structure  # kept alive: the adjoint borrows it
+        d = structure.definition
+        self.membrane_count = int(d.arrays["membrane_triangle"].size)
+        self.segment_count = 0 if d.suspension is None else int(np.asarray(d.suspension["segment_rest"]).size)
+        self.triangle_count = d.triangle_count
+        self.node_count = d.node_count
+        s = settings or native.flight_step_settings()
+        handle = C.c_void_p()
+        check(native.library().sw_adjoint_configure(structure._handle, C.byref(s), int(steps),
+                                                    int(checkpoint_interval), C.byref(handle)))
+        self._handle = handle
+        structure._dependents.append(self)
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            native.library().sw_adjoint_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def _param(values, count: int):
+        if values is None:
+            return None, None
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if a.size != count:
+            raise ValueError(f"expected {count} values, got {a.size}")
+        return a, f64p(a)
+
+    def run(self, target, log_compliance=None, rest_length=None, pressure=None, gradients: bool = True) -> dict:
+        """One forward + reverse run. Returns loss, final positions (N x 3)
+        and, when `gradients`, the three gradients (definition order)."""
+        t = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+        if t.size != 3 * self.node_count:
+            raise ValueError("target needs 3 values per node")
+        lc, lcp = self._param(log_compliance, self.membrane_count)
+        rl, rlp = self._param(rest_length, self.segment_count)
+        pr, prp = self._param(pressure, self.triangle_count)
+        loss = C.c_double()
+        final = np.zeros(3 * self.node_count)
+        out = {}
+        if gradients:
+            out["grad_log_compliance"] = np.zeros(self.membrane_count)
+            out["grad_rest_length"] = np.zeros(self.segment_count)
+            out["grad_pressure"] = np.zeros(self.triangle_count)
+        g = [f64p(out[k]) if gradients else None
+             for k in ("grad_log_compliance", "grad_rest_length", "grad_pressure")]
+        check(native.library().sw_adjoint_run(self._handle, lcp, rlp, prp, f64p(t), C.byref(loss), f64p(final),
+                                              *g))
+        out["loss"] = loss.value
+        out["final_position"] = final.reshape(-1, 3)
+        return out
+
+    def set_workgroups(self, workgroups: int) -> None:
+        check(native.library().sw_adjoint_set_workgroups(self._handle, int(workgroups)))
+
+    def timing(self) -> tuple[float, float, int]:
+        """Device ms of the last run's forward pass and of its replay +
+        reverse kernels, and the device bytes the adjoint holds."""
+        f, r, b = C.c_double(), C.c_double(), C.c_size_t()
+        check(native.library().sw_adjoint_timing(self._handle, C.byref(f), C.byref(r), C.byref(b)))
+        return f.value, r.value, int(b.value)

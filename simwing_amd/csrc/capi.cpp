@@ -98,6 +98,11 @@ struct sw_structure {
     double maxResidual = 0.0;
     std::vector<std::string> segmentNames;  // runtime order
     bool hasLoadPath = false;
+    // An isolated free membrane (no suspension, every node dynamic, membranes
+    // or hinges present): the reference restores its predicted momentum
+    // after each substep (soft_body.cpp:1196-1205, 1446-1448), which the
+    // device path does not compute.
+    bool freeMembrane = false;
 };
 
 namespace {
@@ -120,6 +125,10 @@ swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st) {
             "use workerThreads >= 1 (the coloured sweep)");
     if (s.hasLoadPath && st.cable_sweep_pairs > 0)
         throw std::invalid_argument("load-path cable sweep pairs are not supported by the device path");
+    if (s.freeMembrane)
+        throw std::invalid_argument(
+            "free-membrane momentum restoration is not supported by the device path "
+            "(fix a node, or step the membrane with a suspension)");
     swx::StepArgs a{};
     a.dt = st.time_step / st.substeps;
     a.invDt = 1.0 / a.dt;
@@ -267,6 +276,12 @@ sw_status sw_structure_create(const sw_definition* definition, int device, sw_st
         for (std::size_t c = 0; c < s->definition.constraintCount(); ++c) {
             const auto kind = s->definition.constraintKind[c];
             if (kind == SW_CONSTRAINT_CABLE || kind == SW_CONSTRAINT_SUSPENSION_TIE) s->hasLoadPath = true;
+        }
+        {
+            const swx::Definition& d = s->definition;
+            bool allDynamic = true;
+            for (std::size_t i = 0; i < d.nodeCount(); ++i) allDynamic = allDynamic && d.nodeFixed[i] == 0;
+            s->freeMembrane = !d.suspension && (d.membraneCount() > 0 || d.dihedralCount() > 0) && allDynamic;
         }
         s->wing = std::make_unique<swx::DeviceWing>(s->definition, device);
         s->pending.assign(3 * s->definition.nodeCount(), 0.0);
@@ -803,6 +818,69 @@ sw_status sw_batch_workgroups(const sw_batch* b, int* per_wing, size_t count) {
         const auto& blocks = b->device->blocks();
         if (count != blocks.size()) throw std::invalid_argument("per_wing must hold one entry per wing");
         for (size_t w = 0; w < count; ++w) per_wing[w] = blocks[w];
+    });
+}
+
+// ---- Adjoint -----------------------------------------------------------
+
+struct sw_adjoint {
+    sw_structure* structure = nullptr;
+    std::vector<double> loads;  // pending at configuration, applied at every step
+    std::unique_ptr<swx::DeviceAdjoint> device;
+};
+
+sw_status sw_adjoint_configure(sw_structure* s, const sw_step_settings* settings, uint64_t steps,
+                               uint32_t checkpoint_interval, sw_adjoint** out) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(settings, "settings");
+        requirePointer(out, "out");
+        const swx::StepArgs args = prepare(*s, *settings);
+        if (steps < 1 || steps > (1ull << 40)) throw std::invalid_argument("adjoint step count must be positive");
+        if (checkpoint_interval < 1) throw std::invalid_argument("checkpoint interval must be positive");
+        auto a = std::make_unique<sw_adjoint>();
+        a->structure = s;
+        a->loads = s->pending;
+        a->device = std::make_unique<swx::DeviceAdjoint>(*s->wing, s->definition, args, settings->substeps,
+                                                         static_cast<long>(steps), static_cast<int>(checkpoint_interval));
+        *out = a.release();
+    });
+}
+
+void sw_adjoint_destroy(sw_adjoint* a) { delete a; }
+
+sw_status sw_adjoint_run(sw_adjoint* a, const double* log_compliance, const double* rest_length,
+                         const double* pressure, const double* target, double* loss, double* final_position,
+                         double* grad_log_compliance, double* grad_rest_length, double* grad_pressure) {
+    return guarded([&] {
+        requirePointer(a, "adjoint");
+        requirePointer(target, "target");
+        sw_structure& s = *a->structure;
+        s.wing->setExternalForces(a->loads);
+        s.pendingDirty = true;  // the next regular step uploads its own pending loads
+        try {
+            const double value = a->device->run(log_compliance, rest_length, pressure, target, final_position,
+                                                grad_log_compliance, grad_rest_length, grad_pressure);
+            if (loss) *loss = value;
+        } catch (const swx::DeviceStepFailure& e) {
+            throwDeviceError(s, e.error);
+        }
+    });
+}
+
+sw_status sw_adjoint_set_workgroups(sw_adjoint* a, int workgroups) {
+    return guarded([&] {
+        requirePointer(a, "adjoint");
+        a->device->setWorkgroups(workgroups);
+    });
+}
+
+sw_status sw_adjoint_timing(const sw_adjoint* a, double* forward_ms, double* reverse_ms, size_t* device_bytes) {
+    return guarded([&] {
+        requirePointer(a, "adjoint");
+        if (forward_ms) *forward_ms = a->device->forwardMs();
+        if (reverse_ms) *reverse_ms = a->device->reverseMs();
+        if (device_bytes) *device_bytes = a->device->deviceBytes();
     });
 }
 

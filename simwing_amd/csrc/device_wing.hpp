@@ -9,6 +9,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,7 @@ struct StepArgs {
 };
 
 class DeviceBatch;
+class DeviceAdjoint;
 
 class DeviceWing {
 public:
@@ -158,6 +160,7 @@ public:
 
 private:
     friend class DeviceBatch;
+    friend class DeviceAdjoint;
     void build(const Definition& d);
     int device_;
     hipStream_t stream_ = nullptr;
@@ -196,6 +199,74 @@ private:
     int threads_ = 512;
     bool sharedNodes_ = false;
     std::size_t snapshotBytes_ = 0;
+};
+
+// A failed step inside an adjoint run (the device error word), thrown after
+// the wing's state has been restored.
+struct DeviceStepFailure : std::runtime_error {
+    DeviceError error;
+    explicit DeviceStepFailure(const DeviceError& e) : std::runtime_error("device step failed"), error(e) {}
+};
+
+// Reverse-mode gradient of L = sum_i |x_i(end) - target_i|^2 over `steps`
+// steps (of `substeps` substeps each, the pending loads applied at every
+// step) from the wing's state at construction, with respect to
+//   * the log-compliance theta_m of every membrane (compliance C e^theta;
+//     theta is applied by scaling the element's stiffness by e^-theta
+//     before the cofactor inverse),
+//   * the rest length of every suspension segment,
+//   * the pressure of every triangle.
+// The forward pass keeps a state checkpoint every `interval` steps; the
+// reverse pass replays each interval from its checkpoint with the substep
+// trajectories recorded, then runs the reverse kernels substep by substep,
+// last first. Every run starts from the construction state and leaves the
+// wing's state and parameters as they were.
+class DeviceAdjoint {
+public:
+    DeviceAdjoint(DeviceWing& wing, const Definition& definition, const StepArgs& args, int substeps, long steps,
+                  int interval);
+    ~DeviceAdjoint();
+    DeviceAdjoint(const DeviceAdjoint&) = delete;
+    DeviceAdjoint& operator=(const DeviceAdjoint&) = delete;
+
+    // Parameters in definition order (membranes, segments, triangles); a
+    // null pointer keeps the definition's values. Gradient outputs may be
+    // null (all null: forward and loss only). Returns the loss.
+    double run(const double* logCompliance, const double* restLength, const double* pressure, const double* target,
+               double* finalPosition, double* gradLogCompliance, double* gradRestLength, double* gradPressure);
+    // Workgroups of the reverse kernel (cooperative launch).
+    void setWorkgroups(int workgroups);
+    int workgroups() const { return workgroups_; }
+    // Device time of the last run: the checkpointed forward pass, and the
+    // recorded replay plus the reverse kernels.
+    double forwardMs() const { return forwardMs_; }
+    double reverseMs() const { return reverseMs_; }
+    std::size_t deviceBytes() const { return bytes_; }
+
+private:
+    void applyParameters(const double* logCompliance, const double* restLength, const double* pressure);
+    void restoreBase();
+    void launchRecording(double* trace);
+    void launchReverse(const double* trace);
+    void fail(const DeviceError& e);
+    DeviceWing& wing_;
+    Definition definition_;
+    StepArgs args_;
+    int substeps_;
+    long steps_;
+    int interval_;
+    int workgroups_ = 1;
+    long traceDoubles_ = 0;
+    std::size_t bytes_ = 0;
+    std::vector<void*> owned_;
+    double *initial_ = nullptr, *checkpoints_ = nullptr, *trace_ = nullptr;
+    double *xbar_ = nullptr, *vbar_ = nullptr, *qbar_ = nullptr, *fbar_ = nullptr, *lbar_ = nullptr, *pbar_ = nullptr;
+    double *gTheta_ = nullptr, *gRest_ = nullptr, *gPressure_ = nullptr, *jac_ = nullptr;
+    int lastPayloadPhase_ = -1;
+    std::vector<double> baseCompliance_;  // 6 per membrane slot: xx yy zz xy xz yz
+    std::vector<double> baseRest_;        // per segment slot
+    double forwardMs_ = 0.0, reverseMs_ = 0.0;
+    hipEvent_t events_[3] = {nullptr, nullptr, nullptr};
 };
 
 // Device math conformance probe (function codes as sw_device_math).

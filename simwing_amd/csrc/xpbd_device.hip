@@ -83,10 +83,40 @@ struct WingView {
     // point-to-point synchronisation, dedicated chain workgroup, nodes in
     // LDS, and the wing's workgroup count in a batched launch.
     int neighbourSync, dedicatedChain, sharedNodes, launchBlocks;
+    // Trajectory recording for the adjoint (null otherwise; only with grid
+    // barriers between phases): the state before every phase of one
+    // substep, laid out as traceLayout() describes.
+    double* trace;
     double* state;      // contiguous mutable state block
     double* snapshot;   // copy of `state`
     long stateDoubles;
 };
+
+// Layout of one substep's trace, in doubles:
+//   prologue  x, v (3N each, node-major xyz), payload com, q, v, w (13);
+//   per iteration: every multiplier at its start (lambdaCount: distance,
+//     membrane x, y, z, hinge, segment, sphere), then per phase the node
+//     positions (3N) and payload com, q, inverse inertia (16) before it;
+//   per iteration: payload com, q and the two endpoint positions (13)
+//     before each segment of a serial run, at the segment's slot;
+//   closing: every multiplier, positions (3N) before each closing phase,
+//     then payload com, q (7) after the iterations.
+struct TraceLayout {
+    long nodes3, lambdaCount, phaseSize, iterStride, iterBase, chainBase, chainStride, closeBase, total;
+};
+__host__ __device__ inline TraceLayout traceLayout(const WingView& v, int iterations) {
+    TraceLayout t;
+    t.nodes3 = 3L * v.nodeCount;
+    t.lambdaCount = static_cast<long>(v.distCount) + 3L * v.membCount + v.hingeCount + v.segCount + v.pairCount;
+    t.phaseSize = t.nodes3 + 16;
+    t.iterStride = t.lambdaCount + static_cast<long>(v.iterPhaseCount) * t.phaseSize;
+    t.iterBase = 2 * t.nodes3 + 13;
+    t.chainBase = t.iterBase + static_cast<long>(iterations) * t.iterStride;
+    t.chainStride = 13L * v.segCount;
+    t.closeBase = t.chainBase + static_cast<long>(iterations) * t.chainStride;
+    t.total = t.closeBase + t.lambdaCount + static_cast<long>(v.closePhaseCount) * t.nodes3 + 7;
+    return t;
+}
 
 namespace {
 
@@ -544,7 +574,7 @@ struct SlotRefs {
 // memory; the workgroup finally writes nodes, multipliers and the payload
 // back. Identical arithmetic to projecting the segments in place.
 __device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, PayloadDev& payload,
-                                      const StepArgs& a) {
+                                      const StepArgs& a, double* chainTrace) {
     const int slots = P.slotEnd - P.slotBegin;
     const int items = P.end - P.begin;
     for (int s = threadIdx.x; s < slots; s += blockDim.x) {
@@ -577,6 +607,24 @@ __device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, cons
         for (int j = 0; j < items; ++j) {
             const int fe = chainItem[4 * j], te = chainItem[4 * j + 1];
             double lambda = chainLambda[j];
+            if (chainTrace) {
+                double* t = chainTrace + 13L * (P.begin + j);
+                t[0] = state.com.x;
+                t[1] = state.com.y;
+                t[2] = state.com.z;
+                t[3] = state.q.w;
+                t[4] = state.q.x;
+                t[5] = state.q.y;
+                t[6] = state.q.z;
+                const d3 pa = fe >= 0 ? S.get(fe) : d3{0.0, 0.0, 0.0};
+                const d3 pb = te >= 0 ? S.get(te) : d3{0.0, 0.0, 0.0};
+                t[7] = pa.x;
+                t[8] = pa.y;
+                t[9] = pa.z;
+                t[10] = pb.x;
+                t[11] = pb.y;
+                t[12] = pb.z;
+            }
             if (!projectSegment(v, S, fe, te, chainRest[j], chainStiffness[j], lambda, chainItem[4 * j + 2], state,
                                 def, iinv, iinvFinite, args))
                 break;
@@ -733,11 +781,37 @@ __device__ __forceinline__ void atomicMaxNonNegative(double* address, double val
     atomicMax(reinterpret_cast<unsigned long long*>(address), static_cast<unsigned long long>(__double_as_longlong(value)));
 }
 
+// Trajectory recording (adjoint): every multiplier, and node positions in
+// node-major xyz order, written with the given start and stride.
+__device__ void recordLambdas(const WingView& v, double* out, int first, int stride) {
+    long o = 0;
+    for (int k = first; k < v.distCount; k += stride) out[k] = v.dlam[k];
+    o += v.distCount;
+    for (int k = first; k < v.membCount; k += stride) {
+        out[o + k] = v.mlx[k];
+        out[o + v.membCount + k] = v.mly[k];
+        out[o + 2L * v.membCount + k] = v.mlz[k];
+    }
+    o += 3L * v.membCount;
+    for (int k = first; k < v.hingeCount; k += stride) out[o + k] = v.hlam[k];
+    o += v.hingeCount;
+    for (int k = first; k < v.segCount; k += stride) out[o + k] = v.slam[k];
+    o += v.segCount;
+    for (int k = first; k < v.pairCount; k += stride) out[o + k] = v.splam[k];
+}
+__device__ void recordPositions(const NodeRefs& N, int n, double* out, int first, int stride) {
+    for (int i = first; i < n; i += stride) {
+        out[3L * i] = N.x[i];
+        out[3L * i + 1] = N.y[i];
+        out[3L * i + 2] = N.z[i];
+    }
+}
+
 // Items [first, end) with the given stride of one phase of the iteration
 // program; a serial run (kPhaseSegmentRun) is staged and projected by the
 // calling workgroup as a whole.
 __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, int first, int end, int stride,
-                         PayloadDev& payload, const StepArgs& a) {
+                         PayloadDev& payload, const StepArgs& a, double* chainTrace = nullptr) {
     switch (P.type) {
     case kPhaseDistance:
         for (int k = first; k < end; k += stride) projectDistance(v, N, k, a.invDt2);
@@ -766,7 +840,7 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
         }
         break;
     case kPhaseSegmentRun:
-        runChain(v, N, P, payload, a);
+        runChain(v, N, P, payload, a, chainTrace);
         break;
     case kPhaseSpheres:
         for (int k = first; k < end; k += stride) projectSphere(v, N, k, a.invDt2);
@@ -810,6 +884,26 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         N = NodeRefs{v.px, v.py, v.pz, v.w, v.hangLocal};
     }
     PayloadDev& payload = *v.payload;
+    const TraceLayout layout = traceLayout(v, a.iterations);
+    if (v.trace) {
+        // Step-start state. Phase A writes no position and no velocity, and
+        // only workgroup 0's thread 0 touches the payload.
+        for (int i = gtid; i < n; i += gstride) {
+            v.trace[3L * i] = v.px[i];
+            v.trace[3L * i + 1] = v.py[i];
+            v.trace[3L * i + 2] = v.pz[i];
+            v.trace[layout.nodes3 + 3L * i] = v.vx[i];
+            v.trace[layout.nodes3 + 3L * i + 1] = v.vy[i];
+            v.trace[layout.nodes3 + 3L * i + 2] = v.vz[i];
+        }
+        if (gtid == 0) {
+            double* t = v.trace + 2 * layout.nodes3;
+            for (int k = 0; k < 3; ++k) t[k] = payload.com[k];
+            for (int k = 0; k < 4; ++k) t[3 + k] = payload.q[k];
+            for (int k = 0; k < 3; ++k) t[7 + k] = payload.v[k];
+            for (int k = 0; k < 3; ++k) t[10 + k] = payload.w[k];
+        }
+    }
 
     // A: prev = x, multiplier resets, payload prediction.
     for (int i = gtid; i < n; i += gstride) {
@@ -907,17 +1001,44 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         for (int it = 0; it < a.iterations; ++it)
             for (int ph = 0; ph < v.iterPhaseCount; ++ph) {
                 const Phase P = v.iterPhases[ph];
+                double* chainTrace = nullptr;
+                if (v.trace) {
+                    double* base = v.trace + layout.iterBase + it * layout.iterStride;
+                    if (ph == 0) recordLambdas(v, base, gtid, gstride);
+                    double* slot = base + layout.lambdaCount + ph * layout.phaseSize;
+                    recordPositions(N, n, slot, gtid, gstride);
+                    if (gtid == 0) {
+                        double* t = slot + layout.nodes3;
+                        for (int k = 0; k < 3; ++k) t[k] = payload.com[k];
+                        for (int k = 0; k < 4; ++k) t[3 + k] = payload.q[k];
+                        for (int k = 0; k < 9; ++k) t[7 + k] = payload.iinv[k];
+                    }
+                    chainTrace = v.trace + layout.chainBase + it * layout.chainStride;
+                    gridSync(v, workers);  // the phase starts after its state is recorded
+                }
                 if (P.type == kPhaseSegmentRun) {
-                    if (block == 0) runPhase(v, N, P, P.begin, P.end, 1, payload, a);
+                    if (block == 0) runPhase(v, N, P, P.begin, P.end, 1, payload, a, chainTrace);
                 } else {
-                    runPhase(v, N, P, P.begin + gtid, P.end, gstride, payload, a);
+                    runPhase(v, N, P, P.begin + gtid, P.end, gstride, payload, a, chainTrace);
                 }
                 gridSync(v, workers);
             }
     }
     // Closing seam-stitch pass (soft_body.cpp:1389-1405).
+    if (v.trace) {
+        recordLambdas(v, v.trace + layout.closeBase, gtid, gstride);
+        if (gtid == 0) {
+            double* t = v.trace + layout.total - 7;
+            for (int k = 0; k < 3; ++k) t[k] = payload.com[k];
+            for (int k = 0; k < 4; ++k) t[3 + k] = payload.q[k];
+        }
+    }
     for (int ph = 0; ph < v.closePhaseCount; ++ph) {
         const Phase P = v.closePhases[ph];
+        if (v.trace) {
+            recordPositions(N, n, v.trace + layout.closeBase + layout.lambdaCount + ph * layout.nodes3, gtid, gstride);
+            gridSync(v, workers);
+        }
         for (int k = P.begin + gtid; k < P.end; k += gstride) projectDistance(v, N, v.stitchItems[k], a.invDt2);
         gridSync(v, workers);
     }
@@ -1140,6 +1261,27 @@ T* deviceCopy(std::vector<void*>& owned, const std::vector<T>& host, std::size_t
     if (!host.empty()) check(hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
     else check(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
     return static_cast<T*>(p);
+}
+
+// Membrane compliance from the orthotropic material: the checkedInverse
+// cofactors of the stiffness {xx warp, yy weft, zz shear, xy coupling,
+// xz 0, yz 0} (membrane.cpp:268-271), with the four stiffness entries scaled
+// by `stiffnessScale` first (1 for the definition's own material).
+void membraneCompliance(const double* mat, double stiffnessScale, double out[6]) {
+    const double xx = mat[0] * stiffnessScale, yy = mat[1] * stiffnessScale, zz = mat[3] * stiffnessScale,
+                 xy = mat[2] * stiffnessScale, xz = 0.0, yz = 0.0;
+    const double sMax = std::max({std::abs(xx), std::abs(yy), std::abs(zz), std::abs(xy), std::abs(xz), std::abs(yz)});
+    const double sDet = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
+    const double sTol = 128.0 * 2.220446049250313080847263336181640625e-16 * sMax * sMax * sMax;
+    if (!(sMax > 0.0) || !std::isfinite(sDet) || std::abs(sDet) <= sTol)
+        throw std::invalid_argument("SymmetricMatrix3 is singular or ill-conditioned");
+    const double r = 1.0 / sDet;
+    out[0] = (yy * zz - yz * yz) * r;
+    out[1] = (xx * zz - xz * xz) * r;
+    out[2] = (xx * yy - xy * xy) * r;
+    out[3] = (xz * yz - xy * zz) * r;
+    out[4] = (xy * yz - xz * yy) * r;
+    out[5] = (xy * xz - xx * yz) * r;
 }
 
 template <typename T>
@@ -1393,20 +1535,9 @@ void DeviceWing::build(const Definition& d) {
             m11[k] = r00 * inv;
             sc[k] = std::sqrt(referenceArea);
             const double* mat = &d.membraneMaterial[8 * m];
-            // stiffness {xx warp, yy weft, zz shear, xy coupling, xz 0, yz 0}
-            const double xx = mat[0], yy = mat[1], zz = mat[3], xy = mat[2], xz = 0.0, yz = 0.0;
-            const double sMax = std::max({std::abs(xx), std::abs(yy), std::abs(zz), std::abs(xy), std::abs(xz), std::abs(yz)});
-            const double sDet = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
-            const double sTol = 128.0 * 2.220446049250313080847263336181640625e-16 * sMax * sMax * sMax;
-            if (!(sMax > 0.0) || !std::isfinite(sDet) || std::abs(sDet) <= sTol)
-                throw std::invalid_argument("SymmetricMatrix3 is singular or ill-conditioned");
-            const double r = 1.0 / sDet;
-            cxx[k] = (yy * zz - yz * yz) * r;
-            cyy[k] = (xx * zz - xz * xz) * r;
-            czz[k] = (xx * yy - xy * xy) * r;
-            cxy[k] = (xz * yz - xy * zz) * r;
-            cxz[k] = (xy * yz - xz * yy) * r;
-            cyz[k] = (xy * xz - xx * yz) * r;
+            double cm[6];
+            membraneCompliance(mat, 1.0, cm);
+            cxx[k] = cm[0], cyy[k] = cm[1], czz[k] = cm[2], cxy[k] = cm[3], cxz[k] = cm[4], cyz[k] = cm[5];
             pw[k] = mat[4];
             pt[k] = mat[5];
             damp[k] = mat[6];
@@ -2315,3 +2446,5 @@ void deviceMath(int device, int function, const double* a, const double* b, doub
 }
 
 }  // namespace swx
+
+#include "adjoint_device.inc"

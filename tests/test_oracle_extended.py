@@ -1,0 +1,48 @@
+"""The long-double build of the oracle (the reference values of the adjoint
+fixtures) computes the same function as the double build, and the
+finite-difference fixtures are self-consistent."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import simwing_amd as sw
+import swfx
+from helpers import flight_case
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def run(preset, steps, extended):
+    d, p, l = flight_case(preset)
+    b = orc.OracleBody(d, extended=extended)
+    b.set_face_pressure(p)
+    for _ in range(steps):
+        b.set_external_forces(l)
+        b.step(sw.flight_step_settings())
+    return b.state()
+
+
+def test_extended_build_is_the_same_function():
+    a = run("tiny", 6, False)
+    b = run("tiny", 6, True)
+    for key in ("position", "velocity", "segment_lambda", "payload"):
+        scale = max(1.0, np.abs(a[key]).max())
+        assert np.abs(a[key] - b[key]).max() <= 1e-11 * scale, key
+    # ... and is not the double build itself.
+    assert not np.array_equal(a["position"], b["position"])
+
+
+@pytest.mark.parametrize("wing", ["tiny", "small"])
+def test_difference_fixtures_are_consistent(wing):
+    f = swfx.read(str(GOLDEN / f"adjoint.{wing}.swfx"))
+    n = f["family"].size
+    assert n == f["index"].size == f["h"].size == f["fd_extended"].size == f["fd_double"].size
+    assert np.all(np.isfinite(f["fd_extended"])) and np.all(f["fd_extended"] != 0.0)
+    # The double differences scatter around the extended ones by their
+    # roundoff noise, which stays well below the values for the rest-length
+    # and pressure families.
+    rel = np.abs(f["fd_double"] - f["fd_extended"]) / np.abs(f["fd_extended"])
+    assert np.all(rel[f["family"] != 0] < 1e-2)
+    assert abs(f["loss_double"][0] - f["loss_extended"][0]) <= 1e-8 * f["loss_double"][0]
