@@ -52,6 +52,7 @@ def test_adjoint_matches_oracle_derivatives(wing):
     # of squares of differences of that size from the target.
     assert abs(r["loss"] - f["loss_double"][0]) <= 1e-7 * f["loss_double"][0]
     worst_tangent = worst_fd = 0.0
+    kinks = []
     for k in range(f["family"].size):
         family, index = int(f["family"][k]), int(f["index"][k])
         ad = r[FAMILIES[family]][index]
@@ -65,15 +66,27 @@ def test_adjoint_matches_oracle_derivatives(wing):
         print(f"  {FAMILIES[family]:20s} {index:6d} adjoint {ad: .12e} tangent rel {rel_t:.1e} "
               f"differences rel {rel_fd:.1e} (their noise {noise / abs(fd):.1e})", flush=True)
         assert rel_t <= 1e-8, (family, index, ad, exact)
-        # The long-double differences are no noisier than the double ones.
-        assert abs(ad - fd) <= 1e-6 * abs(fd) + abs(f["fd_double"][k] - fd), (family, index, ad, fd)
+        # A kink inside [p - h, p + h] (a line going slack or taut in some
+        # iteration: the max(0, .) of the unilateral projection): the two
+        # differences agree with each other, being no noise, but not with the
+        # one-sided derivative. Reported, not compared.
+        spread = abs(f["fd_double"][k] - fd)
+        if abs(fd - exact) > 1e-5 * abs(exact) and spread < 0.5 * abs(fd - exact):
+            kinks.append((FAMILIES[family], index))
+            print(f"    near a kink: differences {fd: .6e} / {f['fd_double'][k]: .6e}", flush=True)
+            continue
+        # Elsewhere: within 1e-6 plus the differences' own noise, of which
+        # the double build's spread is an upper bound.
+        assert abs(ad - fd) <= 1e-6 * abs(fd) + spread, (family, index, ad, fd)
     for family in range(3):
         direction = f[("direction_log_compliance", "direction_rest_length", "direction_pressure")[family]]
         ad = float(r[FAMILIES[family]] @ direction)
         exact = f["tangent_direction"][family]
         print(f"  direction {FAMILIES[family]:20s} adjoint {ad: .12e} tangent {exact: .12e}", flush=True)
         assert abs(ad - exact) <= 1e-8 * abs(exact)
-    print(f"  worst relative error: tangent {worst_tangent:.1e}, differences {worst_fd:.1e}", flush=True)
+    print(f"  worst relative error: tangent {worst_tangent:.1e}, differences {worst_fd:.1e}; "
+          f"{len(kinks)} of {f['family'].size} parameters near a kink: {kinks}", flush=True)
+    assert len(kinks) <= f["family"].size // 4
     fwd, rev, nbytes = adj.timing()
     print(f"  device ms: forward {fwd:.2f}, replay + reverse {rev:.2f}; {nbytes / 2**20:.1f} MiB", flush=True)
     adj.close()
