@@ -109,6 +109,27 @@ inline double value(double a) { return a; }
 inline double value(long double a) { return static_cast<double>(a); }
 inline double tangentOf(double) { return 0.0; }
 inline double tangentOf(long double) { return 0.0; }
+// The portable transcendentals (the device kernels' functions): their own
+// values for double, the same values with the analytic derivative for the
+// tangent pair; the extended build has none and keeps the C library's.
+inline double portableAtan2(double y, double x) { return swx::pm::atan2(y, x); }
+inline double portableSin(double x) { return swx::pm::sin(x); }
+inline double portableCos(double x) { return swx::pm::cos(x); }
+inline double portableAcos(double x) { return swx::pm::acos(x); }
+inline long double portableAtan2(long double y, long double x) { return std::atan2(y, x); }
+inline long double portableSin(long double x) { return std::sin(x); }
+inline long double portableCos(long double x) { return std::cos(x); }
+inline long double portableAcos(long double x) { return std::acos(x); }
+inline Tangent portableAtan2(const Tangent& y, const Tangent& x) {
+    const double r2 = x.v * x.v + y.v * y.v;
+    return {swx::pm::atan2(y.v, x.v), r2 > 0.0 ? (x.v * y.d - y.v * x.d) / r2 : 0.0};
+}
+inline Tangent portableSin(const Tangent& a) { return {swx::pm::sin(a.v), swx::pm::cos(a.v) * a.d}; }
+inline Tangent portableCos(const Tangent& a) { return {swx::pm::cos(a.v), -swx::pm::sin(a.v) * a.d}; }
+inline Tangent portableAcos(const Tangent& a) {
+    const double r = 1.0 - a.v * a.v;
+    return {swx::pm::acos(a.v), r > 0.0 ? -a.d / std::sqrt(r) : 0.0};
+}
 inline void seedTangent(Tangent& a, double d) { a.d = d; }
 inline void seedTangent(double&, double) {}
 inline void seedTangent(long double&, double) {}
@@ -144,25 +165,13 @@ thread_local std::string lastError;
 // Transcendentals: the C library's, as the reference calls them (default),
 // or the portable Real-Real functions that the device kernels use
 // (oracle_set_portable_math), which return the same bits on every platform.
-// In the extended and tangent builds the C library's functions are used
-// (the portable ones are off).
+// The tangent build follows the flag too (portable values, analytic
+// derivatives); the extended build has only the C library's functions.
 thread_local bool portableMath = false;
-Real mAtan2(Real y, Real x) {
-    if constexpr (std::is_same_v<Real, double>) return portableMath ? Real(swx::pm::atan2(value(y), value(x))) : atan2(y, x);
-    else return atan2(y, x);
-}
-Real mSin(Real x) {
-    if constexpr (std::is_same_v<Real, double>) return portableMath ? Real(swx::pm::sin(value(x))) : sin(x);
-    else return sin(x);
-}
-Real mCos(Real x) {
-    if constexpr (std::is_same_v<Real, double>) return portableMath ? Real(swx::pm::cos(value(x))) : cos(x);
-    else return cos(x);
-}
-Real mAcos(Real x) {
-    if constexpr (std::is_same_v<Real, double>) return portableMath ? Real(swx::pm::acos(value(x))) : acos(x);
-    else return acos(x);
-}
+Real mAtan2(Real y, Real x) { return portableMath ? oracle::portableAtan2(y, x) : atan2(y, x); }
+Real mSin(Real x) { return portableMath ? oracle::portableSin(x) : sin(x); }
+Real mCos(Real x) { return portableMath ? oracle::portableCos(x) : cos(x); }
+Real mAcos(Real x) { return portableMath ? oracle::portableAcos(x) : acos(x); }
 
 struct V {
     Real x = 0.0, y = 0.0, z = 0.0;

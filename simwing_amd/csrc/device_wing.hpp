@@ -261,7 +261,11 @@ private:
     std::vector<void*> owned_;
     double *initial_ = nullptr, *checkpoints_ = nullptr, *trace_ = nullptr;
     double *xbar_ = nullptr, *vbar_ = nullptr, *qbar_ = nullptr, *fbar_ = nullptr, *lbar_ = nullptr, *pbar_ = nullptr;
-    double *gTheta_ = nullptr, *gRest_ = nullptr, *gPressure_ = nullptr, *jac_ = nullptr;
+    double *gTheta_ = nullptr, *gRest_ = nullptr, *gPressure_ = nullptr;
+    double *jDist_ = nullptr, *jMemb_ = nullptr, *jHinge_ = nullptr, *jSeg_ = nullptr, *jSphere_ = nullptr,
+           *jClose_ = nullptr;
+    int *distPhase_ = nullptr, *membPhase_ = nullptr, *hingePhase_ = nullptr, *segPhase_ = nullptr,
+        *spherePhase_ = nullptr, *closePhase_ = nullptr;
     int lastPayloadPhase_ = -1;
     std::vector<double> baseCompliance_;  // 6 per membrane slot: xx yy zz xy xz yz
     std::vector<double> baseRest_;        // per segment slot

@@ -15,9 +15,10 @@ the parameter's magnitude for rest lengths and pressures) of
 
 and the exact derivatives of the oracle along the same parameters and along
 one random direction per family, from the oracle's tangent build (the same
-operation sequence carrying forward-mode tangent pairs): the values the
-adjoint is checked against to roundoff (the tangent-adjoint dot-product
-test).
+operation sequence carrying forward-mode tangent pairs, with the portable
+transcendentals, so that its trajectory is the device's bit for bit): the
+values the adjoint is checked against to roundoff (the tangent-adjoint
+dot-product test).
 
 Log-compliance theta is applied as the device library applies it: the
 element's four stiffness entries times exp(-theta) before the cofactor
@@ -86,6 +87,7 @@ def tangent_loss(job):
     (preset, steps), seeds, target = job
     definition, pressure, loads = flight_case(preset)
     body = orc.OracleBody(definition, extended="tangent")
+    body.use_portable_math(True)  # the device's transcendentals: the same trajectory bit for bit
     body.set_face_pressure(pressure)
     body.set_tangent(*seeds)
     settings = sw.flight_step_settings()

@@ -34,6 +34,27 @@ def test_extended_build_is_the_same_function():
     assert not np.array_equal(a["position"], b["position"])
 
 
+def test_tangent_build_values_are_the_double_build():
+    """The tangent pairs' values follow the double build's operations one for
+    one: with the portable transcendentals both return the same bits."""
+    d, p, l = flight_case("tiny")
+    bodies = [orc.OracleBody(d), orc.OracleBody(d, extended="tangent")]
+    for b in bodies:
+        b.use_portable_math(True)
+        b.set_face_pressure(p)
+    bodies[1].set_tangent(np.ones(d.arrays["membrane_triangle"].size), None, np.ones(d.triangle_count))
+    for b in bodies:
+        for _ in range(4):
+            b.set_external_forces(l)
+            b.step(sw.flight_step_settings())
+    a, t = bodies[0].state(), bodies[1].state()
+    for key in ("position", "velocity", "membrane_lambda", "segment_lambda", "payload"):
+        assert np.array_equal(a[key], t[key]), key
+    assert np.any(bodies[1].tangent_positions() != 0.0)
+    with pytest.raises(orc.OracleError):
+        bodies[0].tangent_positions()
+
+
 @pytest.mark.parametrize("wing", ["tiny", "small"])
 def test_difference_fixtures_are_consistent(wing):
     """The exact tangent derivatives, the long-double differences and the
