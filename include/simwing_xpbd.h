@@ -378,6 +378,11 @@ sw_status sw_adjoint_set_workgroups(sw_adjoint* adjoint, int workgroups);
 /* Device milliseconds of the last run: checkpointed forward pass, and
  * recorded replay + reverse kernels; device bytes the adjoint holds. */
 sw_status sw_adjoint_timing(const sw_adjoint* adjoint, double* forward_ms, double* reverse_ms, size_t* device_bytes);
+/* The last run's reverse pass split: recorded replay, Jacobian passes (every
+ * item's columns, one parallel launch per family and substep) and the
+ * cooperative sweeps that contract them. */
+sw_status sw_adjoint_timing_detail(const sw_adjoint* adjoint, double* replay_ms, double* jacobian_ms,
+                                   double* sweep_ms);
 
 #ifdef __cplusplus
 }

@@ -609,3 +609,10 @@ class Adjoint:
         f, r, b = C.c_double(), C.c_double(), C.c_size_t()
         check(native.library().sw_adjoint_timing(self._handle, C.byref(f), C.byref(r), C.byref(b)))
         return f.value, r.value, int(b.value)
+
+    def timing_detail(self) -> tuple[float, float, float]:
+        """The last run's reverse pass split (ms): recorded replay, Jacobian
+        passes, sweeps."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(native.library().sw_adjoint_timing_detail(self._handle, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value

@@ -875,6 +875,15 @@ sw_status sw_adjoint_set_workgroups(sw_adjoint* a, int workgroups) {
     });
 }
 
+sw_status sw_adjoint_timing_detail(const sw_adjoint* a, double* replay_ms, double* jacobian_ms, double* sweep_ms) {
+    return guarded([&] {
+        requirePointer(a, "adjoint");
+        if (replay_ms) *replay_ms = a->device->replayMs();
+        if (jacobian_ms) *jacobian_ms = a->device->jacobianMs();
+        if (sweep_ms) *sweep_ms = a->device->sweepMs();
+    });
+}
+
 sw_status sw_adjoint_timing(const sw_adjoint* a, double* forward_ms, double* reverse_ms, size_t* device_bytes) {
     return guarded([&] {
         requirePointer(a, "adjoint");

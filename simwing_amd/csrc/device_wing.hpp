@@ -93,8 +93,9 @@ public:
     void setExternalForces(const std::vector<double>& forces);
     void setFacePressure(const std::vector<double>& pressure);
 
-    // One substep on the wing's stream (asynchronous).
-    void launchSubstep(const StepArgs& args);
+    // One substep on the wing's stream (asynchronous); with `trace`, every
+    // projection also records its inputs there (the adjoint's replay).
+    void launchSubstep(const StepArgs& args, double* trace = nullptr);
     // Snapshot of the mutable state (taken only while no error is pending),
     // and restore from it.
     void snapshot();
@@ -241,6 +242,10 @@ public:
     // recorded replay plus the reverse kernels.
     double forwardMs() const { return forwardMs_; }
     double reverseMs() const { return reverseMs_; }
+    // The reverse pass split: recorded replay, Jacobian passes, sweeps.
+    double replayMs() const { return replayMs_; }
+    double jacobianMs() const { return jacobianMs_; }
+    double sweepMs() const { return sweepMs_; }
     std::size_t deviceBytes() const { return bytes_; }
 
 private:
@@ -264,13 +269,24 @@ private:
     double *gTheta_ = nullptr, *gRest_ = nullptr, *gPressure_ = nullptr;
     double *jDist_ = nullptr, *jMemb_ = nullptr, *jHinge_ = nullptr, *jSeg_ = nullptr, *jSphere_ = nullptr,
            *jClose_ = nullptr;
-    int *distPhase_ = nullptr, *membPhase_ = nullptr, *hingePhase_ = nullptr, *segPhase_ = nullptr,
-        *spherePhase_ = nullptr, *closePhase_ = nullptr;
+    int firstPayloadSlot_ = -1;
+    // Point-to-point schedule of the sweep (see buildReverseSync).
+    void buildReverseSync();
+    bool neighbourSync_ = false;
+    int workers_ = 1, runOwner_ = 0, inertiaOwner_ = 0, maxParts_ = 1;
+    void* depAlloc_[3] = {nullptr, nullptr, nullptr};
+    unsigned* flags_ = nullptr;
     int lastPayloadPhase_ = -1;
     std::vector<double> baseCompliance_;  // 6 per membrane slot: xx yy zz xy xz yz
     std::vector<double> baseRest_;        // per segment slot
-    double forwardMs_ = 0.0, reverseMs_ = 0.0;
+    double forwardMs_ = 0.0, reverseMs_ = 0.0, replayMs_ = 0.0, jacobianMs_ = 0.0, sweepMs_ = 0.0;
     hipEvent_t events_[3] = {nullptr, nullptr, nullptr};
+    // Events marking the reverse pass's pieces (grown on demand), and which
+    // pairs belong to which piece.
+    std::vector<hipEvent_t> marks_;
+    std::size_t marksUsed_ = 0;
+    std::vector<std::pair<std::size_t, int>> spans_;  // first mark of a pair, piece (0 replay, 1 jacobian, 2 sweep)
+    std::size_t mark();
 };
 
 // Device math conformance probe (function codes as sw_device_math).

@@ -83,9 +83,9 @@ struct WingView {
     // point-to-point synchronisation, dedicated chain workgroup, nodes in
     // LDS, and the wing's workgroup count in a batched launch.
     int neighbourSync, dedicatedChain, sharedNodes, launchBlocks;
-    // Trajectory recording for the adjoint (null otherwise; only with grid
-    // barriers between phases): the state before every phase of one
-    // substep, laid out as traceLayout() describes.
+    // Trajectory recording for the adjoint (null otherwise): every item
+    // writes the values its projection reads (its input vector), laid out as
+    // traceLayout() describes; any launch mode records the same values.
     double* trace;
     double* state;      // contiguous mutable state block
     double* snapshot;   // copy of `state`
@@ -93,28 +93,32 @@ struct WingView {
 };
 
 // Layout of one substep's trace, in doubles:
-//   prologue  x, v (3N each, node-major xyz), payload com, q, v, w (13);
-//   per iteration: every multiplier at its start (lambdaCount: distance,
-//     membrane x, y, z, hinge, segment, sphere), then per phase the node
-//     positions (3N) and payload com, q, inverse inertia (16) before it;
-//   per iteration: payload com, q and the two endpoint positions (13)
-//     before each segment of a serial run, at the segment's slot;
-//   closing: every multiplier, positions (3N) before each closing phase,
-//     then payload com, q (7) after the iterations.
+//   prologue: x, v (3N each, node-major xyz), payload com, q, v, w (13);
+//   per iteration, the input vector of every item before its projection,
+//     by family in slot order: distance (pa, pb, lambda: 7), membrane
+//     (p0, p1, p2, lambda xyz: 12), hinge (a, b, c, d, lambda: 13), segment
+//     (A, B, lambda, rest, payload com, q, inverse inertia: 24; zero for a
+//     harness endpoint), sphere (x, centre, lambda: 7);
+//   closing: the input vector (7) of every closing-stitch entry;
+//   end: payload com, q (7) after the iterations.
+constexpr int kRecDist = 7, kRecMemb = 12, kRecHinge = 13, kRecSeg = 24, kRecSphere = 7;
 struct TraceLayout {
-    long nodes3, lambdaCount, phaseSize, iterStride, iterBase, chainBase, chainStride, closeBase, total;
+    long nodes3, lambdaCount, distBase, membBase, hingeBase, segBase, sphereBase, iterStride, iterBase, closeBase,
+        total;
 };
 __host__ __device__ inline TraceLayout traceLayout(const WingView& v, int iterations) {
     TraceLayout t;
     t.nodes3 = 3L * v.nodeCount;
     t.lambdaCount = static_cast<long>(v.distCount) + 3L * v.membCount + v.hingeCount + v.segCount + v.pairCount;
-    t.phaseSize = t.nodes3 + 16;
-    t.iterStride = t.lambdaCount + static_cast<long>(v.iterPhaseCount) * t.phaseSize;
+    t.distBase = 0;
+    t.membBase = t.distBase + static_cast<long>(kRecDist) * v.distCount;
+    t.hingeBase = t.membBase + static_cast<long>(kRecMemb) * v.membCount;
+    t.segBase = t.hingeBase + static_cast<long>(kRecHinge) * v.hingeCount;
+    t.sphereBase = t.segBase + static_cast<long>(kRecSeg) * v.segCount;
+    t.iterStride = t.sphereBase + static_cast<long>(kRecSphere) * v.pairCount;
     t.iterBase = 2 * t.nodes3 + 13;
-    t.chainBase = t.iterBase + static_cast<long>(iterations) * t.iterStride;
-    t.chainStride = 13L * v.segCount;
-    t.closeBase = t.chainBase + static_cast<long>(iterations) * t.chainStride;
-    t.total = t.closeBase + t.lambdaCount + static_cast<long>(v.closePhaseCount) * t.nodes3 + 7;
+    t.closeBase = t.iterBase + static_cast<long>(iterations) * t.iterStride;
+    t.total = t.closeBase + static_cast<long>(kRecDist) * v.stitchCount + 7;
     return t;
 }
 
@@ -202,9 +206,22 @@ struct NodeRefs {
 };
 
 // Distance / cable / tie / stitch projection (soft_body.cpp:1598-1640).
-__device__ __forceinline__ void projectDistance(const WingView& v, const NodeRefs& N, int k, double invDt2) {
+__device__ __forceinline__ void put3(double* r, const d3& p) {
+    r[0] = p.x;
+    r[1] = p.y;
+    r[2] = p.z;
+}
+
+// `rec` (recording only): the item's input vector is written there first.
+__device__ __forceinline__ void projectDistance(const WingView& v, const NodeRefs& N, int k, double invDt2,
+                                                double* rec = nullptr) {
     const int a = v.da[k], b = v.db[k];
     d3 pa = N.get(a), pb = N.get(b);
+    if (rec) {
+        put3(rec, pa);
+        put3(rec + 3, pb);
+        rec[6] = v.dlam[k];
+    }
     const double wa = N.w[a], wb = N.w[b];
     const d3 diff = sub(pb, pa);
     const double length = len(diff);
@@ -263,9 +280,18 @@ __device__ __forceinline__ MembraneEval evaluateMembrane(d3 p0, d3 p1, d3 p2, do
 }
 
 // One membrane element (soft_body.cpp:1822-1929).
-__device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRefs& N, int k, const StepArgs& a) {
+__device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRefs& N, int k, const StepArgs& a,
+                                                double* rec = nullptr) {
     const int i0 = v.ma[k], i1 = v.mb[k], i2 = v.mc[k];
     const d3 p[3] = {N.get(i0), N.get(i1), N.get(i2)};
+    if (rec) {
+        put3(rec, p[0]);
+        put3(rec + 3, p[1]);
+        put3(rec + 6, p[2]);
+        rec[9] = v.mlx[k];
+        rec[10] = v.mly[k];
+        rec[11] = v.mlz[k];
+    }
     const double w[3] = {N.w[i0], N.w[i1], N.w[i2]};
     const double preW = v.anyPreTension ? v.preW[k] : 0.0;
     const double preT = v.anyPreTension ? v.preT[k] : 0.0;
@@ -335,9 +361,17 @@ __device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRef
 }
 
 // Dihedral hinge (soft_body.cpp:1648-1711 and 1754-1792).
-__device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& N, int k, double invDt2) {
+__device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& N, int k, double invDt2,
+                                             double* rec = nullptr) {
     const int idx[4] = {v.ha[k], v.hb[k], v.hc[k], v.hd[k]};
     const d3 a = N.get(idx[0]), b = N.get(idx[1]), c = N.get(idx[2]), d = N.get(idx[3]);
+    if (rec) {
+        put3(rec, a);
+        put3(rec + 3, b);
+        put3(rec + 6, c);
+        put3(rec + 9, d);
+        rec[12] = v.hlam[k];
+    }
     const d3 edge = sub(b, a);
     const d3 toC = sub(c, a);
     const d3 toD = sub(d, a);
@@ -573,8 +607,25 @@ struct SlotRefs {
 // inverse inertia in registers, so the serial chain waits on no global
 // memory; the workgroup finally writes nodes, multipliers and the payload
 // back. Identical arithmetic to projecting the segments in place.
+// Input vector of a segment projection: endpoint positions (zero for a
+// harness endpoint), multiplier, rest length, payload com, q, inverse inertia.
+__device__ __forceinline__ void recordSegment(double* r, const d3& pa, const d3& pb, double lambda, double rest,
+                                              const Payload& state, const m3& iinv) {
+    put3(r, pa);
+    put3(r + 3, pb);
+    r[6] = lambda;
+    r[7] = rest;
+    put3(r + 8, state.com);
+    r[11] = state.q.w;
+    r[12] = state.q.x;
+    r[13] = state.q.y;
+    r[14] = state.q.z;
+    for (int c = 0; c < 9; ++c) r[15 + c] = iinv.a[c];
+}
+
+// `rec`: the iteration's segment records (recording only).
 __device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, PayloadDev& payload,
-                                      const StepArgs& a, double* chainTrace) {
+                                      const StepArgs& a, double* rec) {
     const int slots = P.slotEnd - P.slotBegin;
     const int items = P.end - P.begin;
     for (int s = threadIdx.x; s < slots; s += blockDim.x) {
@@ -607,24 +658,9 @@ __device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, cons
         for (int j = 0; j < items; ++j) {
             const int fe = chainItem[4 * j], te = chainItem[4 * j + 1];
             double lambda = chainLambda[j];
-            if (chainTrace) {
-                double* t = chainTrace + 13L * (P.begin + j);
-                t[0] = state.com.x;
-                t[1] = state.com.y;
-                t[2] = state.com.z;
-                t[3] = state.q.w;
-                t[4] = state.q.x;
-                t[5] = state.q.y;
-                t[6] = state.q.z;
-                const d3 pa = fe >= 0 ? S.get(fe) : d3{0.0, 0.0, 0.0};
-                const d3 pb = te >= 0 ? S.get(te) : d3{0.0, 0.0, 0.0};
-                t[7] = pa.x;
-                t[8] = pa.y;
-                t[9] = pa.z;
-                t[10] = pb.x;
-                t[11] = pb.y;
-                t[12] = pb.z;
-            }
+            if (rec)
+                recordSegment(rec + kRecSeg * static_cast<long>(P.begin + j), fe >= 0 ? S.get(fe) : d3{0.0, 0.0, 0.0},
+                              te >= 0 ? S.get(te) : d3{0.0, 0.0, 0.0}, lambda, chainRest[j], state, iinv);
             if (!projectSegment(v, S, fe, te, chainRest[j], chainStiffness[j], lambda, chainItem[4 * j + 2], state,
                                 def, iinv, iinvFinite, args))
                 break;
@@ -687,10 +723,16 @@ __device__ __forceinline__ void publishCounter(unsigned* counter, unsigned value
 
 // Contact sphere pair (build-defined): node held off the centre node to the
 // radius; the centre is not moved by contact.
-__device__ __forceinline__ void projectSphere(const WingView& v, const NodeRefs& N, int k, double invDt2) {
+__device__ __forceinline__ void projectSphere(const WingView& v, const NodeRefs& N, int k, double invDt2,
+                                              double* rec = nullptr) {
     const int node = v.spNode[k];
     const d3 c = N.get(v.spCentre[k]);
     d3 x = N.get(node);
+    if (rec) {
+        put3(rec, x);
+        put3(rec + 3, c);
+        rec[6] = v.splam[k];
+    }
     const d3 diff = sub(x, c);
     const double length = len(diff);
     if (length <= 1.0e-12) return;
@@ -781,46 +823,24 @@ __device__ __forceinline__ void atomicMaxNonNegative(double* address, double val
     atomicMax(reinterpret_cast<unsigned long long*>(address), static_cast<unsigned long long>(__double_as_longlong(value)));
 }
 
-// Trajectory recording (adjoint): every multiplier, and node positions in
-// node-major xyz order, written with the given start and stride.
-__device__ void recordLambdas(const WingView& v, double* out, int first, int stride) {
-    long o = 0;
-    for (int k = first; k < v.distCount; k += stride) out[k] = v.dlam[k];
-    o += v.distCount;
-    for (int k = first; k < v.membCount; k += stride) {
-        out[o + k] = v.mlx[k];
-        out[o + v.membCount + k] = v.mly[k];
-        out[o + 2L * v.membCount + k] = v.mlz[k];
-    }
-    o += 3L * v.membCount;
-    for (int k = first; k < v.hingeCount; k += stride) out[o + k] = v.hlam[k];
-    o += v.hingeCount;
-    for (int k = first; k < v.segCount; k += stride) out[o + k] = v.slam[k];
-    o += v.segCount;
-    for (int k = first; k < v.pairCount; k += stride) out[o + k] = v.splam[k];
-}
-__device__ void recordPositions(const NodeRefs& N, int n, double* out, int first, int stride) {
-    for (int i = first; i < n; i += stride) {
-        out[3L * i] = N.x[i];
-        out[3L * i + 1] = N.y[i];
-        out[3L * i + 2] = N.z[i];
-    }
-}
-
 // Items [first, end) with the given stride of one phase of the iteration
 // program; a serial run (kPhaseSegmentRun) is staged and projected by the
-// calling workgroup as a whole.
+// calling workgroup as a whole. `rec`: the iteration's trace records
+// (recording only).
 __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, int first, int end, int stride,
-                         PayloadDev& payload, const StepArgs& a, double* chainTrace = nullptr) {
+                         PayloadDev& payload, const StepArgs& a, double* rec, const TraceLayout& L) {
     switch (P.type) {
     case kPhaseDistance:
-        for (int k = first; k < end; k += stride) projectDistance(v, N, k, a.invDt2);
+        for (int k = first; k < end; k += stride)
+            projectDistance(v, N, k, a.invDt2, rec ? rec + L.distBase + kRecDist * static_cast<long>(k) : nullptr);
         break;
     case kPhaseMembrane:
-        for (int k = first; k < end; k += stride) projectMembrane(v, N, k, a);
+        for (int k = first; k < end; k += stride)
+            projectMembrane(v, N, k, a, rec ? rec + L.membBase + kRecMemb * static_cast<long>(k) : nullptr);
         break;
     case kPhaseHinge:
-        for (int k = first; k < end; k += stride) projectHinge(v, N, k, a.invDt2);
+        for (int k = first; k < end; k += stride)
+            projectHinge(v, N, k, a.invDt2, rec ? rec + L.hingeBase + kRecHinge * static_cast<long>(k) : nullptr);
         break;
     case kPhaseSegments:
         for (int k = first; k < end; k += stride) {
@@ -828,6 +848,9 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
             const m3 iinv = loadInverseInertia(payload);
             const int fe = v.sfrom[k], te = v.sto[k];
             double lambda = v.slam[k];
+            if (rec)
+                recordSegment(rec + L.segBase + kRecSeg * static_cast<long>(k), fe >= 0 ? N.get(fe) : d3{0.0, 0.0, 0.0},
+                              te >= 0 ? N.get(te) : d3{0.0, 0.0, 0.0}, lambda, v.srest[k], state, iinv);
             if (!projectSegment(v, N, fe, te, v.srest[k], v.sstiff[k], lambda, v.sruntime[k], state, payload, iinv,
                                 finiteM(iinv), a))
                 break;
@@ -840,10 +863,11 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
         }
         break;
     case kPhaseSegmentRun:
-        runChain(v, N, P, payload, a, chainTrace);
+        runChain(v, N, P, payload, a, rec ? rec + L.segBase : nullptr);
         break;
     case kPhaseSpheres:
-        for (int k = first; k < end; k += stride) projectSphere(v, N, k, a.invDt2);
+        for (int k = first; k < end; k += stride)
+            projectSphere(v, N, k, a.invDt2, rec ? rec + L.sphereBase + kRecSphere * static_cast<long>(k) : nullptr);
         break;
     default:
         break;
@@ -987,7 +1011,8 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                 if (b >= e) continue;
                 const int step = it * m + ph;
                 waitDeps(v, p * m + ph, step);
-                runPhase(v, N, P, b + static_cast<int>(threadIdx.x), e, static_cast<int>(blockDim.x), payload, a);
+                runPhase(v, N, P, b + static_cast<int>(threadIdx.x), e, static_cast<int>(blockDim.x), payload, a,
+                         v.trace ? v.trace + layout.iterBase + it * layout.iterStride : nullptr, layout);
                 __syncthreads();
                 publishCounter(v.flags + 32 * p, static_cast<unsigned>(step + 1));
             }
@@ -998,48 +1023,30 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         if (threadIdx.x == 0) __hip_atomic_store(v.flags + 32 * p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (chainGroup) return;
     } else {
-        for (int it = 0; it < a.iterations; ++it)
+        for (int it = 0; it < a.iterations; ++it) {
+            double* rec = v.trace ? v.trace + layout.iterBase + it * layout.iterStride : nullptr;
             for (int ph = 0; ph < v.iterPhaseCount; ++ph) {
                 const Phase P = v.iterPhases[ph];
-                double* chainTrace = nullptr;
-                if (v.trace) {
-                    double* base = v.trace + layout.iterBase + it * layout.iterStride;
-                    if (ph == 0) recordLambdas(v, base, gtid, gstride);
-                    double* slot = base + layout.lambdaCount + ph * layout.phaseSize;
-                    recordPositions(N, n, slot, gtid, gstride);
-                    if (gtid == 0) {
-                        double* t = slot + layout.nodes3;
-                        for (int k = 0; k < 3; ++k) t[k] = payload.com[k];
-                        for (int k = 0; k < 4; ++k) t[3 + k] = payload.q[k];
-                        for (int k = 0; k < 9; ++k) t[7 + k] = payload.iinv[k];
-                    }
-                    chainTrace = v.trace + layout.chainBase + it * layout.chainStride;
-                    gridSync(v, workers);  // the phase starts after its state is recorded
-                }
                 if (P.type == kPhaseSegmentRun) {
-                    if (block == 0) runPhase(v, N, P, P.begin, P.end, 1, payload, a, chainTrace);
+                    if (block == 0) runPhase(v, N, P, P.begin, P.end, 1, payload, a, rec, layout);
                 } else {
-                    runPhase(v, N, P, P.begin + gtid, P.end, gstride, payload, a, chainTrace);
+                    runPhase(v, N, P, P.begin + gtid, P.end, gstride, payload, a, rec, layout);
                 }
                 gridSync(v, workers);
             }
+        }
     }
     // Closing seam-stitch pass (soft_body.cpp:1389-1405).
-    if (v.trace) {
-        recordLambdas(v, v.trace + layout.closeBase, gtid, gstride);
-        if (gtid == 0) {
-            double* t = v.trace + layout.total - 7;
-            for (int k = 0; k < 3; ++k) t[k] = payload.com[k];
-            for (int k = 0; k < 4; ++k) t[3 + k] = payload.q[k];
-        }
+    if (v.trace && gtid == 0) {
+        double* t = v.trace + layout.total - 7;
+        for (int k = 0; k < 3; ++k) t[k] = payload.com[k];
+        for (int k = 0; k < 4; ++k) t[3 + k] = payload.q[k];
     }
     for (int ph = 0; ph < v.closePhaseCount; ++ph) {
         const Phase P = v.closePhases[ph];
-        if (v.trace) {
-            recordPositions(N, n, v.trace + layout.closeBase + layout.lambdaCount + ph * layout.nodes3, gtid, gstride);
-            gridSync(v, workers);
-        }
-        for (int k = P.begin + gtid; k < P.end; k += gstride) projectDistance(v, N, v.stitchItems[k], a.invDt2);
+        for (int k = P.begin + gtid; k < P.end; k += gstride)
+            projectDistance(v, N, v.stitchItems[k], a.invDt2,
+                            v.trace ? v.trace + layout.closeBase + kRecDist * static_cast<long>(k) : nullptr);
         gridSync(v, workers);
     }
 
@@ -1809,24 +1816,25 @@ void DeviceWing::build(const Definition& d) {
         std::vector<unsigned> barrier(128, 0);  // count, generation, sync-failure flag
         v.barrier = deviceCopy(allocations_, barrier);
     }
-    // Launch shape: one workgroup with LDS-resident nodes when they fit
-    // beside the kernel's static LDS (the serial-run staging).
+    // Launch shape: one workgroup with LDS-resident nodes for a small wing
+    // whose nodes fit beside the kernel's static LDS (the serial-run
+    // staging); else nodes stay in HBM and a cooperative grid of up to one
+    // workgroup per compute unit runs the phases with point-to-point
+    // synchronisation: 16 workgroups up to ~50k items, 64 beyond (measured
+    // on the 1.7k-, 6.6k- and 43k-node flight wings).
     const std::size_t dynamicLds = (4 * nodes_ + kMaxThreads) * sizeof(double);
-    sharedNodes_ = dynamicLds <= sharedNodeBudget();
+    sharedNodes_ = dynamicLds <= sharedNodeBudget() && itemCount() <= 4096;
     workgroups_ = 1;
     threads_ = kMaxThreads;
-    if (sharedNodes_) {
+    if (!sharedNodes_) {
+        int cus = 0;
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
+        workgroups_ = std::max(1, std::min(cus, itemCount() < 50000 ? 16 : 64));
+    }
+    if (dynamicLds <= sharedNodeBudget())  // also for a later single-workgroup setLaunch
         check(hipFuncSetAttribute(reinterpret_cast<const void*>(&substepKernel<true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamicLds)),
               "shared memory attribute");
-    } else {
-        // Nodes stay in HBM: a cooperative grid of one workgroup per compute
-        // unit, capped where the grid barrier starts to cost more than the
-        // extra lanes save.
-        int cus = 0;
-        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
-        workgroups_ = std::max(1, std::min(cus, 64));
-    }
     {
         int cus = 0;
         check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
@@ -1889,8 +1897,9 @@ void DeviceWing::setFacePressure(const std::vector<double>& pressure) {
     check(hipStreamSynchronize(stream_), "pressure");
 }
 
-void DeviceWing::launchSubstep(const StepArgs& stepArgs) {
+void DeviceWing::launchSubstep(const StepArgs& stepArgs, double* trace) {
     WingView v = *view_;
+    v.trace = trace;
     v.neighbourSync = neighbourSync_ ? 1 : 0;
     v.dedicatedChain = dedicatedChain_ ? 1 : 0;
     v.sharedNodes = sharedNodes_ ? 1 : 0;

@@ -233,6 +233,8 @@ _SIGNATURES = {
     "sw_adjoint_set_workgroups": (C.c_int, [C.c_void_p, C.c_int]),
     "sw_adjoint_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_size_t)]),
+    "sw_adjoint_timing_detail": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double)]),
 }
 
 _lib = None
