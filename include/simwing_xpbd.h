@@ -384,6 +384,34 @@ sw_status sw_adjoint_timing(const sw_adjoint* adjoint, double* forward_ms, doubl
 sw_status sw_adjoint_timing_detail(const sw_adjoint* adjoint, double* replay_ms, double* jacobian_ms,
                                    double* sweep_ms);
 
+/* ---- Batched adjoint ---------------------------------------------------- */
+
+typedef struct sw_batch_adjoint sw_batch_adjoint;
+
+/* The adjoint of a batch (no counterpart in the reference): every wing's
+ * shape-loss gradient with respect to its line rest lengths (definition
+ * order), from the wings' current states over `steps` steps with the loads
+ * pending now. The forward pass and the checkpoint replays are the batch's
+ * own persistent launches; the reverse sweeps of all wings run in one
+ * persistent launch per substep and Jacobian chunk (`jacobian_chunk`
+ * iterations of Jacobian columns held at once, 0 = all: memory for launches).
+ * Each wing's gradient equals the one its own sw_adjoint computes. The batch
+ * must outlive the adjoint. */
+sw_status sw_batch_adjoint_configure(sw_batch* b, const sw_step_settings* settings, uint64_t steps,
+                                     uint32_t checkpoint_interval, uint32_t jacobian_chunk, sw_batch_adjoint** out);
+void sw_batch_adjoint_destroy(sw_batch_adjoint* adjoint);
+/* Per wing: rest lengths (rest_lengths NULL or an entry NULL: the
+ * definition's), target positions; outputs per wing (NULL arrays or entries
+ * skipped): loss, rest-length gradient, final positions. A failing step
+ * restores every wing and reports the first failing wing's error. */
+sw_status sw_batch_adjoint_run(sw_batch_adjoint* adjoint, const double* const* rest_lengths,
+                               const double* const* targets, double* losses, double* const* grad_rest_lengths,
+                               double* const* final_positions);
+sw_status sw_batch_adjoint_timing(const sw_batch_adjoint* adjoint, double* forward_ms, double* reverse_ms,
+                                  size_t* device_bytes);
+/* Workgroups of each wing in the batched reverse sweep. */
+sw_status sw_batch_adjoint_workgroups(const sw_batch_adjoint* adjoint, int* per_wing, size_t count);
+
 #ifdef __cplusplus
 }
 #endif

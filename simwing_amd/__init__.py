@@ -27,6 +27,7 @@ __all__ = [
     "Structure",
     "Batch",
     "Adjoint",
+    "BatchAdjoint",
     "generate_scene",
     "flight_step_settings",
     "flight_assembly_settings",
@@ -492,8 +493,11 @@ class Batch:
         handle = C.c_void_p()
         check(lib.sw_batch_create(ptrs, len(views), device, C.byref(handle)))
         self._handle = handle
+        self._dependents = []  # adjoints borrowing this batch: closed first
 
     def close(self) -> None:
+        for dependent in getattr(self, "_dependents", []):
+            dependent.close()
         if getattr(self, "_handle", None):
             native.library().sw_batch_destroy(self._handle)
             self._handle = None
@@ -616,3 +620,70 @@ class Adjoint:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         check(native.library().sw_adjoint_timing_detail(self._handle, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+
+class BatchAdjoint:
+    """Rest-length gradients of every wing's shape loss, the wings stepped
+    together (sw_batch_adjoint_*)."""
+
+    def __init__(self, batch: Batch, steps: int, checkpoint_interval: int = 16, jacobian_chunk: int = 0,
+                 settings: native.StepSettings | None = None):
+        self._batch = batch
+        self.wings = len(batch._definitions)
+        self.segment_counts = [0 if d.suspension is None else int(np.asarray(d.suspension["segment_rest"]).size)
+                               for d in batch._definitions]
+        self.node_counts = [d.node_count for d in batch._definitions]
+        s = settings or native.flight_step_settings()
+        handle = C.c_void_p()
+        check(native.library().sw_batch_adjoint_configure(batch._handle, C.byref(s), int(steps),
+                                                          int(checkpoint_interval), int(jacobian_chunk),
+                                                          C.byref(handle)))
+        self._handle = handle
+        batch._dependents.append(self)
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            native.library().sw_batch_adjoint_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, targets, rest_lengths=None, gradients: bool = True) -> dict:
+        n = self.wings
+        keep = []
+
+        def table(arrays, sizes):
+            ptrs = (native._F64P * n)()
+            for w in range(n):
+                if arrays is not None and arrays[w] is not None:
+                    a = np.ascontiguousarray(arrays[w], dtype=np.float64).reshape(-1)
+                    if a.size != sizes[w]:
+                        raise ValueError(f"wing {w}: expected {sizes[w]} values, got {a.size}")
+                    keep.append(a)
+                    ptrs[w] = f64p(a)
+            return ptrs
+
+        t = table(targets, [3 * c for c in self.node_counts])
+        r = table(rest_lengths, self.segment_counts) if rest_lengths is not None else None
+        losses = np.zeros(n)
+        grads = [np.zeros(c) for c in self.segment_counts]
+        finals = [np.zeros(3 * c) for c in self.node_counts]
+        g = table(grads, self.segment_counts) if gradients else None
+        fp = table(finals, [3 * c for c in self.node_counts])
+        # `keep` holds every array the pointer tables point into until the call returns.
+        check(native.library().sw_batch_adjoint_run(self._handle, r, t, f64p(losses), g, fp))
+        out = {"loss": losses, "final_position": [f.reshape(-1, 3) for f in finals]}
+        if gradients:
+            out["grad_rest_length"] = grads
+        return out
+
+    def timing(self) -> tuple[float, float, int]:
+        f, r, b = C.c_double(), C.c_double(), C.c_size_t()
+        check(native.library().sw_batch_adjoint_timing(self._handle, C.byref(f), C.byref(r), C.byref(b)))
+        return f.value, r.value, int(b.value)
+
+    def workgroups(self) -> np.ndarray:
+        out = (C.c_int * self.wings)()
+        check(native.library().sw_batch_adjoint_workgroups(self._handle, out, self.wings))
+        return np.array(out[:], dtype=np.int64)

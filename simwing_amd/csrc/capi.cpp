@@ -821,6 +821,84 @@ sw_status sw_batch_workgroups(const sw_batch* b, int* per_wing, size_t count) {
     });
 }
 
+// ---- Batched adjoint ---------------------------------------------------
+
+struct sw_batch_adjoint {
+    sw_batch* batch = nullptr;
+    std::vector<std::vector<double>> loads;  // per wing, pending at configuration
+    std::unique_ptr<swx::DeviceBatchAdjoint> device;
+};
+
+sw_status sw_batch_adjoint_configure(sw_batch* b, const sw_step_settings* settings, uint64_t steps,
+                                     uint32_t checkpoint_interval, uint32_t jacobian_chunk, sw_batch_adjoint** out) {
+    return guarded([&] {
+        requirePointer(b, "batch");
+        requirePointer(settings, "settings");
+        requirePointer(out, "out");
+        if (steps < 1 || steps > (1ull << 40)) throw std::invalid_argument("adjoint step count must be positive");
+        if (checkpoint_interval < 1) throw std::invalid_argument("checkpoint interval must be positive");
+        auto a = std::make_unique<sw_batch_adjoint>();
+        a->batch = b;
+        std::vector<const swx::Definition*> definitions;
+        std::vector<swx::StepArgs> args;
+        for (auto& s : b->wings) {
+            args.push_back(prepare(*s, *settings));
+            definitions.push_back(&s->definition);
+            a->loads.push_back(s->pending);
+        }
+        a->device = std::make_unique<swx::DeviceBatchAdjoint>(*b->device, definitions, args, settings->substeps,
+                                                              static_cast<long>(steps),
+                                                              static_cast<int>(checkpoint_interval),
+                                                              static_cast<int>(jacobian_chunk));
+        *out = a.release();
+    });
+}
+
+void sw_batch_adjoint_destroy(sw_batch_adjoint* a) { delete a; }
+
+sw_status sw_batch_adjoint_run(sw_batch_adjoint* a, const double* const* rest_lengths, const double* const* targets,
+                               double* losses, double* const* grad_rest_lengths, double* const* final_positions) {
+    return guarded([&] {
+        requirePointer(a, "adjoint");
+        requirePointer(targets, "targets");
+        const size_t n = a->batch->wings.size();
+        for (size_t w = 0; w < n; ++w) {
+            sw_structure& s = *a->batch->wings[w];
+            s.wing->setExternalForces(a->loads[w]);
+            s.pendingDirty = true;
+        }
+        std::vector<const double*> rest, target(targets, targets + n);
+        if (rest_lengths) rest.assign(rest_lengths, rest_lengths + n);
+        std::vector<double*> grads, finals;
+        if (grad_rest_lengths) grads.assign(grad_rest_lengths, grad_rest_lengths + n);
+        if (final_positions) finals.assign(final_positions, final_positions + n);
+        try {
+            a->device->run(rest, target, losses, grads, finals);
+        } catch (const swx::DeviceStepFailure& e) {
+            throwDeviceError(*a->batch->wings[static_cast<size_t>(std::max(a->device->failedWing(), 0))], e.error);
+        }
+    });
+}
+
+sw_status sw_batch_adjoint_timing(const sw_batch_adjoint* a, double* forward_ms, double* reverse_ms,
+                                  size_t* device_bytes) {
+    return guarded([&] {
+        requirePointer(a, "adjoint");
+        if (forward_ms) *forward_ms = a->device->forwardMs();
+        if (reverse_ms) *reverse_ms = a->device->reverseMs();
+        if (device_bytes) *device_bytes = a->device->deviceBytes();
+    });
+}
+
+sw_status sw_batch_adjoint_workgroups(const sw_batch_adjoint* a, int* per_wing, size_t count) {
+    return guarded([&] {
+        requirePointer(a, "adjoint");
+        const auto& blocks = a->device->sweepWorkgroups();
+        if (count != blocks.size()) throw std::invalid_argument("per_wing must hold one entry per wing");
+        for (size_t w = 0; w < count; ++w) per_wing[w] = blocks[w];
+    });
+}
+
 // ---- Adjoint -----------------------------------------------------------
 
 struct sw_adjoint {

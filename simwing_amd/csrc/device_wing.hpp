@@ -66,7 +66,8 @@ struct DeviceError {
     int index;
 };
 
-struct WingView;  // device-side view (defined in the .hip file)
+struct WingView;     // device-side view (defined in the .hip file)
+struct AdjointView;  // device-side adjoint buffers (defined with the adjoint kernels)
 
 struct StepArgs {
     double dt;
@@ -162,6 +163,7 @@ public:
 private:
     friend class DeviceBatch;
     friend class DeviceAdjoint;
+    friend class DeviceBatchAdjoint;
     void build(const Definition& d);
     int device_;
     hipStream_t stream_ = nullptr;
@@ -224,8 +226,10 @@ struct DeviceStepFailure : std::runtime_error {
 // wing's state and parameters as they were.
 class DeviceAdjoint {
 public:
+    // jacobianChunk: iterations whose Jacobian columns are held at once
+    // (0: all of a substep; fewer trades launches for memory).
     DeviceAdjoint(DeviceWing& wing, const Definition& definition, const StepArgs& args, int substeps, long steps,
-                  int interval);
+                  int interval, int jacobianChunk = 0);
     ~DeviceAdjoint();
     DeviceAdjoint(const DeviceAdjoint&) = delete;
     DeviceAdjoint& operator=(const DeviceAdjoint&) = delete;
@@ -249,6 +253,13 @@ public:
     std::size_t deviceBytes() const { return bytes_; }
 
 private:
+    friend class DeviceBatchAdjoint;
+    AdjointView adjointView(const double* trace) const;
+    WingView sweepView() const;
+    void launchJacobians(const WingView& v, const AdjointView& A);
+    void launchPressure(const WingView& v, const AdjointView& A);
+    int jacobianChunk_ = 1;
+    hipStream_t stream_ = nullptr;  // stream of the reverse launches (the wing's, or a batch's)
     void applyParameters(const double* logCompliance, const double* restLength, const double* pressure);
     void restoreBase();
     void launchRecording(double* trace);
@@ -289,6 +300,48 @@ private:
     std::size_t mark();
 };
 
+// The adjoint of many wings stepped together: the forward pass and the
+// recorded replays are the batch's own launches, the reverse sweeps of all
+// wings run in one persistent launch per substep (and Jacobian chunk); each
+// wing's gradient equals that of its own DeviceAdjoint. Parameters: the
+// wings' segment rest lengths (gradients of the shape loss per wing).
+class DeviceBatchAdjoint {
+public:
+    DeviceBatchAdjoint(DeviceBatch& batch, const std::vector<const Definition*>& definitions,
+                       const std::vector<StepArgs>& args, int substeps, long steps, int interval, int jacobianChunk);
+    ~DeviceBatchAdjoint();
+    DeviceBatchAdjoint(const DeviceBatchAdjoint&) = delete;
+    DeviceBatchAdjoint& operator=(const DeviceBatchAdjoint&) = delete;
+
+    // Per wing: rest lengths (definition order; empty vector or null entry:
+    // the definition's), target positions (3 per node); outputs per wing
+    // (null entries skipped): loss, rest-length gradient, final positions.
+    void run(const std::vector<const double*>& rest, const std::vector<const double*>& targets, double* losses,
+             const std::vector<double*>& gradRest, const std::vector<double*>& finalPositions);
+    const std::vector<int>& sweepWorkgroups() const { return blocks_; }
+    int failedWing() const { return failedWing_; }
+    double forwardMs() const { return forwardMs_; }
+    double reverseMs() const { return reverseMs_; }
+    std::size_t deviceBytes() const { return bytes_; }
+
+private:
+    DeviceBatch& batch_;
+    std::vector<StepArgs> args_;
+    int substeps_;
+    long steps_;
+    int interval_;
+    std::vector<std::unique_ptr<DeviceAdjoint>> wings_;
+    std::vector<int> blocks_;
+    int totalBlocks_ = 0, chunks_ = 1, failedWing_ = -1;
+    DeviceError failure_{};
+    std::vector<void*> owned_;
+    void *views_ = nullptr, *sweepArgs_ = nullptr, *blockWing_ = nullptr, *blockLocal_ = nullptr,
+         *blockCount_ = nullptr, *traceTable_ = nullptr, *adjointTable_ = nullptr;
+    std::size_t bytes_ = 0;
+    double forwardMs_ = 0.0, reverseMs_ = 0.0;
+    hipEvent_t events_[3] = {nullptr, nullptr, nullptr};
+};
+
 // Device math conformance probe (function codes as sw_device_math).
 void deviceMath(int device, int function, const double* a, const double* b, double* out, std::size_t count);
 
@@ -320,11 +373,14 @@ public:
 
     void snapshot();                                 // wings without a pending error
     void restore(const std::vector<unsigned char>& which);
-    void launchSubstep(const std::vector<StepArgs>& args);
+    // `traces` (a device array of one pointer per wing, or null): the
+    // adjoint's recording of this substep for each wing.
+    void launchSubstep(const std::vector<StepArgs>& args, double* const* traces = nullptr);
     void synchronize();
     void setCooperative(bool cooperative) { cooperative_ = cooperative; }
 
 private:
+    friend class DeviceBatchAdjoint;
     void release();
     std::vector<DeviceWing*> wings_;
     int device_;

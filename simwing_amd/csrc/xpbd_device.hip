@@ -878,7 +878,8 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
 // `block`. kMode: 0 nodes in HBM, 1 nodes resident in LDS, 2 either, as the
 // wing's view says (batched launches mix both kinds of wing).
 template <int kMode>
-__device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a, int block, int blocks) {
+// `trace`: the adjoint's recording of this substep, or null.
+__device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a, int block, int blocks, double* trace) {
     extern __shared__ double lds[];
     // A pending failure stops later substeps of a queued batch. Only a
     // single-workgroup wing may leave early: with several workgroups the
@@ -909,19 +910,19 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
     }
     PayloadDev& payload = *v.payload;
     const TraceLayout layout = traceLayout(v, a.iterations);
-    if (v.trace) {
+    if (trace) {
         // Step-start state. Phase A writes no position and no velocity, and
         // only workgroup 0's thread 0 touches the payload.
         for (int i = gtid; i < n; i += gstride) {
-            v.trace[3L * i] = v.px[i];
-            v.trace[3L * i + 1] = v.py[i];
-            v.trace[3L * i + 2] = v.pz[i];
-            v.trace[layout.nodes3 + 3L * i] = v.vx[i];
-            v.trace[layout.nodes3 + 3L * i + 1] = v.vy[i];
-            v.trace[layout.nodes3 + 3L * i + 2] = v.vz[i];
+            trace[3L * i] = v.px[i];
+            trace[3L * i + 1] = v.py[i];
+            trace[3L * i + 2] = v.pz[i];
+            trace[layout.nodes3 + 3L * i] = v.vx[i];
+            trace[layout.nodes3 + 3L * i + 1] = v.vy[i];
+            trace[layout.nodes3 + 3L * i + 2] = v.vz[i];
         }
         if (gtid == 0) {
-            double* t = v.trace + 2 * layout.nodes3;
+            double* t = trace + 2 * layout.nodes3;
             for (int k = 0; k < 3; ++k) t[k] = payload.com[k];
             for (int k = 0; k < 4; ++k) t[3 + k] = payload.q[k];
             for (int k = 0; k < 3; ++k) t[7 + k] = payload.v[k];
@@ -1012,7 +1013,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                 const int step = it * m + ph;
                 waitDeps(v, p * m + ph, step);
                 runPhase(v, N, P, b + static_cast<int>(threadIdx.x), e, static_cast<int>(blockDim.x), payload, a,
-                         v.trace ? v.trace + layout.iterBase + it * layout.iterStride : nullptr, layout);
+                         trace ? trace + layout.iterBase + it * layout.iterStride : nullptr, layout);
                 __syncthreads();
                 publishCounter(v.flags + 32 * p, static_cast<unsigned>(step + 1));
             }
@@ -1024,7 +1025,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         if (chainGroup) return;
     } else {
         for (int it = 0; it < a.iterations; ++it) {
-            double* rec = v.trace ? v.trace + layout.iterBase + it * layout.iterStride : nullptr;
+            double* rec = trace ? trace + layout.iterBase + it * layout.iterStride : nullptr;
             for (int ph = 0; ph < v.iterPhaseCount; ++ph) {
                 const Phase P = v.iterPhases[ph];
                 if (P.type == kPhaseSegmentRun) {
@@ -1037,8 +1038,8 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         }
     }
     // Closing seam-stitch pass (soft_body.cpp:1389-1405).
-    if (v.trace && gtid == 0) {
-        double* t = v.trace + layout.total - 7;
+    if (trace && gtid == 0) {
+        double* t = trace + layout.total - 7;
         for (int k = 0; k < 3; ++k) t[k] = payload.com[k];
         for (int k = 0; k < 4; ++k) t[3 + k] = payload.q[k];
     }
@@ -1046,7 +1047,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         const Phase P = v.closePhases[ph];
         for (int k = P.begin + gtid; k < P.end; k += gstride)
             projectDistance(v, N, v.stitchItems[k], a.invDt2,
-                            v.trace ? v.trace + layout.closeBase + kRecDist * static_cast<long>(k) : nullptr);
+                            trace ? trace + layout.closeBase + kRecDist * static_cast<long>(k) : nullptr);
         gridSync(v, workers);
     }
 
@@ -1135,16 +1136,18 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
 
 template <bool kShared>
 __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArgs a) {
-    substepBody<kShared ? 1 : 0>(v, a, static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+    substepBody<kShared ? 1 : 0>(v, a, static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), v.trace);
 }
 
 // Batched substep: workgroup b runs workgroup blockLocal[b] of wing
 // blockWing[b]; each wing's view holds its own workgroup count and modes.
+// `traces` (recording only): each wing's trace for this substep.
 __global__ void __launch_bounds__(kMaxThreads) batchSubstepKernel(const WingView* views, const StepArgs* args,
-                                                                   const int* blockWing, const int* blockLocal) {
+                                                                   const int* blockWing, const int* blockLocal,
+                                                                   double* const* traces) {
     const int wing = blockWing[blockIdx.x];
     const WingView& v = views[wing];
-    substepBody<2>(v, args[wing], blockLocal[blockIdx.x], v.launchBlocks);
+    substepBody<2>(v, args[wing], blockLocal[blockIdx.x], v.launchBlocks, traces ? traces[wing] : nullptr);
 }
 
 // Copies the mutable state block into the snapshot while no error is
@@ -2374,7 +2377,7 @@ void DeviceBatch::restore(const std::vector<unsigned char>& which) {
     check(hipStreamSynchronize(stream_), "batch restore");
 }
 
-void DeviceBatch::launchSubstep(const std::vector<StepArgs>& args) {
+void DeviceBatch::launchSubstep(const std::vector<StepArgs>& args, double* const* traces) {
     if (!views_) throw std::logic_error("the batch has no launch plan");
     check(hipMemcpyAsync(args_, args.data(), args.size() * sizeof(StepArgs), hipMemcpyHostToDevice, stream_),
           "batch arguments");
@@ -2384,9 +2387,9 @@ void DeviceBatch::launchSubstep(const std::vector<StepArgs>& args) {
     const int* blockLocal = static_cast<const int*>(blockLocal_);
     if (!cooperative_) {
         hipLaunchKernelGGL(batchSubstepKernel, dim3(total_), dim3(threads_), lds_, stream_, views, stepArgs, blockWing,
-                           blockLocal);
+                           blockLocal, traces);
     } else {
-        void* params[] = {&views, &stepArgs, &blockWing, &blockLocal};
+        void* params[] = {&views, &stepArgs, &blockWing, &blockLocal, &traces};
         check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&batchSubstepKernel), dim3(total_),
                                          dim3(threads_), params, lds_, stream_),
               "cooperative batch launch");

@@ -228,6 +228,14 @@ _SIGNATURES = {
     "sw_adjoint_configure": (C.c_int, [C.c_void_p, C.POINTER(StepSettings), C.c_uint64, C.c_uint32,
                                        C.POINTER(C.c_void_p)]),
     "sw_adjoint_destroy": (None, [C.c_void_p]),
+    "sw_batch_adjoint_configure": (C.c_int, [C.c_void_p, C.POINTER(StepSettings), C.c_uint64, C.c_uint32, C.c_uint32,
+                                             C.POINTER(C.c_void_p)]),
+    "sw_batch_adjoint_destroy": (None, [C.c_void_p]),
+    "sw_batch_adjoint_run": (C.c_int, [C.c_void_p, C.POINTER(_F64P), C.POINTER(_F64P), _F64P, C.POINTER(_F64P),
+                                       C.POINTER(_F64P)]),
+    "sw_batch_adjoint_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_size_t)]),
+    "sw_batch_adjoint_workgroups": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_size_t]),
     "sw_adjoint_run": (C.c_int, [C.c_void_p, _F64P, _F64P, _F64P, _F64P, C.POINTER(C.c_double), _F64P, _F64P,
                                  _F64P, _F64P]),
     "sw_adjoint_set_workgroups": (C.c_int, [C.c_void_p, C.c_int]),
