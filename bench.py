@@ -256,6 +256,7 @@ def adjoint_leg(wing: str, steps: int, interval: int, seed: int = 0x5EED0003) ->
     result = adj.run(target)
     gradient_s = time.perf_counter() - start
     fwd_ms, rev_ms, device_bytes = adj.timing()
+    replay_ms, jacobian_ms, sweep_ms = adj.timing_detail()
     finite = all(np.all(np.isfinite(result[k])) for k in ("grad_log_compliance", "grad_rest_length", "grad_pressure"))
     if not finite or result["loss"] != forward["loss"]:
         raise RuntimeError("adjoint run is not finite or not reproducible")
@@ -270,10 +271,87 @@ def adjoint_leg(wing: str, steps: int, interval: int, seed: int = 0x5EED0003) ->
         "forward_ms": forward_ms,
         "gradient_forward_ms": fwd_ms,
         "gradient_reverse_ms": rev_ms,
+        "gradient_replay_ms": replay_ms,
+        "gradient_jacobian_ms": jacobian_ms,
+        "gradient_sweep_ms": sweep_ms,
         "device_bytes": device_bytes,
         "workload": (f"{wing} wing: {definition.node_count} nodes, {adj.membrane_count} membranes, "
                      f"{rest.size} line segments; {steps} steps x 48 iterations, checkpoints every {interval} steps; "
                      f"{adj.membrane_count + rest.size + adj.triangle_count} parameters"),
+    }
+
+
+def batch_adjoint_leg(count: int, steps: int, interval: int, chunk: int, seed: int = 0x5EED0004) -> dict:
+    """Rest-length gradients of `count` wings (triangle counts log-uniform
+    5k-70k at the quantiles) stepped together, against the batched forward
+    pass alone. Each wing's target is a hidden batched run with its membrane
+    compliances x U(0.8, 1.2) and line rest lengths x U(0.98, 1.02)."""
+    import math
+    import simwing_amd as sw
+    settings = sw.flight_step_settings()
+    t0 = time.perf_counter()
+    cases = [make_workload(f"adjbatch:{seed}:{i}:{count}") for i in range(count)]
+    rng = np.random.default_rng(seed)
+    hidden = []
+    for d, _, _ in cases:
+        h = d.copy()
+        mat = h.arrays["membrane_material"].reshape(-1, 8)
+        for m, scale in enumerate(rng.uniform(0.8, 1.2, mat.shape[0])):
+            mat[m, :4] = mat[m, :4] / scale  # compliance x scale
+        h.suspension["segment_rest"] = np.asarray(h.suspension["segment_rest"]) * rng.uniform(0.98, 1.02, len(
+            h.suspension["segment_rest"]))
+        hidden.append(h)
+    target_batch = sw.Batch(hidden)
+    for w, (_, pressure, loads) in enumerate(cases):
+        target_batch.set_face_pressure(w, pressure)
+        target_batch.set_external_forces(w, loads)
+    target_batch.step(steps, settings)
+    targets = [target_batch.node_states(w)[0] for w in range(count)]
+    target_batch.close()
+    batch = sw.Batch([d for d, _, _ in cases])
+    for w, (_, pressure, loads) in enumerate(cases):
+        batch.set_face_pressure(w, pressure)
+        batch.set_external_forces(w, loads)
+    adj = sw.BatchAdjoint(batch, steps, interval, chunk, settings)
+    setup = time.perf_counter() - t0
+    start = time.perf_counter()
+    forward = adj.run(targets, gradients=False)
+    forward_s = time.perf_counter() - start
+    start = time.perf_counter()
+    result = adj.run(targets)
+    gradient_s = time.perf_counter() - start
+    fwd_ms, rev_ms, device_bytes = adj.timing()
+    replay_ms, jacobian_ms, sweep_ms = adj.timing_detail()
+    # Large wings released from the flat state fold violently in the first
+    # steps; where a hinge angle crosses its +-pi wrap the loss is
+    # discontinuous and the exact derivative overflows over long horizons.
+    # The run's cost does not depend on it; the count is reported.
+    finite = [bool(np.all(np.isfinite(g))) for g in result["grad_rest_length"]]
+    if not np.array_equal(forward["loss"], result["loss"]):
+        raise RuntimeError("batched adjoint forward pass is not reproducible")
+    triangles = [d.triangle_count for d, _, _ in cases]
+    workgroups = adj.workgroups()
+    adj.close()
+    batch.close()
+    return {
+        "metric": "adjoint/fwd cost (batched)",
+        "value": gradient_s / forward_s,
+        "unit": "x (gradient run / forward run, wall clock)",
+        "higher_is_better": False,
+        "forward_ms": 1e3 * forward_s,
+        "gradient_ms": 1e3 * gradient_s,
+        "device_forward_ms": fwd_ms,
+        "device_reverse_ms": rev_ms,
+        "device_replay_ms": replay_ms,
+        "device_jacobian_ms": jacobian_ms,
+        "device_sweep_ms": sweep_ms,
+        "device_bytes": device_bytes,
+        "wing_gradients_per_s": count / gradient_s,
+        "wings_with_finite_gradients": int(sum(finite)),
+        "workload": (f"{count} wings, {min(triangles)}-{max(triangles)} triangles (total {sum(triangles)}), seed "
+                     f"{seed:#x}; {steps} steps x 48 iterations, checkpoints every {interval} steps, Jacobian "
+                     f"columns {chunk} iterations at a time, {int(workgroups.sum())} sweep workgroups; line "
+                     f"rest-length gradients; setup {setup:.1f} s"),
     }
 
 
@@ -299,6 +377,7 @@ def main() -> int:
     parser.add_argument("--batch-steps", type=int, default=120)
     parser.add_argument("--no-adjoint", action="store_true")
     parser.add_argument("--adjoint-steps", type=int, default=120)
+    parser.add_argument("--adjoint-wings", type=int, default=32)
     parser.add_argument("--launch", default=None, help="fixed launch shape WGxTHREADS")
     parser.add_argument("--plain-launch", action="store_true",
                         help="plain instead of cooperative launches (for profilers that do not follow the latter)")
@@ -360,6 +439,7 @@ def main() -> int:
     structure.close()
     batched = None if args.no_batch else batched_leg(args.batch_wings, args.batch_steps)
     adjoint = None if args.no_adjoint else [adjoint_leg(w, args.adjoint_steps, 8) for w in ("small", "variant10k")]
+    batched_adjoint = None if args.no_adjoint else batch_adjoint_leg(args.adjoint_wings, 240, 16, 8)
 
     cpu = None
     if not args.no_cpu:
@@ -401,6 +481,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "batched_adjoint": batched_adjoint,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -409,6 +409,9 @@ sw_status sw_batch_adjoint_run(sw_batch_adjoint* adjoint, const double* const* r
                                double* const* final_positions);
 sw_status sw_batch_adjoint_timing(const sw_batch_adjoint* adjoint, double* forward_ms, double* reverse_ms,
                                   size_t* device_bytes);
+/* The last run's reverse pass split, as sw_adjoint_timing_detail. */
+sw_status sw_batch_adjoint_timing_detail(const sw_batch_adjoint* adjoint, double* replay_ms, double* jacobian_ms,
+                                         double* sweep_ms);
 /* Workgroups of each wing in the batched reverse sweep. */
 sw_status sw_batch_adjoint_workgroups(const sw_batch_adjoint* adjoint, int* per_wing, size_t count);
 

@@ -687,3 +687,9 @@ class BatchAdjoint:
         out = (C.c_int * self.wings)()
         check(native.library().sw_batch_adjoint_workgroups(self._handle, out, self.wings))
         return np.array(out[:], dtype=np.int64)
+
+    def timing_detail(self) -> tuple[float, float, float]:
+        """The last run's reverse pass split (ms): replays, Jacobian passes, sweeps."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(native.library().sw_batch_adjoint_timing_detail(self._handle, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value

@@ -593,6 +593,15 @@ sw_status sw_scene_generate(const char* preset, sw_scene** out) {
                 index >= count)
                 throw std::invalid_argument("batch preset must be batch:<seed>:<index>:<count>");
             params = swx::batchPresets(seed, count, 2000.0, 40000.0)[static_cast<std::size_t>(index)];
+        } else if (name.rfind("adjbatch:", 0) == 0) {
+            // The batched-adjoint wings: triangle counts log-uniform in
+            // about 5k..70k (node counts 3.3k..42.7k), at the quantiles.
+            unsigned long long seed = 0;
+            int index = 0, count = 0;
+            if (std::sscanf(name.c_str(), "adjbatch:%llu:%d:%d", &seed, &index, &count) != 3 || index < 0 ||
+                index >= count)
+                throw std::invalid_argument("adjoint batch preset must be adjbatch:<seed>:<index>:<count>");
+            params = swx::batchPresets(seed, count, 3300.0, 42700.0, true)[static_cast<std::size_t>(index)];
         } else {
             params = swx::wingPreset(name);
         }
@@ -887,6 +896,16 @@ sw_status sw_batch_adjoint_timing(const sw_batch_adjoint* a, double* forward_ms,
         if (forward_ms) *forward_ms = a->device->forwardMs();
         if (reverse_ms) *reverse_ms = a->device->reverseMs();
         if (device_bytes) *device_bytes = a->device->deviceBytes();
+    });
+}
+
+sw_status sw_batch_adjoint_timing_detail(const sw_batch_adjoint* a, double* replay_ms, double* jacobian_ms,
+                                         double* sweep_ms) {
+    return guarded([&] {
+        requirePointer(a, "adjoint");
+        if (replay_ms) *replay_ms = a->device->replayMs();
+        if (jacobian_ms) *jacobian_ms = a->device->jacobianMs();
+        if (sweep_ms) *sweep_ms = a->device->sweepMs();
     });
 }
 

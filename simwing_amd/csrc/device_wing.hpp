@@ -338,8 +338,15 @@ private:
     void *views_ = nullptr, *sweepArgs_ = nullptr, *blockWing_ = nullptr, *blockLocal_ = nullptr,
          *blockCount_ = nullptr, *traceTable_ = nullptr, *adjointTable_ = nullptr;
     std::size_t bytes_ = 0;
-    double forwardMs_ = 0.0, reverseMs_ = 0.0;
+    double forwardMs_ = 0.0, reverseMs_ = 0.0, replayMs_ = 0.0, jacobianMs_ = 0.0, sweepMs_ = 0.0;
     hipEvent_t events_[3] = {nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> marks_;
+
+public:
+    // The reverse pass split: recorded replays, Jacobian passes, sweeps.
+    double replayMs() const { return replayMs_; }
+    double jacobianMs() const { return jacobianMs_; }
+    double sweepMs() const { return sweepMs_; }
 };
 
 // Device math conformance probe (function codes as sw_device_math).

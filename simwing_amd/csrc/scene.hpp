@@ -238,10 +238,11 @@ namespace swx {
 [[nodiscard]] WingParams wingPreset(const std::string& name);
 
 // Per-wing parameters of an uneven batch: node counts log-uniform in
-// [minNodes, maxNodes], span U(6, 12) m, pressure U(20, 100) Pa and line
-// rest-length scale U(0.97, 1.03), all drawn from `seed`.
-[[nodiscard]] std::vector<WingParams> batchPresets(std::uint64_t seed, int count,
-                                                   double minNodes, double maxNodes);
+// [minNodes, maxNodes] (drawn, or with `quantiles` the distribution's
+// (w + 1/2)/count quantiles), span U(6, 12) m, pressure U(20, 100) Pa and
+// line rest-length scale U(0.97, 1.03), all drawn from `seed`.
+[[nodiscard]] std::vector<WingParams> batchPresets(std::uint64_t seed, int count, double minNodes,
+                                                   double maxNodes, bool quantiles = false);
 
 // Applies "key=value&key=value" overrides (WingParams field names) to a
 // preset; throws std::invalid_argument on an unknown key or bad value.

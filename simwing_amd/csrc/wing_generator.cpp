@@ -563,7 +563,7 @@ void applyOverrides(WingParams& p, const std::string& overrides) {
 }
 
 std::vector<WingParams> batchPresets(std::uint64_t seed, int count, double minNodes,
-                                     double maxNodes) {
+                                     double maxNodes, bool quantiles) {
     if (count < 1 || !(minNodes > 0.0) || !(maxNodes >= minNodes)) {
         throw std::invalid_argument("invalid batch preset request");
     }
@@ -585,7 +585,10 @@ std::vector<WingParams> batchPresets(std::uint64_t seed, int count, double minNo
     for (int w = 0; w < count; ++w) {
         WingParams p;
         p.seed = seed + 0x1000ULL * static_cast<std::uint64_t>(w + 1);
-        const double nodes = minNodes * seriesExp(rng.uniform() * logRatio);
+        // Node counts log-uniform: random draws, or the distribution's
+        // (w + 1/2)/count quantiles (the sum then sits at its expectation).
+        const double u = rng.uniform();
+        const double nodes = minNodes * seriesExp((quantiles ? (w + 0.5) / count : u) * logRatio);
         // nodes ~ 3 * span * chord for a 2:1 grid with ribs every 2 stations.
         int chord = static_cast<int>(std::sqrt(nodes / 6.0) + 0.5);
         if (chord < 16) chord = 16;

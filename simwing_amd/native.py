@@ -236,6 +236,8 @@ _SIGNATURES = {
     "sw_batch_adjoint_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                           C.POINTER(C.c_size_t)]),
     "sw_batch_adjoint_workgroups": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_size_t]),
+    "sw_batch_adjoint_timing_detail": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                 C.POINTER(C.c_double)]),
     "sw_adjoint_run": (C.c_int, [C.c_void_p, _F64P, _F64P, _F64P, _F64P, C.POINTER(C.c_double), _F64P, _F64P,
                                  _F64P, _F64P]),
     "sw_adjoint_set_workgroups": (C.c_int, [C.c_void_p, C.c_int]),
