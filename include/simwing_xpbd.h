@@ -216,6 +216,77 @@ sw_status sw_structure_node_states(const sw_structure* s, double* position, doub
  * linear velocity[3], angular velocity[3] = 13 doubles. */
 sw_status sw_structure_payload_state(const sw_structure* s, double* payload13);
 
+/* Diagnostic-frame fields of the committed state: the per-vertex and per-line
+ * data buildStructureFrame reads (structure_frame.cpp:491-640), computed on
+ * the device. Frame vertices are the nodes, then the pilot harness points in
+ * definition order (positions and velocities of the rigid payload points,
+ * suspension.cpp:1018-1032; pending force 0); lines are the constraints, then
+ * the suspension segments, in definition order: current length
+ * |second - first| and violation (|L - rest| for distance, tie and stitch,
+ * max(0, L - rest) for cables and segments, structure_frame.cpp:60-73,
+ * 577-598). Arrays: 3 (N + H), 3 (N + H), 3 (N + H), C + S, C + S doubles;
+ * any may be NULL. The global fields come from sw_structure_diagnostics and
+ * sw_structure_progress. */
+sw_status sw_structure_frame_fields(const sw_structure* s, double* vertex_position, double* vertex_velocity,
+                                    double* vertex_pending_force, double* line_length, double* line_violation);
+
+/* Stable identity of a frame: StructureFrameMappingDefinition
+ * (structure_frame.h:35-39). vertex_count = N + H, triangle_count = T,
+ * line_count = C + S. Ids nonzero and unique per kind, side regions
+ * nonzero; a segment line's id is the segment's stable id. The line
+ * endpoints are not passed: constraints use their nodes, segments the
+ * endpoint rule of makeStructureFrameMapping (structure_frame.cpp:427-474;
+ * harness point k is vertex N + k). */
+typedef struct sw_frame_mapping {
+    size_t vertex_count;
+    const uint64_t* vertex_ids;
+    size_t triangle_count;
+    const uint64_t* triangle_ids;
+    const uint64_t* triangle_negative_region;
+    const uint64_t* triangle_positive_region;
+    size_t line_count;
+    const uint64_t* line_ids;
+    const uint32_t* line_roles;
+} sw_frame_mapping;
+
+/* StructureFrameContext (structure_frame.h:86-93) with CouplingResiduals and
+ * ConservationValues (viewer_protocol.h:127-145); strings NUL-terminated,
+ * nonempty, at most 1024 bytes. */
+typedef struct sw_frame_context {
+    const char* scene_checksum;
+    const char* solver_commit;
+    double time_step;
+    uint32_t coupling_iteration;
+    double coupling_residuals[5]; /* displacement, traction, fluid, structure, interface power */
+    double fluid_mass;
+    double total_momentum[3];
+    double total_energy;
+    double interface_force_residual[3];
+    double interface_moment_residual[3];
+    double interface_power_residual;
+} sw_frame_context;
+
+/* buildStructureFrame (structure_frame.cpp:480-666) followed by
+ * serializeFrame (viewer_protocol.cpp:838-882): the frame of the committed
+ * state as SWVF bytes. Field arrays come from the device
+ * (sw_structure_frame_fields), the global fields from
+ * sw_structure_diagnostics and the step and time from
+ * sw_structure_progress. bytes NULL: only *size is set. A capacity below
+ * *size fails with SW_INVALID_ARGUMENT and *size set. Rejections follow
+ * the reference (mapping checks, validateFrame). */
+sw_status sw_structure_frame(sw_structure* s, const sw_frame_mapping* mapping, const sw_frame_context* context,
+                             uint8_t* bytes, size_t capacity, size_t* size);
+
+/* The same encoding from caller-held fields (host only; no device is used):
+ * vertex arrays 3 (N + H) doubles, line arrays C + S doubles, the global
+ * fields from *diagnostics. */
+sw_status sw_frame_encode(const sw_definition* definition, const sw_frame_mapping* mapping,
+                          const sw_frame_context* context, uint64_t step, double simulation_time,
+                          const double* vertex_position, const double* vertex_velocity,
+                          const double* vertex_pending_force, const double* line_length,
+                          const double* line_violation, const sw_diagnostics* diagnostics, uint8_t* bytes,
+                          size_t capacity, size_t* size);
+
 /* Solver multipliers and residuals, in definition order (constraints,
  * membranes, dihedrals) and runtime segment order (string-id order of
  * "segment:<id>", suspension.cpp:996-1005). Any pointer may be NULL.

@@ -34,6 +34,7 @@ def library(extended=False):
         lib.oracle_set_external_forces.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         lib.oracle_step.argtypes = [C.c_void_p, C.c_void_p]
         lib.oracle_state.argtypes = [C.c_void_p] + [C.c_void_p] * 11
+        lib.oracle_frame_fields.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         lib.oracle_colouring.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_size_t), C.c_void_p,
                                          C.POINTER(C.c_size_t)]
         lib.oracle_set_angle_perturbation.argtypes = [C.c_void_p, C.c_int]
@@ -154,6 +155,20 @@ class OracleBody:
         _check(self._lib.oracle_state(self._handle, *[_p(out[k]) for k in (
             "position", "previous", "velocity", "constraint_lambda", "membrane_lambda", "membrane_resultant",
             "membrane_residual", "dihedral_lambda", "segment_lambda", "payload", "suspension_residual")]), self._lib)
+        return out
+
+    def frame_fields(self) -> dict:
+        """Diagnostic-frame fields (oracle_frame_fields): vertices are the
+        nodes then the harness points, lines the constraints then the
+        segments, all in definition order."""
+        hcount = 0 if self._definition.suspension is None else self._definition.suspension["harness_ids"].size
+        v = self.n + hcount
+        out = {"vertex_position": np.zeros(3 * v), "vertex_velocity": np.zeros(3 * v),
+               "vertex_pending_force": np.zeros(3 * v), "line_length": np.zeros(self.c + self.s),
+               "line_violation": np.zeros(self.c + self.s)}
+        _check(self._lib.oracle_frame_fields(self._handle, *[_p(out[k]) for k in (
+            "vertex_position", "vertex_velocity", "vertex_pending_force", "line_length", "line_violation")]),
+            self._lib)
         return out
 
     def colouring(self, family):

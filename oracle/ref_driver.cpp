@@ -22,8 +22,18 @@
 //      iterations, 3 cable pairs, damping 8/s, gravity -9.80665;
 //   5. the full solver state is written at the requested steps.
 //
+// The --frame mode drives the reference's own simwing::fsi::Structure
+// (structure.cpp) instead, on the same definition with its dihedrals listed
+// in the colour-major order (so Structure's serial hinge sweep is again the
+// coloured sweep) and without face pressure (Structure has no pressure
+// input), applies the flight loads before every step and once more after
+// the last, and writes what simwing::viewer::buildStructureFrame
+// (structure_frame.cpp:480-666) makes of that committed state, with the
+// frame's serialized bytes (viewer_protocol.h:167-171).
+//
 // Usage: ref_driver <preset> <out-prefix> <steps> <workers> [snapshot ...]
 //        ref_driver --compare-workers <preset> <steps>
+//        ref_driver --frame <preset> <out-file> <steps>
 #include "fixture_io.hpp"
 #include "oracle_colouring.hpp"
 
@@ -31,6 +41,8 @@
 
 #include "scene_structure.h"
 #include "structure.h"
+#include "structure_frame.h"
+#include "viewer_protocol.h"
 #include <softwing/soft_body.h>
 #include <softwing/suspension.h>
 
@@ -505,6 +517,157 @@ Prepared prepare(const std::string& preset) {
     return p;
 }
 
+namespace viewer = simwing::viewer;
+
+// The flight loads of buildRig, as Structure forces.
+std::vector<fsi::StructureVector3> structureLoads(const fsi::StructureDefinition& d) {
+    double total = 0.0;
+    const std::size_t vertexNodes = d.nodes.size() - (d.suspension ? d.suspension->junctions.size() : 0);
+    std::vector<double> pos, mass;
+    for (const auto& n : d.nodes) {
+        total += n.massKg;
+        pos.insert(pos.end(), {n.positionMeters.x, n.positionMeters.y, n.positionMeters.z});
+        mass.push_back(n.massKg);
+    }
+    swx::V3 centre{};
+    if (d.suspension) {
+        total += d.suspension->pilotMassKg;
+        centre = {d.suspension->pilotInitialCenterOfMassWorldMeters.x,
+                  d.suspension->pilotInitialCenterOfMassWorldMeters.y,
+                  d.suspension->pilotInitialCenterOfMassWorldMeters.z};
+    }
+    std::vector<double> loads(3 * d.nodes.size());
+    swx::flightLoads(pos.data(), mass.data(), d.nodes.size(), vertexNodes, total * 9.80665, centre, loads.data());
+    std::vector<fsi::StructureVector3> out(d.nodes.size());
+    for (std::size_t i = 0; i < d.nodes.size(); ++i) out[i] = {loads[3 * i], loads[3 * i + 1], loads[3 * i + 2]};
+    return out;
+}
+
+// A one-to-one frame mapping with synthetic stable ids: vertices 1..N+H,
+// triangles 1..T with side regions 1 and 2, constraint lines after the
+// largest segment id, segments under their own ids with the endpoint
+// vertices makeStructureFrameMapping derives (structure_frame.cpp:427-474).
+viewer::StructureFrameMapping frameMapping(const fsi::Structure& structure) {
+    const fsi::StructureDefinition& d = structure.definition();
+    viewer::StructureFrameMappingDefinition m;
+    const std::size_t harness = d.suspension ? d.suspension->harnessPoints.size() : 0;
+    for (std::size_t i = 0; i < d.nodes.size() + harness; ++i) m.vertexStableIds.push_back(i + 1);
+    for (std::size_t t = 0; t < d.triangles.size(); ++t) m.triangles.push_back({t + 1, 1, 2});
+    std::uint64_t base = 1;
+    if (d.suspension)
+        for (const auto& g : d.suspension->segments) base = std::max(base, g.stableId + 1);
+    for (std::size_t k = 0; k < d.constraints.size(); ++k)
+        m.lines.push_back({base + k, static_cast<std::uint32_t>(d.constraints[k].kind)});
+    if (d.suspension) {
+        const auto& s = *d.suspension;
+        const auto vertex = [&](const fsi::StructureSuspensionEndpointDefinition& e) -> std::uint32_t {
+            switch (e.kind) {
+            case fsi::StructureSuspensionEndpointKind::SurfaceAttachment:
+                for (const auto& a : s.attachments)
+                    if (a.stableId == e.stableId) return static_cast<std::uint32_t>(a.node);
+                break;
+            case fsi::StructureSuspensionEndpointKind::Junction:
+                for (const auto& j : s.junctions)
+                    if (j.stableId == e.stableId) return static_cast<std::uint32_t>(j.node);
+                break;
+            case fsi::StructureSuspensionEndpointKind::PilotHarness:
+                for (std::size_t h = 0; h < s.harnessPoints.size(); ++h)
+                    if (s.harnessPoints[h].stableId == e.stableId)
+                        return static_cast<std::uint32_t>(d.nodes.size() + h);
+                break;
+            }
+            throw std::runtime_error("unresolved suspension endpoint");
+        };
+        for (const auto& g : s.segments)
+            m.lines.push_back({g.stableId, g.role, vertex(g.from), vertex(g.to)});
+    }
+    return viewer::StructureFrameMapping(structure, std::move(m));
+}
+
+int frameMode(const std::string& preset, const std::string& out, int steps) {
+    Prepared p = prepare(preset);
+    fsi::StructureDefinition def = p.assembly.definition;
+    std::vector<fsi::StructureDihedralDefinition> hinges;
+    for (const std::size_t index : p.hinges.order) hinges.push_back(def.dihedrals[index]);
+    def.dihedrals = std::move(hinges);
+    const std::vector<fsi::StructureVector3> loads = structureLoads(def);
+    fsi::Structure structure(def);
+    fsi::StructureStepSettings settings;
+    settings.timeStepSeconds = 1.0 / 120.0;
+    settings.substeps = 1;
+    settings.constraintIterations = 48;
+    settings.cableConstraintSweepPairs = 3;
+    settings.gravityMetersPerSecondSquared = {0.0, 0.0, -9.80665};
+    settings.velocityDampingPerSecond = 8.0;
+    settings.workerThreads = 1;  // softwing's coloured sweep; 0 selects the serial one
+    for (int s = 0; s < steps; ++s) {
+        structure.setExternalForces(loads);
+        static_cast<void>(structure.step(settings));
+    }
+    structure.setExternalForces(loads);
+    const viewer::StructureFrameMapping mapping = frameMapping(structure);
+    viewer::StructureFrameContext context;
+    context.sceneChecksum = "synthetic-wing";
+    context.solverCommit = "reference";
+    context.timeStepSeconds = settings.timeStepSeconds;
+    const viewer::DiagnosticFrame frame = viewer::buildStructureFrame(structure, mapping, context);
+
+    swfx::Writer w(out);
+    w.u64("steps", {static_cast<std::uint64_t>(steps)});
+    const fsi::StructureCheckpoint cp = structure.checkpoint();
+    std::vector<double> x, v, f;
+    for (const auto& n : cp.nodes) {
+        x.insert(x.end(), {n.positionMeters.x, n.positionMeters.y, n.positionMeters.z});
+        v.insert(v.end(), {n.velocityMetersPerSecond.x, n.velocityMetersPerSecond.y, n.velocityMetersPerSecond.z});
+    }
+    for (const auto& g : cp.pendingExternalForcesNewtons) f.insert(f.end(), {g.x, g.y, g.z});
+    w.f64("position", x);
+    w.f64("velocity", v);
+    w.f64("pending_force", f);
+    if (const auto ss = structure.suspensionState()) {
+        const auto& q = ss->payload;
+        w.f64("payload", {q.centerOfMassWorldMeters.x, q.centerOfMassWorldMeters.y, q.centerOfMassWorldMeters.z,
+                          q.bodyToWorld.w, q.bodyToWorld.x, q.bodyToWorld.y, q.bodyToWorld.z,
+                          q.linearVelocityMetersPerSecond.x, q.linearVelocityMetersPerSecond.y,
+                          q.linearVelocityMetersPerSecond.z, q.angularVelocityRadiansPerSecond.x,
+                          q.angularVelocityRadiansPerSecond.y, q.angularVelocityRadiansPerSecond.z});
+        std::vector<double> commanded;
+        for (const auto& g : ss->segments) commanded.push_back(g.commandedRestLengthMeters);
+        w.f64("segment_commanded_rest", commanded);
+    }
+    std::vector<double> vp;
+    for (const auto& vert : frame.vertices) vp.insert(vp.end(), {vert.positionMetres.x, vert.positionMetres.y, vert.positionMetres.z});
+    w.f64("vertex_position", vp);
+    std::vector<std::uint64_t> lineVertex;
+    for (const auto& l : frame.lines) lineVertex.insert(lineVertex.end(), {l.vertex0, l.vertex1});
+    w.u64("line_vertex", lineVertex);
+    std::vector<double> globals, globalVectors;
+    std::vector<std::uint8_t> names;
+    for (const auto& field : frame.scalarFields) {
+        if (field.association == viewer::FieldAssociation::Global) globals.push_back(field.values.at(0));
+        else if (field.name == "structure.constraint_length") w.f64("line_length", field.values);
+        else if (field.name == "structure.constraint_violation") w.f64("line_violation", field.values);
+    }
+    for (const auto& field : frame.vectorFields) {
+        std::vector<double> flat;
+        for (const auto& e : field.values) flat.insert(flat.end(), {e.x, e.y, e.z});
+        if (field.association == viewer::FieldAssociation::Global) globalVectors.insert(globalVectors.end(), flat.begin(), flat.end());
+        else if (field.name == "structure.velocity") w.f64("vertex_velocity", flat);
+        else if (field.name == "structure.pending_external_force") w.f64("vertex_pending_force", flat);
+    }
+    w.f64("global_scalars", globals);        // mass, KE, max distance error, max cable extension,
+                                             // max |membrane strain|, max membrane residual,
+                                             // max contact penetration, max suspension residual
+    w.f64("global_vectors", globalVectors);  // total pending, last applied, centre of mass, momentum
+    std::vector<std::uint8_t> bytes;
+    viewer::ProtocolError error;
+    if (!viewer::serializeFrame(frame, bytes, &error)) throw std::runtime_error("serializeFrame: " + error.message);
+    w.u8("frame_bytes", bytes);
+    std::printf("frame preset=%s steps=%d vertices=%zu lines=%zu bytes=%zu\n", preset.c_str(), steps,
+                frame.vertices.size(), frame.lines.size(), bytes.size());
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -524,6 +687,8 @@ int main(int argc, char** argv) {
                         static_cast<unsigned long long>(hb), ha == hb ? "IDENTICAL" : "DIFFERENT");
             return ha == hb ? 0 : 1;
         }
+        if (argc >= 5 && std::strcmp(argv[1], "--frame") == 0)
+            return frameMode(argv[2], argv[3], std::atoi(argv[4]));
         if (argc >= 5 && std::strcmp(argv[1], "--bench") == 0) {
             // Timing in the softwing-bench manner (tools/softwing_bench.cpp:1783-1801):
             // warm-up steps, then steady_clock around each step until the budget is spent.

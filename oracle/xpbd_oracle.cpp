@@ -425,6 +425,7 @@ struct Payload {
     Real mass = 0.0;
     M3 inertia;
     std::vector<V> hang;
+    std::vector<std::size_t> hangDefinition;  // string-id slot -> definition index
     Real minLength = 1e-10, maxResidual = 2e-4, certified = 0.0;
     int iterations = 12;
 };
@@ -1019,6 +1020,7 @@ int oracle_create(const sw_definition* d, oracle_body** out) {
                 endpoint[hangs[r].first] = -static_cast<long>(r) - 1;
                 const double* l = &s.harness_local[3 * hangs[r].second];
                 p.hang.push_back({l[0], l[1], l[2]});
+                p.hangDefinition.push_back(hangs[r].second);
             }
             for (std::size_t k = 0; k < s.attachment_count; ++k)
                 endpoint["attachment:" + std::to_string(s.attachment_ids[k])] = static_cast<long>(s.attachment_nodes[k]);
@@ -1198,6 +1200,62 @@ int oracle_state(const oracle_body* b, double* position, double* previous, doubl
             for (int k = 0; k < 13; ++k) payload[k] = value(v[k]);
         }
         if (suspensionResidual) *suspensionResidual = value(b->payload.certified);
+    });
+}
+
+// Diagnostic-frame fields of the current state, restating
+// simwing::viewer::buildStructureFrame (structure_frame.cpp:526-612) over
+// Structure::suspensionState (structure.cpp:718-776): vertices are the nodes
+// then the harness points in definition order, at payloadPointPosition /
+// payloadPointVelocity (suspension.cpp:1018-1032); lines are the constraints
+// (length |second - first|, violation per constraintViolation,
+// structure_frame.cpp:60-73) then the segments in definition order (length
+// between the frame vertices of its ends, violation max(0, L - commanded
+// rest), the commanded rest being the definition's rest length). The pending
+// force of a node is its current external load, of a harness point zero.
+int oracle_frame_fields(const oracle_body* b, double* vertexPosition, double* vertexVelocity, double* pendingForce,
+                        double* lineLength, double* lineViolation) {
+    return guard([&] {
+        const std::size_t n = b->nodes.size();
+        const Payload& p = b->payload;
+        const std::size_t h = b->hasSuspension ? p.hang.size() : 0;
+        const auto put = [](double* out, std::size_t i, const V& v) {
+            if (!out) return;
+            out[3 * i] = value(v.x);
+            out[3 * i + 1] = value(v.y);
+            out[3 * i + 2] = value(v.z);
+        };
+        std::vector<V> vertex(n + h);
+        for (std::size_t i = 0; i < n; ++i) {
+            vertex[i] = b->nodes[i].position;
+            put(vertexVelocity, i, b->nodes[i].velocity);
+            put(pendingForce, i, b->external[i]);
+        }
+        std::vector<std::size_t> slotVertex(h);
+        for (std::size_t r = 0; r < h; ++r) {
+            const std::size_t i = n + p.hangDefinition[r];
+            slotVertex[r] = i;
+            const V offset = rotation(p.q) * (p.hang[r] - p.comLocal);
+            vertex[i] = p.com + offset;
+            put(vertexVelocity, i, p.v + cross(p.w, offset));
+            put(pendingForce, i, V{});
+        }
+        for (std::size_t i = 0; i < n + h; ++i) put(vertexPosition, i, vertex[i]);
+        const std::size_t c = b->constraints.size();
+        for (std::size_t k = 0; k < c; ++k) {
+            const Constraint& q = b->constraints[k];
+            const Real l = length(b->nodes[q.b].position - b->nodes[q.a].position);
+            const Real extension = l - q.rest;
+            if (lineLength) lineLength[k] = value(l);
+            if (lineViolation)
+                lineViolation[k] = value(q.kind == SW_CONSTRAINT_CABLE ? std::max<Real>(0.0, extension) : abs(extension));
+        }
+        for (const Segment& g : b->segments) {
+            const auto end = [&](long e) { return e >= 0 ? vertex[e] : vertex[slotVertex[-e - 1]]; };
+            const Real l = length(end(g.to) - end(g.from));
+            if (lineLength) lineLength[c + g.definition] = value(l);
+            if (lineViolation) lineViolation[c + g.definition] = value(std::max<Real>(0.0, l - g.rest));
+        }
     });
 }
 

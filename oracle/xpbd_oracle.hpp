@@ -24,6 +24,8 @@ int oracle_set_external_forces(oracle_body* body, const double* forces, size_t n
 int oracle_step(oracle_body* body, const sw_step_settings* settings);
 /* State in definition order; segment lambdas in string-id order; any
  * pointer may be NULL. payload: com, q(w,x,y,z), v, w = 13 doubles. */
+int oracle_frame_fields(const oracle_body* body, double* vertex_position, double* vertex_velocity,
+                        double* pending_force, double* line_length, double* line_violation);
 int oracle_state(const oracle_body* body, double* position, double* previous, double* velocity,
                  double* constraint_lambda, double* membrane_lambda, double* membrane_resultant,
                  double* membrane_residual, double* dihedral_lambda, double* segment_lambda, double* payload,

@@ -317,6 +317,105 @@ def _diagnostics_dict(d: native.Diagnostics) -> dict:
     return out
 
 
+def _diagnostics_struct(values: dict) -> native.Diagnostics:
+    d = native.Diagnostics()
+    for name, _ in native.Diagnostics._fields_:
+        if name in values:
+            value = values[name]
+            if hasattr(getattr(d, name), "__len__"):
+                getattr(d, name)[:] = [float(x) for x in value]
+            else:
+                setattr(d, name, value)
+    return d
+
+
+class FrameMapping:
+    """Stable identity of a structure's diagnostic frames
+    (StructureFrameMappingDefinition, structure_frame.h:35-39): one id per
+    frame vertex (the N nodes, then the H harness points), per triangle (with
+    its two side regions) and per line (the C constraints, then the S
+    segments, a segment line carrying its segment's stable id) plus a role
+    per line. Line endpoints are derived from the definition."""
+
+    def __init__(self, vertex_ids, triangle_ids, triangle_negative_region, triangle_positive_region, line_ids,
+                 line_roles):
+        self.vertex_ids = np.ascontiguousarray(vertex_ids, np.uint64)
+        self.triangle_ids = np.ascontiguousarray(triangle_ids, np.uint64)
+        self.triangle_negative_region = np.ascontiguousarray(triangle_negative_region, np.uint64)
+        self.triangle_positive_region = np.ascontiguousarray(triangle_positive_region, np.uint64)
+        self.line_ids = np.ascontiguousarray(line_ids, np.uint64)
+        self.line_roles = np.ascontiguousarray(line_roles, np.uint32)
+
+    @classmethod
+    def sequential(cls, definition: "DefinitionArrays") -> "FrameMapping":
+        """Vertices 1..N+H, triangles 1..T between side regions 1 and 2,
+        segment lines under their stable ids with their roles, constraint
+        lines numbered on from the largest segment id with their kind as
+        role."""
+        a, sus = definition.arrays, definition.suspension
+        h = 0 if sus is None else sus["harness_ids"].size
+        t = a["triangle_nodes"].size // 3
+        c = a["constraint_kind"].size
+        seg_ids = np.zeros(0, np.uint64) if sus is None else sus["segment_ids"].astype(np.uint64)
+        seg_roles = np.zeros(0, np.uint32) if sus is None else sus["segment_role"].astype(np.uint32)
+        base = int(seg_ids.max()) + 1 if seg_ids.size else 1
+        return cls(np.arange(1, definition.node_count + h + 1, dtype=np.uint64),
+                   np.arange(1, t + 1, dtype=np.uint64), np.ones(t, np.uint64), np.full(t, 2, np.uint64),
+                   np.concatenate([np.arange(base, base + c, dtype=np.uint64), seg_ids]),
+                   np.concatenate([a["constraint_kind"].astype(np.uint32), seg_roles]))
+
+    def _ctypes(self) -> native.FrameMapping:
+        m = native.FrameMapping()
+        m.vertex_count = self.vertex_ids.size
+        m.vertex_ids = u64p(self.vertex_ids)
+        m.triangle_count = self.triangle_ids.size
+        m.triangle_ids = u64p(self.triangle_ids)
+        m.triangle_negative_region = u64p(self.triangle_negative_region)
+        m.triangle_positive_region = u64p(self.triangle_positive_region)
+        m.line_count = self.line_ids.size
+        m.line_ids = u64p(self.line_ids)
+        m.line_roles = u32p(self.line_roles)
+        return m
+
+
+def frame_context(scene_checksum: str, solver_commit: str, time_step: float, coupling_iteration: int = 0,
+                  coupling_residuals=(0.0,) * 5, fluid_mass: float = 0.0, total_momentum=(0.0,) * 3,
+                  total_energy: float = 0.0, interface_force_residual=(0.0,) * 3,
+                  interface_moment_residual=(0.0,) * 3, interface_power_residual: float = 0.0) -> native.FrameContext:
+    """StructureFrameContext (structure_frame.h:86-93)."""
+    c = native.FrameContext()
+    c.scene_checksum = scene_checksum.encode()
+    c.solver_commit = solver_commit.encode()
+    c.time_step = time_step
+    c.coupling_iteration = coupling_iteration
+    c.coupling_residuals[:] = [float(x) for x in coupling_residuals]
+    c.fluid_mass = fluid_mass
+    c.total_momentum[:] = [float(x) for x in total_momentum]
+    c.total_energy = total_energy
+    c.interface_force_residual[:] = [float(x) for x in interface_force_residual]
+    c.interface_moment_residual[:] = [float(x) for x in interface_moment_residual]
+    c.interface_power_residual = interface_power_residual
+    return c
+
+
+def frame_encode(definition: "DefinitionArrays", mapping: FrameMapping, context: native.FrameContext, step: int,
+                 simulation_time: float, fields: dict, diagnostics: dict) -> bytes:
+    """SWVF bytes of the frame of caller-held fields (sw_frame_encode)."""
+    lib = native.library()
+    view = definition.to_ctypes()
+    m = mapping._ctypes()
+    arrays = [np.ascontiguousarray(fields[k], np.float64).reshape(-1) for k in (
+        "vertex_position", "vertex_velocity", "vertex_pending_force", "line_length", "line_violation")]
+    d = _diagnostics_struct(diagnostics)
+    size = C.c_size_t()
+    check(lib.sw_frame_encode(C.byref(view), C.byref(m), C.byref(context), step, simulation_time,
+                              *[f64p(x) for x in arrays], C.byref(d), None, 0, C.byref(size)))
+    out = np.zeros(size.value, np.uint8)
+    check(lib.sw_frame_encode(C.byref(view), C.byref(m), C.byref(context), step, simulation_time,
+                              *[f64p(x) for x in arrays], C.byref(d), u8p(out), out.size, C.byref(size)))
+    return out.tobytes()
+
+
 class Structure:
     """simwing::fsi::Structure over the device (structure.h:273-309)."""
 
@@ -357,6 +456,37 @@ class Structure:
     @property
     def definition(self) -> DefinitionArrays:
         return self._definition
+
+    def frame_fields(self) -> dict:
+        """Diagnostic-frame fields (sw_structure_frame_fields): vertices are
+        the nodes then the harness points, lines the constraints then the
+        suspension segments (definition order)."""
+        d = self._definition
+        h = 0 if d.suspension is None else int(np.asarray(d.suspension["harness_ids"]).size)
+        lines = int(d.arrays["constraint_kind"].size) + (
+            0 if d.suspension is None else int(np.asarray(d.suspension["segment_rest"]).size))
+        v = self.node_count + h
+        out = {"vertex_position": np.zeros(3 * v), "vertex_velocity": np.zeros(3 * v),
+               "vertex_pending_force": np.zeros(3 * v), "line_length": np.zeros(lines),
+               "line_violation": np.zeros(lines)}
+        check(native.library().sw_structure_frame_fields(self._handle, *[f64p(out[k]) for k in (
+            "vertex_position", "vertex_velocity", "vertex_pending_force", "line_length", "line_violation")]))
+        for k in ("vertex_position", "vertex_velocity", "vertex_pending_force"):
+            out[k] = out[k].reshape(-1, 3)
+        return out
+
+    def frame(self, mapping: FrameMapping | None = None, context: native.FrameContext | None = None) -> bytes:
+        """The diagnostic frame of the committed state as SWVF bytes
+        (sw_structure_frame: buildStructureFrame, then serializeFrame)."""
+        lib = native.library()
+        mapping = mapping or FrameMapping.sequential(self._definition)  # owns the arrays m points into
+        m = mapping._ctypes()
+        c = context or frame_context("scene", "simwing-amd", 0.0)
+        size = C.c_size_t()
+        check(lib.sw_structure_frame(self._handle, C.byref(m), C.byref(c), None, 0, C.byref(size)))
+        out = np.zeros(size.value, np.uint8)
+        check(lib.sw_structure_frame(self._handle, C.byref(m), C.byref(c), u8p(out), out.size, C.byref(size)))
+        return out.tobytes()
 
     def definition_fingerprint(self) -> int:
         value = C.c_uint64()

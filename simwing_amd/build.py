@@ -49,6 +49,7 @@ SOURCES = [
     ("definition.cpp", False),
     ("assembly.cpp", False),
     ("schedule.cpp", False),
+    ("frame.cpp", False),
     ("wing_generator.cpp", False),
 ]
 

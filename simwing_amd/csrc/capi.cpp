@@ -5,6 +5,7 @@
 
 #include "assembly.hpp"
 #include "definition.hpp"
+#include "frame.hpp"
 #include "device_wing.hpp"
 #include "scene.hpp"
 
@@ -386,6 +387,99 @@ sw_status sw_structure_node_states(const sw_structure* s, double* position, doub
         if (position) std::copy(st.position.begin(), st.position.end(), position);
         if (previous) std::copy(st.previous.begin(), st.previous.end(), previous);
         if (velocity) std::copy(st.velocity.begin(), st.velocity.end(), velocity);
+    });
+}
+
+sw_status sw_structure_frame_fields(const sw_structure* s, double* vertex_position, double* vertex_velocity,
+                                    double* vertex_pending_force, double* line_length, double* line_violation) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        const swx::DeviceWing::State st = s->wing->download();
+        const swx::DeviceWing::Frame f = s->wing->frame();
+        const std::size_t n = s->definition.nodeCount();
+        const std::size_t h = f.harnessPosition.size() / 3;
+        if (vertex_position) {
+            std::copy(st.position.begin(), st.position.end(), vertex_position);
+            std::copy(f.harnessPosition.begin(), f.harnessPosition.end(), vertex_position + 3 * n);
+        }
+        if (vertex_velocity) {
+            std::copy(st.velocity.begin(), st.velocity.end(), vertex_velocity);
+            std::copy(f.harnessVelocity.begin(), f.harnessVelocity.end(), vertex_velocity + 3 * n);
+        }
+        if (vertex_pending_force) {
+            std::copy(s->pending.begin(), s->pending.end(), vertex_pending_force);
+            std::fill(vertex_pending_force + 3 * n, vertex_pending_force + 3 * (n + h), 0.0);
+        }
+        if (line_length) std::copy(f.lineLength.begin(), f.lineLength.end(), line_length);
+        if (line_violation) std::copy(f.lineViolation.begin(), f.lineViolation.end(), line_violation);
+    });
+}
+
+namespace {
+
+void emitFrame(const std::vector<std::uint8_t>& frame, std::uint8_t* bytes, std::size_t capacity, std::size_t* size) {
+    *size = frame.size();
+    if (capacity < frame.size()) throw std::invalid_argument("frame buffer is smaller than the frame");
+    std::copy(frame.begin(), frame.end(), bytes);
+}
+
+}  // namespace
+
+sw_status sw_structure_frame(sw_structure* s, const sw_frame_mapping* mapping, const sw_frame_context* context,
+                             uint8_t* bytes, size_t capacity, size_t* size) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(mapping, "mapping");
+        requirePointer(context, "context");
+        requirePointer(size, "size");
+        swx::frameLineVertices(s->definition, *mapping);
+        *size = swx::frameBytes(s->definition, *mapping, *context);
+        if (!bytes) return;
+        if (capacity < *size) throw std::invalid_argument("frame buffer is smaller than the frame");
+        const std::size_t v = mapping->vertex_count, lines = mapping->line_count;
+        std::vector<double> position(3 * v), velocity(3 * v), pending(3 * v), length(lines), violation(lines);
+        check(sw_structure_frame_fields(s, position.data(), velocity.data(), pending.data(), length.data(),
+                                        violation.data()));
+        sw_diagnostics diagnostics;
+        check(sw_structure_diagnostics(s, &diagnostics));
+        swx::FrameFields f;
+        f.step = s->acceptedSteps;
+        f.simulationTime = s->time;
+        f.vertexPosition = position.data();
+        f.vertexVelocity = velocity.data();
+        f.vertexPendingForce = pending.data();
+        f.lineLength = length.data();
+        f.lineViolation = violation.data();
+        f.diagnostics = &diagnostics;
+        emitFrame(swx::encodeFrame(s->definition, *mapping, *context, f), bytes, capacity, size);
+    });
+}
+
+sw_status sw_frame_encode(const sw_definition* definition, const sw_frame_mapping* mapping,
+                          const sw_frame_context* context, uint64_t step, double simulation_time,
+                          const double* vertex_position, const double* vertex_velocity,
+                          const double* vertex_pending_force, const double* line_length,
+                          const double* line_violation, const sw_diagnostics* diagnostics, uint8_t* bytes,
+                          size_t capacity, size_t* size) {
+    return guarded([&] {
+        requirePointer(definition, "definition");
+        requirePointer(mapping, "mapping");
+        requirePointer(context, "context");
+        requirePointer(size, "size");
+        const swx::Definition d = swx::Definition::fromView(*definition);
+        swx::frameLineVertices(d, *mapping);
+        *size = swx::frameBytes(d, *mapping, *context);
+        if (!bytes) return;
+        swx::FrameFields f;
+        f.step = step;
+        f.simulationTime = simulation_time;
+        f.vertexPosition = vertex_position;
+        f.vertexVelocity = vertex_velocity;
+        f.vertexPendingForce = vertex_pending_force;
+        f.lineLength = line_length;
+        f.lineViolation = line_violation;
+        f.diagnostics = diagnostics;
+        emitFrame(swx::encodeFrame(d, *mapping, *context, f), bytes, capacity, size);
     });
 }
 

@@ -116,6 +116,15 @@ public:
     State download() const;
     void upload(const State& state);
 
+    // Diagnostic-frame fields of the committed state: harness point
+    // positions and velocities (3 per point, definition order), and per line
+    // (constraints, then suspension segments, definition order) the current
+    // length and its violation.
+    struct Frame {
+        std::vector<double> harnessPosition, harnessVelocity, lineLength, lineViolation;
+    };
+    Frame frame();
+
     // Reductions for StructureDiagnostics (see capi.cpp for the fields).
     struct Reductions {
         double mass, weighted[3], momentum[3], kinetic;

@@ -1251,6 +1251,65 @@ __global__ void __launch_bounds__(256) diagnosticsKernel(WingView v, Partial* ou
     }
 }
 
+// Diagnostic-frame fields of the committed state (structure_frame.cpp:
+// 491-640): harness point positions and velocities
+// (payloadPointPosition/Velocity, suspension.cpp:1018-1032), and per line
+// (distance slots, then segment slots) the current length |second - first|
+// and its violation (structure_frame.cpp:60-73: |L - rest| for distance,
+// tie and stitch, max(0, L - rest) for cables and suspension segments).
+// out: harness position 3H, velocity 3H (runtime hang order), then length
+// and violation per distance slot, then per segment slot.
+__global__ void frameKernel(WingView v, double* out) {
+    const int h = v.hangCount;
+    double* hpos = out;
+    double* hvel = out + 3L * h;
+    double* length = out + 6L * h;
+    double* violation = length + v.distCount + v.segCount;
+    const PayloadDev& p = *v.payload;
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    const long first = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    m3 r{};
+    bool rigid = false;
+    if (v.segCount > 0) rigid = rotationMatrix(quat{p.q[0], p.q[1], p.q[2], p.q[3]}, r);
+    const d3 com{p.com[0], p.com[1], p.com[2]};
+    const auto hang = [&](int i, d3& position, d3& offset) {
+        const d3 rel = sub(d3{v.hangLocal[3 * i], v.hangLocal[3 * i + 1], v.hangLocal[3 * i + 2]},
+                           d3{p.comLocal[0], p.comLocal[1], p.comLocal[2]});
+        offset = mulM(r, rel);
+        position = add(com, offset);
+    };
+    for (long i = first; i < h; i += stride) {
+        d3 position, offset;
+        hang(static_cast<int>(i), position, offset);
+        const d3 velocity = add(d3{p.v[0], p.v[1], p.v[2]}, cross(d3{p.w[0], p.w[1], p.w[2]}, offset));
+        const double nan = __builtin_nan("");
+        hpos[3 * i] = rigid ? position.x : nan;
+        hpos[3 * i + 1] = rigid ? position.y : nan;
+        hpos[3 * i + 2] = rigid ? position.z : nan;
+        hvel[3 * i] = rigid ? velocity.x : nan;
+        hvel[3 * i + 1] = rigid ? velocity.y : nan;
+        hvel[3 * i + 2] = rigid ? velocity.z : nan;
+    }
+    for (long k = first; k < v.distCount; k += stride) {
+        const int a = v.da[k], b = v.db[k];
+        const double l = len(sub(d3{v.px[b], v.py[b], v.pz[b]}, d3{v.px[a], v.py[a], v.pz[a]}));
+        const double extension = l - v.drest[k];
+        length[k] = l;
+        violation[k] = v.dcable[k] ? stdmax(0.0, extension) : fabs(extension);
+    }
+    for (long k = first; k < v.segCount; k += stride) {
+        d3 ends[2], offset;
+        const int codes[2] = {v.sfrom[k], v.sto[k]};
+        for (int e = 0; e < 2; ++e) {
+            if (codes[e] >= 0) ends[e] = d3{v.px[codes[e]], v.py[codes[e]], v.pz[codes[e]]};
+            else hang(-codes[e] - 1, ends[e], offset);
+        }
+        const double l = len(sub(ends[1], ends[0]));
+        length[v.distCount + k] = l;
+        violation[v.distCount + k] = stdmax(0.0, l - v.srest[k]);
+    }
+}
+
 void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
 }
@@ -2246,6 +2305,50 @@ DeviceWing::Reductions DeviceWing::reduce() {
         r.dynamicNodes += p.dynamicNodes;
     }
     return r;
+}
+
+DeviceWing::Frame DeviceWing::frame() {
+    const WingView& v = *view_;
+    const std::size_t h = static_cast<std::size_t>(v.hangCount);
+    const std::size_t doubles = 6 * h + 2 * (constraints_ + segments_);
+    double* out = nullptr;
+    check(hipMalloc(&out, std::max<std::size_t>(doubles, 1) * sizeof(double)), "hipMalloc frame");
+    std::vector<double> host(doubles);
+    try {
+        const long work = static_cast<long>(std::max({h, constraints_, segments_, std::size_t{1}}));
+        const int blocks = static_cast<int>(std::min<long>(1024, (work + 255) / 256));
+        hipLaunchKernelGGL(frameKernel, dim3(blocks), dim3(256), 0, stream_, v, out);
+        check(hipGetLastError(), "frame");
+        check(hipMemcpyAsync(host.data(), out, doubles * sizeof(double), hipMemcpyDeviceToHost, stream_), "frame");
+        check(hipStreamSynchronize(stream_), "frame");
+    } catch (...) {
+        (void)hipFree(out);
+        throw;
+    }
+    (void)hipFree(out);
+    Frame f;
+    // Harness points in definition order; lines in definition order.
+    f.harnessPosition.assign(3 * h, 0.0);
+    f.harnessVelocity.assign(3 * h, 0.0);
+    for (std::size_t r = 0; r < h; ++r)
+        for (int c = 0; c < 3; ++c) {
+            f.harnessPosition[3 * layout_.hangOrder[r] + c] = host[3 * r + c];
+            f.harnessVelocity[3 * layout_.hangOrder[r] + c] = host[3 * h + 3 * r + c];
+        }
+    const double* length = host.data() + 6 * h;
+    const double* violation = length + constraints_ + segments_;
+    f.lineLength.assign(constraints_ + segments_, 0.0);
+    f.lineViolation.assign(constraints_ + segments_, 0.0);
+    for (std::size_t k = 0; k < constraints_; ++k) {
+        f.lineLength[distance_.order[k]] = length[k];
+        f.lineViolation[distance_.order[k]] = violation[k];
+    }
+    for (std::size_t r = 0; r < segments_; ++r) {
+        const std::size_t slot = segmentDevicePosition_[r];
+        f.lineLength[constraints_ + layout_.segmentOrder[r]] = length[constraints_ + slot];
+        f.lineViolation[constraints_ + layout_.segmentOrder[r]] = violation[constraints_ + slot];
+    }
+    return f;
 }
 
 WingView DeviceWing::launchView() const {

@@ -136,6 +136,36 @@ class Diagnostics(C.Structure):
     ]
 
 
+class FrameMapping(C.Structure):
+    _fields_ = [
+        ("vertex_count", C.c_size_t),
+        ("vertex_ids", _U64P),
+        ("triangle_count", C.c_size_t),
+        ("triangle_ids", _U64P),
+        ("triangle_negative_region", _U64P),
+        ("triangle_positive_region", _U64P),
+        ("line_count", C.c_size_t),
+        ("line_ids", _U64P),
+        ("line_roles", _U32P),
+    ]
+
+
+class FrameContext(C.Structure):
+    _fields_ = [
+        ("scene_checksum", C.c_char_p),
+        ("solver_commit", C.c_char_p),
+        ("time_step", C.c_double),
+        ("coupling_iteration", C.c_uint32),
+        ("coupling_residuals", C.c_double * 5),
+        ("fluid_mass", C.c_double),
+        ("total_momentum", C.c_double * 3),
+        ("total_energy", C.c_double),
+        ("interface_force_residual", C.c_double * 3),
+        ("interface_moment_residual", C.c_double * 3),
+        ("interface_power_residual", C.c_double),
+    ]
+
+
 class AssemblySettings(C.Structure):
     _fields_ = [
         ("seam_coincidence_tolerance", C.c_double),
@@ -215,6 +245,12 @@ _SIGNATURES = {
     "sw_assembly_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "sw_assembly_flight_loads": (C.c_int, [C.c_void_p, _F64P]),
+    "sw_structure_frame_fields": (C.c_int, [C.c_void_p, _F64P, _F64P, _F64P, _F64P, _F64P]),
+    "sw_structure_frame": (C.c_int, [C.c_void_p, C.POINTER(FrameMapping), C.POINTER(FrameContext), _U8P,
+                                     C.c_size_t, C.POINTER(C.c_size_t)]),
+    "sw_frame_encode": (C.c_int, [C.c_void_p, C.POINTER(FrameMapping), C.POINTER(FrameContext), C.c_uint64,
+                                  C.c_double, _F64P, _F64P, _F64P, _F64P, _F64P, C.POINTER(Diagnostics), _U8P,
+                                  C.c_size_t, C.POINTER(C.c_size_t)]),
     "sw_batch_create": (C.c_int, [C.POINTER(C.POINTER(Definition)), C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
     "sw_batch_destroy": (None, [C.c_void_p]),
     "sw_batch_set_face_pressure": (C.c_int, [C.c_void_p, C.c_size_t, _F64P, C.c_size_t]),

@@ -9,6 +9,11 @@ wings, and writes into tests/golden/:
   tables (tiny, small), plus the build's hinge colour order;
 * ``<preset>.colouring.swfx`` -- SoftBody::constraintColouringView();
 * ``<preset>.states.swfx``    -- full solver state after the listed steps;
+* ``<preset>.frame.swfx``     -- ``ref_driver --frame``: the reference's own
+  simwing::fsi::Structure stepped with the flight loads (no face pressure,
+  loads pending again after the last step) and the diagnostic-frame fields
+  simwing::viewer::buildStructureFrame makes of it (tiny: with the frame's
+  serialized bytes; small: their SHA-256 digest and size);
 * ``export.digest.swfx``      -- for the ~70k-triangle wing: SHA-256 digests
   of every assembly and colouring array, and strided samples plus digests of
   the state after 1 and 240 steps (the full arrays are too large to commit;
@@ -40,6 +45,7 @@ PLAN = {
     "export": {"steps": 240, "snapshots": [1, 240], "workers": 8},
 }
 STRIDE = 101
+FRAME_STEPS = {"tiny": 20, "small": 40}
 
 
 def digest(array: np.ndarray) -> np.ndarray:
@@ -86,12 +92,35 @@ def run(preset: str) -> None:
     swfx.write(str(HERE / "export.digest.swfx"), out)
 
 
+def run_frame(preset: str) -> None:
+    WORK.mkdir(parents=True, exist_ok=True)
+    path = WORK / f"{preset}.frame.swfx"
+    cmd = [str(DRIVER), "--frame", preset, str(path), str(FRAME_STEPS[preset])]
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    frame = swfx.read(str(path))
+    n = frame["position"].size
+    for key, field in (("position", "vertex_position"), ("velocity", "vertex_velocity"),
+                       ("pending_force", "vertex_pending_force")):
+        # The frame's first N vertices are the committed nodes themselves.
+        assert np.array_equal(frame[key].view(np.uint64), frame[field][:n].view(np.uint64)), key
+        del frame[key]
+    frame["node_count"] = np.array([n // 3], np.uint64)
+    if preset != "tiny":
+        data = frame.pop("frame_bytes")
+        frame["frame_bytes_digest"] = digest(data)
+        frame["frame_bytes_size"] = np.array([data.size], np.uint64)
+    swfx.write(str(HERE / f"{preset}.frame.swfx"), frame)
+
+
 def main(argv: list[str]) -> int:
     if not DRIVER.exists():
         print(f"{DRIVER} is missing: run make -C oracle ref (needs /root/reference)", file=sys.stderr)
         return 2
     for preset in argv or list(PLAN):
         run(preset)
+        if preset in FRAME_STEPS:
+            run_frame(preset)
     return 0
 
 
