@@ -551,6 +551,49 @@ __device__ bool projectSegment(const WingView& v, const Store& N, int fe, int te
     return true;
 }
 
+// normalizedCanonical with the failure returned as a flag after the values
+// are computed (the same values as canonical where it succeeds), so no
+// branch waits on the normalisation.
+__device__ __forceinline__ bool canonicalDeferred(const quat& q, quat& out) {
+    const double n2 = qnorm2(q);
+    const bool ok = finQ(q) && (n2 > 1.0e-28);
+    const double inv = 1.0 / sqrt(n2);
+    const quat r{q.w * inv, q.x * inv, q.y * inv, q.z * inv};
+    const bool flip = r.w < 0.0 || (r.w == 0.0 && (r.x < 0.0 || (r.x == 0.0 && (r.y < 0.0 || (r.y == 0.0 && r.z < 0.0)))));
+    out = quat{flip ? -r.w : r.w, flip ? -r.x : r.x, flip ? -r.y : r.y, flip ? -r.z : r.z};
+    return ok;
+}
+
+__device__ __forceinline__ bool rotationMatrixDeferred(const quat& orientation, m3& r) {
+    quat q;
+    const bool ok = canonicalDeferred(orientation, q);
+    const double xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
+    const double xy = q.x * q.y, xz = q.x * q.z, yz = q.y * q.z;
+    const double wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
+    r.a[0] = 1.0 - 2.0 * (yy + zz);
+    r.a[1] = 2.0 * (xy - wz);
+    r.a[2] = 2.0 * (xz + wy);
+    r.a[3] = 2.0 * (xy + wz);
+    r.a[4] = 1.0 - 2.0 * (xx + zz);
+    r.a[5] = 2.0 * (yz - wx);
+    r.a[6] = 2.0 * (xz - wy);
+    r.a[7] = 2.0 * (yz + wx);
+    r.a[8] = 1.0 - 2.0 * (xx + yy);
+    return ok;
+}
+
+__device__ __forceinline__ bool rotationIncrementDeferred(d3 v, quat& out) {
+    const bool finite = fin3(v);
+    const double angle = len(v);
+    if (angle < 1.0e-12) return canonicalDeferred(quat{1.0, 0.5 * v.x, 0.5 * v.y, 0.5 * v.z}, out) && finite;
+    const double half = 0.5 * angle;
+    double sine, cosine;
+    pm::sincos(half, &sine, &cosine);
+    const double s = sine / angle;
+    out = quat{cosine, v.x * s, v.y * s, v.z * s};
+    return finite;
+}
+
 __device__ __forceinline__ bool refreshInverseInertia(PayloadDev& def, const quat& q) {
     m3 world, inv;
     if (!worldInertia(q, def.inertia, world) || !inverseM(world, inv)) return false;
@@ -585,6 +628,9 @@ __shared__ double chainX[kChainSlots], chainY[kChainSlots], chainZ[kChainSlots],
 __shared__ double chainHang[3 * kChainHang];
 __shared__ double chainLambda[kChainItems], chainRest[kChainItems], chainStiffness[kChainItems];
 __shared__ int chainItem[4 * kChainItems];  // from, to, runtime index, last-payload flag
+__shared__ double chainAlpha[kChainItems];   // 1 / (stiffness dt^2)
+__shared__ int chainStatic[kChainItems];     // rest, stiffness and dt pass projectSegment's checks
+__shared__ d3 chainRel[kChainHang];          // harness point local - comLocal
 
 struct SlotRefs {
     __device__ __forceinline__ d3 get(int i) const { return d3{chainX[i], chainY[i], chainZ[i]}; }
@@ -598,6 +644,116 @@ struct SlotRefs {
         chainZ[i] = p.z;
     }
 };
+
+// One segment of a staged serial run: projectSegment's operations in the
+// same order, specialised on the endpoint kinds (FromHang / ToHang: a
+// harness point, whose offset from the centre of mass local - comLocal is
+// staged in `rel`; otherwise a run slot). The per-segment constants are
+// staged: alphaTilde = 1 / (stiffness dt^2) and `staticOk`, the checks of
+// rest, stiffness and dt. Every failure becomes a flag tested once before
+// the corrections (and once after the payload rotation), so no branch waits
+// on the projection's arithmetic; the error raised is the one
+// projectSegment raises.
+struct ChainEnd {
+    d3 position, offset;
+    double invMass;
+};
+
+template <bool Hang>
+__device__ __forceinline__ ChainEnd chainEnd(int e, const m3& r, const Payload& P, const d3* rel,
+                                             double invPayloadMass) {
+    ChainEnd out;
+    if constexpr (Hang) {
+        out.position = add(P.com, mulM(r, rel[-e - 1]));
+        out.offset = sub(out.position, P.com);
+        out.invMass = invPayloadMass;
+    } else {
+        out.position = d3{chainX[e], chainY[e], chainZ[e]};
+        out.offset = d3{0.0, 0.0, 0.0};
+        out.invMass = chainW[e];
+    }
+    return out;
+}
+
+template <bool Hang>
+__device__ __forceinline__ bool chainEffectiveMass(const ChainEnd& e, d3 g, const m3& iinv, bool iinvFinite,
+                                                   double& out) {
+    const bool pre = (e.invMass >= 0.0) && isfinite(e.invMass) && fin3(g) && fin3(e.offset) && iinvFinite;
+    double r = e.invMass;
+    if constexpr (Hang) {
+        if (e.invMass > 0.0) {
+            const d3 rot = cross(e.offset, g);
+            r += dot(rot, mulM(iinv, rot));
+        }
+    }
+    out = r;
+    return pre && (r >= 0.0) && isfinite(r);
+}
+
+template <bool Hang>
+__device__ __forceinline__ bool chainCorrect(int e, const ChainEnd& end, d3 grad, double increment, Payload& P,
+                                             const m3& iinv) {
+    if constexpr (Hang) {
+        const d3 step = scale(grad, end.invMass * increment);
+        P.com = add(P.com, step);
+        const d3 rotation = scale(mulM(iinv, cross(end.offset, grad)), increment);
+        quat inc, rotated;
+        const bool ok = rotationIncrementDeferred(rotation, inc);
+        const bool normal = canonicalDeferred(qmul(inc, P.q), rotated);
+        P.q = rotated;
+        return ok && normal;
+    } else {
+        if (end.invMass == 0.0) return true;  // fixed node
+        const d3 step = scale(grad, end.invMass * increment);
+        const d3 p{chainX[e], chainY[e], chainZ[e]};
+        const d3 q = add(p, step);
+        chainX[e] = q.x;
+        chainY[e] = q.y;
+        chainZ[e] = q.z;
+        return true;
+    }
+}
+
+template <bool FromHang, bool ToHang>
+__device__ __forceinline__ bool chainSegment(const WingView& v, int fe, int te, double rest, double alphaTilde,
+                                             bool staticOk, double& lambda, int runtime, Payload& P, const d3* rel,
+                                             double invPayloadMass, double minLength, const m3& iinv,
+                                             bool iinvFinite) {
+    m3 r{};
+    bool endsOk = true;
+    if constexpr (FromHang || ToHang) endsOk = rotationMatrixDeferred(P.q, r);
+    const ChainEnd from = chainEnd<FromHang>(fe, r, P, rel, invPayloadMass);
+    const ChainEnd to = chainEnd<ToHang>(te, r, P, rel, invPayloadMass);
+    const d3 diff = sub(to.position, from.position);
+    const double length = len(diff);
+    const d3 dir = divide(diff, length);
+    const d3 toGrad = neg(dir);
+    const double gradLength = len(dir);
+    double effFrom, effTo;
+    const bool effOk = chainEffectiveMass<FromHang>(from, dir, iinv, iinvFinite, effFrom) &
+                       chainEffectiveMass<ToHang>(to, toGrad, iinv, iinvFinite, effTo);
+    const double eff = effFrom + effTo;
+    const bool lineOk = isfinite(length) && !(length <= minLength) && (length > 0.0) && staticOk &&
+                        (lambda >= 0.0) && isfinite(lambda) && !(fabs(gradLength - 1.0) > 1.0e-10) && effOk &&
+                        (eff + alphaTilde > 0.0);
+    const double gap = rest - length;
+    const double candidate = lambda + (-gap - alphaTilde * lambda) / (eff + alphaTilde);
+    const double next = stdmax(0.0, candidate);
+    const double increment = next - lambda;
+    if (!(endsOk && lineOk)) {
+        if (!endsOk) raise(v, SW_SUSPENSION_ERROR, 1, 2, runtime);
+        else raise(v, SW_SUSPENSION_ERROR, 6, 1, runtime);
+        return false;
+    }
+    lambda = next;
+    const bool fromOk = chainCorrect<FromHang>(fe, from, dir, increment, P, iinv);
+    const bool toOk = chainCorrect<ToHang>(te, to, toGrad, increment, P, iinv);
+    if (!(fromOk && toOk)) {
+        raise(v, SW_SUSPENSION_ERROR, 5, 2, 0);
+        return false;
+    }
+    return true;
+}
 
 // A serial segment run: consecutive single-item dependency levels, i.e.
 // segments that share the payload or a node, in the reference's string-id
@@ -637,20 +793,28 @@ __device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, cons
     }
     for (int j = threadIdx.x; j < items; j += blockDim.x) {
         const int k = P.begin + j;
+        const double rest = v.srest[k], stiffness = v.sstiff[k];
         chainLambda[j] = v.slam[k];
-        chainRest[j] = v.srest[k];
-        chainStiffness[j] = v.sstiff[k];
+        chainRest[j] = rest;
+        chainStiffness[j] = stiffness;
+        chainAlpha[j] = 1.0 / (stiffness * a.dt * a.dt);
+        chainStatic[j] = (rest > 0.0) && isfinite(rest) && (stiffness > 0.0) && isfinite(stiffness) && (a.dt > 0.0) &&
+                         isfinite(a.dt);
         chainItem[4 * j] = v.sfromSlot[k];
         chainItem[4 * j + 1] = v.stoSlot[k];
         chainItem[4 * j + 2] = v.sruntime[k];
         chainItem[4 * j + 3] = v.slastPayload[k];
     }
     for (int h = threadIdx.x; h < 3 * v.hangCount; h += blockDim.x) chainHang[h] = v.hangLocal[h];
+    for (int h = threadIdx.x; h < v.hangCount; h += blockDim.x)
+        chainRel[h] = sub(d3{v.hangLocal[3 * h], v.hangLocal[3 * h + 1], v.hangLocal[3 * h + 2]},
+                          d3{payload.comLocal[0], payload.comLocal[1], payload.comLocal[2]});
     __syncthreads();
     if (threadIdx.x == 0) {
         const SlotRefs S{};
-        const StepArgs args = a;  // by value: kept in registers across the loop
         const PayloadDev def = payload;
+        const double invPayloadMass = 1.0 / def.mass;
+        const double minLength = def.minLineLength;
         Payload state = loadPayload(def);
         m3 iinv = loadInverseInertia(def);
         bool iinvFinite = finiteM(iinv);
@@ -661,9 +825,23 @@ __device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, cons
             if (rec)
                 recordSegment(rec + kRecSeg * static_cast<long>(P.begin + j), fe >= 0 ? S.get(fe) : d3{0.0, 0.0, 0.0},
                               te >= 0 ? S.get(te) : d3{0.0, 0.0, 0.0}, lambda, chainRest[j], state, iinv);
-            if (!projectSegment(v, S, fe, te, chainRest[j], chainStiffness[j], lambda, chainItem[4 * j + 2], state,
-                                def, iinv, iinvFinite, args))
-                break;
+            const double rest = chainRest[j], alpha = chainAlpha[j];
+            const bool fixedOk = chainStatic[j] != 0;
+            const int runtime = chainItem[4 * j + 2];
+            bool ok;
+            if (fe >= 0 && te >= 0)
+                ok = chainSegment<false, false>(v, fe, te, rest, alpha, fixedOk, lambda, runtime, state, chainRel,
+                                                invPayloadMass, minLength, iinv, iinvFinite);
+            else if (fe >= 0)
+                ok = chainSegment<false, true>(v, fe, te, rest, alpha, fixedOk, lambda, runtime, state, chainRel,
+                                               invPayloadMass, minLength, iinv, iinvFinite);
+            else if (te >= 0)
+                ok = chainSegment<true, false>(v, fe, te, rest, alpha, fixedOk, lambda, runtime, state, chainRel,
+                                               invPayloadMass, minLength, iinv, iinvFinite);
+            else
+                ok = chainSegment<true, true>(v, fe, te, rest, alpha, fixedOk, lambda, runtime, state, chainRel,
+                                              invPayloadMass, minLength, iinv, iinvFinite);
+            if (!ok) break;
             chainLambda[j] = lambda;
             if (fe >= 0 && te >= 0) continue;
             touched = true;
@@ -701,6 +879,38 @@ __device__ __forceinline__ void waitDeps(const WingView& v, int index, int step)
             const int need = step - v.depLag[e] + 1;
             if (need <= 0) continue;
             unsigned* flag = v.flags + 32 * v.depPart[e];
+            unsigned spins = 0;
+            while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < static_cast<unsigned>(need)) {
+                __builtin_amdgcn_s_sleep(1);
+                if (syncTimedOut(v, ++spins)) break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+// A participant's iteration program staged in LDS at the start of the
+// point-to-point loop: the phase table and the participant's dependency
+// entries (progress word, lag), so the loop reads no phase descriptor and no
+// dependency list from global memory. Programs beyond these capacities read
+// them from global memory instead.
+constexpr int kStagedPhases = 128;
+constexpr int kStagedDeps = 2048;
+__shared__ Phase stagedPhase[kStagedPhases];
+__shared__ int stagedDepOffset[kStagedPhases + 1];
+__shared__ int2 stagedDep[kStagedDeps];  // participant, lag
+
+// waitDeps over the staged entries of phase `ph`.
+__device__ __forceinline__ void waitStagedDeps(const WingView& v, int ph, int step) {
+    const int e0 = stagedDepOffset[ph], e1 = stagedDepOffset[ph + 1];
+    if (e0 == e1) return;
+    if (threadIdx.x < 64) {
+        for (int e = e0 + static_cast<int>(threadIdx.x); e < e1; e += 64) {
+            const int2 dep = stagedDep[e];
+            const int need = step - dep.y + 1;
+            if (need <= 0) continue;
+            unsigned* flag = v.flags + 32 * dep.x;
             unsigned spins = 0;
             while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < static_cast<unsigned>(need)) {
                 __builtin_amdgcn_s_sleep(1);
@@ -998,9 +1208,18 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         const int m = v.iterPhaseCount;
         const int p = block;
         const bool chainGroup = chainOnly;
+        // The host enables this mode only for programs that fit the LDS
+        // staging (DeviceWing::buildSync).
+        const int depBase = v.depOffsets[p * m];
+        const int depCount = v.depOffsets[p * m + m] - depBase;
+        for (int k = threadIdx.x; k < m; k += blockDim.x) stagedPhase[k] = v.iterPhases[k];
+        for (int k = threadIdx.x; k <= m; k += blockDim.x) stagedDepOffset[k] = v.depOffsets[p * m + k] - depBase;
+        for (int e = threadIdx.x; e < depCount; e += blockDim.x)
+            stagedDep[e] = int2{v.depPart[depBase + e], v.depLag[depBase + e]};
+        __syncthreads();
         for (int it = 0; it < a.iterations; ++it) {
             for (int ph = 0; ph < m; ++ph) {
-                const Phase P = v.iterPhases[ph];
+                const Phase P = stagedPhase[ph];
                 int b = P.begin, e = P.begin;
                 if (P.type == kPhaseSegmentRun) {
                     if (p == (dedicated ? workers : 0)) e = P.end;
@@ -1011,7 +1230,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                 }
                 if (b >= e) continue;
                 const int step = it * m + ph;
-                waitDeps(v, p * m + ph, step);
+                waitStagedDeps(v, ph, step);
                 runPhase(v, N, P, b + static_cast<int>(threadIdx.x), e, static_cast<int>(blockDim.x), payload, a,
                          trace ? trace + layout.iterBase + it * layout.iterStride : nullptr, layout);
                 __syncthreads();
@@ -1910,15 +2129,18 @@ void DeviceWing::build(const Definition& d) {
     check(hipDeviceSynchronize(), "sync build");
 }
 
-// Dynamic LDS available to the LDS-resident kernel: the per-workgroup
-// maximum less the kernel's static LDS.
+// Dynamic LDS available to an LDS-resident wing: the per-workgroup maximum
+// less the static LDS of the kernels that run it (the solo LDS kernel and the
+// batched kernel, whose static LDS also holds the point-to-point staging).
 std::size_t DeviceWing::sharedNodeBudget() const {
     int maxShared = 0;
     check(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device_), "attribute");
-    hipFuncAttributes attributes{};
-    check(hipFuncGetAttributes(&attributes, reinterpret_cast<const void*>(&substepKernel<true>)), "kernel attributes");
+    hipFuncAttributes solo{}, batch{};
+    check(hipFuncGetAttributes(&solo, reinterpret_cast<const void*>(&substepKernel<true>)), "kernel attributes");
+    check(hipFuncGetAttributes(&batch, reinterpret_cast<const void*>(&batchSubstepKernel)), "kernel attributes");
     const std::size_t total = static_cast<std::size_t>(maxShared);
-    return total > attributes.sharedSizeBytes ? total - attributes.sharedSizeBytes : 0;
+    const std::size_t fixed = std::max(solo.sharedSizeBytes, batch.sharedSizeBytes);
+    return total > fixed ? total - fixed : 0;
 }
 
 void DeviceWing::setLaunch(int workgroups, int threads) {
@@ -2131,7 +2353,9 @@ void DeviceWing::buildSync() {
         }
     }
     std::vector<int> offsets(static_cast<std::size_t>(parts) * m + 1, 0), part, lag;
-    for (int p = 0; p < parts; ++p)
+    bool fits = m <= kStagedPhases;
+    for (int p = 0; p < parts; ++p) {
+        const std::size_t first = part.size();
         for (int ph = 0; ph < m; ++ph) {
             const int* need = needStep.data() + (static_cast<std::size_t>(p) * m + ph) * parts;
             for (int q = 0; q < parts; ++q)
@@ -2143,6 +2367,15 @@ void DeviceWing::buildSync() {
                 }
             offsets[static_cast<std::size_t>(p) * m + ph + 1] = static_cast<int>(part.size());
         }
+        fits = fits && part.size() - first <= static_cast<std::size_t>(kStagedDeps);
+    }
+    if (!fits) {
+        // The loop stages each participant's program in LDS; a program
+        // beyond that capacity runs with grid barriers instead.
+        neighbourSync_ = false;
+        dedicatedChain_ = false;
+        return;
+    }
     const auto upload = [&](const std::vector<int>& host, int index) {
         void* ptr = nullptr;
         const std::size_t bytes = std::max<std::size_t>(host.size(), 1) * sizeof(int);
