@@ -193,6 +193,7 @@ private:
     // current launch shape.
     void buildSync();
     std::size_t sharedNodeBudget() const;
+    std::size_t stageBytes_ = 0;  // dynamic LDS of the staged point-to-point program
     bool hasRun_ = false, neighbourSync_ = false, dedicatedChain_ = false, cooperative_ = true;
     int participants_ = 0, maxParticipants_ = 0;
     unsigned long long launchGeneration_ = 0;

@@ -890,24 +890,42 @@ __device__ __forceinline__ void waitDeps(const WingView& v, int index, int step)
     __syncthreads();
 }
 
-// A participant's iteration program staged in LDS at the start of the
-// point-to-point loop: the phase table and the participant's dependency
-// entries (progress word, lag), so the loop reads no phase descriptor and no
-// dependency list from global memory. Programs beyond these capacities read
-// them from global memory instead.
-constexpr int kStagedPhases = 128;
-constexpr int kStagedDeps = 2048;
-__shared__ Phase stagedPhase[kStagedPhases];
-__shared__ int stagedDepOffset[kStagedPhases + 1];
-__shared__ int2 stagedDep[kStagedDeps];  // participant, lag
+// A participant's iteration program staged in dynamic LDS (after the
+// reduction scratch) at the start of the point-to-point loop: the phase
+// table, the participant's dependency offsets and its entries (progress
+// word, lag), so the loop reads no phase descriptor and no dependency list
+// from global memory. Layout for m phases and d entries: m Phase records,
+// then m + 1 offsets, then d int2 entries, each part 8-byte aligned
+// (stagedProgramBytes).
+struct StagedProgram {
+    Phase* phase;
+    int* offset;
+    int2* dep;
+};
+
+__host__ __device__ constexpr std::size_t alignTo8(std::size_t bytes) { return (bytes + 7) & ~std::size_t{7}; }
+__host__ __device__ constexpr std::size_t stagedProgramBytes(int m, int deps) {
+    return alignTo8(static_cast<std::size_t>(m) * sizeof(Phase)) + alignTo8(static_cast<std::size_t>(m + 1) * sizeof(int)) +
+           static_cast<std::size_t>(deps) * sizeof(int2);
+}
+constexpr std::size_t kMaxStagedProgramBytes = 48 * 1024;
+
+__device__ __forceinline__ StagedProgram stagedProgram(char* base, int m) {
+    StagedProgram s;
+    s.phase = reinterpret_cast<Phase*>(base);
+    s.offset = reinterpret_cast<int*>(base + alignTo8(static_cast<std::size_t>(m) * sizeof(Phase)));
+    s.dep = reinterpret_cast<int2*>(base + alignTo8(static_cast<std::size_t>(m) * sizeof(Phase)) +
+                                    alignTo8(static_cast<std::size_t>(m + 1) * sizeof(int)));
+    return s;
+}
 
 // waitDeps over the staged entries of phase `ph`.
-__device__ __forceinline__ void waitStagedDeps(const WingView& v, int ph, int step) {
-    const int e0 = stagedDepOffset[ph], e1 = stagedDepOffset[ph + 1];
+__device__ __forceinline__ void waitStagedDeps(const WingView& v, const StagedProgram& S, int ph, int step) {
+    const int e0 = S.offset[ph], e1 = S.offset[ph + 1];
     if (e0 == e1) return;
     if (threadIdx.x < 64) {
         for (int e = e0 + static_cast<int>(threadIdx.x); e < e1; e += 64) {
-            const int2 dep = stagedDep[e];
+            const int2 dep = S.dep[e];
             const int need = step - dep.y + 1;
             if (need <= 0) continue;
             unsigned* flag = v.flags + 32 * dep.x;
@@ -1208,18 +1226,19 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         const int m = v.iterPhaseCount;
         const int p = block;
         const bool chainGroup = chainOnly;
-        // The host enables this mode only for programs that fit the LDS
-        // staging (DeviceWing::buildSync).
+        // The host sizes the dynamic LDS for this participant's program
+        // (DeviceWing::buildSync); it follows the reduction scratch.
+        const StagedProgram S = stagedProgram(reinterpret_cast<char*>(lds + kMaxThreads), m);
         const int depBase = v.depOffsets[p * m];
         const int depCount = v.depOffsets[p * m + m] - depBase;
-        for (int k = threadIdx.x; k < m; k += blockDim.x) stagedPhase[k] = v.iterPhases[k];
-        for (int k = threadIdx.x; k <= m; k += blockDim.x) stagedDepOffset[k] = v.depOffsets[p * m + k] - depBase;
+        for (int k = threadIdx.x; k < m; k += blockDim.x) S.phase[k] = v.iterPhases[k];
+        for (int k = threadIdx.x; k <= m; k += blockDim.x) S.offset[k] = v.depOffsets[p * m + k] - depBase;
         for (int e = threadIdx.x; e < depCount; e += blockDim.x)
-            stagedDep[e] = int2{v.depPart[depBase + e], v.depLag[depBase + e]};
+            S.dep[e] = int2{v.depPart[depBase + e], v.depLag[depBase + e]};
         __syncthreads();
         for (int it = 0; it < a.iterations; ++it) {
             for (int ph = 0; ph < m; ++ph) {
-                const Phase P = stagedPhase[ph];
+                const Phase P = S.phase[ph];
                 int b = P.begin, e = P.begin;
                 if (P.type == kPhaseSegmentRun) {
                     if (p == (dedicated ? workers : 0)) e = P.end;
@@ -1230,7 +1249,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                 }
                 if (b >= e) continue;
                 const int step = it * m + ph;
-                waitStagedDeps(v, ph, step);
+                waitStagedDeps(v, S, ph, step);
                 runPhase(v, N, P, b + static_cast<int>(threadIdx.x), e, static_cast<int>(blockDim.x), payload, a,
                          trace ? trace + layout.iterBase + it * layout.iterStride : nullptr, layout);
                 __syncthreads();
@@ -2131,7 +2150,7 @@ void DeviceWing::build(const Definition& d) {
 
 // Dynamic LDS available to an LDS-resident wing: the per-workgroup maximum
 // less the static LDS of the kernels that run it (the solo LDS kernel and the
-// batched kernel, whose static LDS also holds the point-to-point staging).
+// batched kernel).
 std::size_t DeviceWing::sharedNodeBudget() const {
     int maxShared = 0;
     check(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device_), "attribute");
@@ -2197,7 +2216,7 @@ void DeviceWing::launchSubstep(const StepArgs& stepArgs, double* trace) {
         }
         check(hipEventRecord(events_[eventsUsed_], stream_), "hipEventRecord");
     }
-    const std::size_t scratch = kMaxThreads * sizeof(double);
+    const std::size_t scratch = kMaxThreads * sizeof(double) + (neighbourSync_ ? stageBytes_ : 0);
     if (sharedNodes_) {
         const std::size_t bytes = 4 * nodes_ * sizeof(double) + scratch;
         hipLaunchKernelGGL(substepKernel<true>, dim3(1), dim3(threads_), bytes, stream_, v, args);
@@ -2287,6 +2306,7 @@ void DeviceWing::buildSync() {
     }
     v.depOffsets = v.depPart = v.depLag = nullptr;
     participants_ = 0;
+    stageBytes_ = 0;
     if (!neighbourSync_) return;
     const int parts = workgroups_;
     const int workers = dedicatedChain_ ? parts - 1 : parts;
@@ -2353,7 +2373,7 @@ void DeviceWing::buildSync() {
         }
     }
     std::vector<int> offsets(static_cast<std::size_t>(parts) * m + 1, 0), part, lag;
-    bool fits = m <= kStagedPhases;
+    int maxDeps = 0;
     for (int p = 0; p < parts; ++p) {
         const std::size_t first = part.size();
         for (int ph = 0; ph < m; ++ph) {
@@ -2367,15 +2387,16 @@ void DeviceWing::buildSync() {
                 }
             offsets[static_cast<std::size_t>(p) * m + ph + 1] = static_cast<int>(part.size());
         }
-        fits = fits && part.size() - first <= static_cast<std::size_t>(kStagedDeps);
+        maxDeps = std::max(maxDeps, static_cast<int>(part.size() - first));
     }
-    if (!fits) {
+    if (stagedProgramBytes(m, maxDeps) > kMaxStagedProgramBytes) {
         // The loop stages each participant's program in LDS; a program
         // beyond that capacity runs with grid barriers instead.
         neighbourSync_ = false;
         dedicatedChain_ = false;
         return;
     }
+    stageBytes_ = stagedProgramBytes(m, maxDeps);
     const auto upload = [&](const std::vector<int>& host, int index) {
         void* ptr = nullptr;
         const std::size_t bytes = std::max<std::size_t>(host.size(), 1) * sizeof(int);
@@ -2594,7 +2615,8 @@ WingView DeviceWing::launchView() const {
 }
 
 std::size_t DeviceWing::dynamicLdsBytes() const {
-    return (sharedNodes_ ? 4 * nodes_ * sizeof(double) : 0) + kMaxThreads * sizeof(double);
+    return (sharedNodes_ ? 4 * nodes_ * sizeof(double) : 0) + kMaxThreads * sizeof(double) +
+           (neighbourSync_ ? stageBytes_ : 0);
 }
 
 bool DeviceWing::fitsInLds() const { return (4 * nodes_ + kMaxThreads) * sizeof(double) <= sharedNodeBudget(); }
