@@ -212,26 +212,48 @@ __device__ __forceinline__ void put3(double* r, const d3& p) {
     r[2] = p.z;
 }
 
+// Per-item values of a projection that no other item of the iteration
+// changes: indices, inverse masses, constants and the item's own multiplier
+// (only the thread that projects the item writes it). The point-to-point
+// loop loads them before it waits for its dependencies.
+struct DistanceItem {
+    int a, b;
+    double wa, wb, rest, compliance, lambda;
+    bool cable;
+};
+
+__device__ __forceinline__ DistanceItem distanceItem(const WingView& v, const NodeRefs& N, int k) {
+    DistanceItem it;
+    it.a = v.da[k];
+    it.b = v.db[k];
+    it.wa = N.w[it.a];
+    it.wb = N.w[it.b];
+    it.rest = v.drest[k];
+    it.compliance = v.dcomp[k];
+    it.lambda = v.dlam[k];
+    it.cable = v.dcable[k] != 0;
+    return it;
+}
+
 // `rec` (recording only): the item's input vector is written there first.
-__device__ __forceinline__ void projectDistance(const WingView& v, const NodeRefs& N, int k, double invDt2,
-                                                double* rec = nullptr) {
-    const int a = v.da[k], b = v.db[k];
-    d3 pa = N.get(a), pb = N.get(b);
+__device__ __forceinline__ void projectDistance(const WingView& v, const NodeRefs& N, int k, const DistanceItem& it,
+                                                double invDt2, double* rec) {
+    d3 pa = N.get(it.a), pb = N.get(it.b);
     if (rec) {
         put3(rec, pa);
         put3(rec + 3, pb);
-        rec[6] = v.dlam[k];
+        rec[6] = it.lambda;
     }
-    const double wa = N.w[a], wb = N.w[b];
+    const double wa = it.wa, wb = it.wb;
     const d3 diff = sub(pb, pa);
     const double length = len(diff);
     if (length <= 1.0e-12) return;
-    const double c = length - v.drest[k];
-    const bool cable = v.dcable[k] != 0;
-    double lambda = v.dlam[k];
+    const double c = length - it.rest;
+    const bool cable = it.cable;
+    double lambda = it.lambda;
     if (cable && c <= 0.0 && lambda == 0.0) return;
     const double wsum = wa + wb;
-    const double alpha = v.dcomp[k] * invDt2;
+    const double alpha = it.compliance * invDt2;
     if (wsum + alpha <= 0.0) return;
     const double dl = (-c - alpha * lambda) / (wsum + alpha);
     double applied = dl;
@@ -247,8 +269,13 @@ __device__ __forceinline__ void projectDistance(const WingView& v, const NodeRef
     const double sa = wa * s, sb = wb * s;
     pa = sub(pa, scale(diff, sa));
     pb = add(pb, scale(diff, sb));
-    N.put(a, pa);
-    N.put(b, pb);
+    N.put(it.a, pa);
+    N.put(it.b, pb);
+}
+
+__device__ __forceinline__ void projectDistance(const WingView& v, const NodeRefs& N, int k, double invDt2,
+                                                double* rec = nullptr) {
+    projectDistance(v, N, k, distanceItem(v, N, k), invDt2, rec);
 }
 
 // Membrane strain and its gradients (soft_body.cpp:204-282, StrainOnly).
@@ -280,23 +307,63 @@ __device__ __forceinline__ MembraneEval evaluateMembrane(d3 p0, d3 p1, d3 p2, do
 }
 
 // One membrane element (soft_body.cpp:1822-1929).
-__device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRefs& N, int k, const StepArgs& a,
-                                                double* rec = nullptr) {
-    const int i0 = v.ma[k], i1 = v.mb[k], i2 = v.mc[k];
+struct MembraneItem {
+    int i0, i1, i2;
+    double w[3];
+    double preW, preT, m00, m01, m10, m11, areaScale;
+    double cxx, cyy, czz, cxy, cxz, cyz;
+    double damping;
+    d3 lambda;
+    d3 prev[3];  // start-of-substep positions (damped membranes only)
+};
+
+__device__ __forceinline__ MembraneItem membraneItem(const WingView& v, const NodeRefs& N, int k) {
+    MembraneItem it;
+    it.i0 = v.ma[k];
+    it.i1 = v.mb[k];
+    it.i2 = v.mc[k];
+    it.w[0] = N.w[it.i0];
+    it.w[1] = N.w[it.i1];
+    it.w[2] = N.w[it.i2];
+    it.preW = v.anyPreTension ? v.preW[k] : 0.0;
+    it.preT = v.anyPreTension ? v.preT[k] : 0.0;
+    it.m00 = v.m00[k];
+    it.m01 = v.m01[k];
+    it.m10 = v.m10[k];
+    it.m11 = v.m11[k];
+    it.areaScale = v.mscale[k];
+    it.cxx = v.cxx[k];
+    it.cyy = v.cyy[k];
+    it.czz = v.czz[k];
+    it.cxy = v.cxy[k];
+    it.cxz = v.cxz[k];
+    it.cyz = v.cyz[k];
+    it.lambda = d3{v.mlx[k], v.mly[k], v.mlz[k]};
+    it.damping = v.anyDamping ? v.mdamp[k] : 0.0;
+    if (v.anyDamping) {
+        const int idx[3] = {it.i0, it.i1, it.i2};
+#pragma unroll
+        for (int corner = 0; corner < 3; ++corner)
+            it.prev[corner] = d3{v.qx[idx[corner]], v.qy[idx[corner]], v.qz[idx[corner]]};
+    }
+    return it;
+}
+
+__device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRefs& N, int k, const MembraneItem& it,
+                                                const StepArgs& a, double* rec) {
+    const int i0 = it.i0, i1 = it.i1, i2 = it.i2;
     const d3 p[3] = {N.get(i0), N.get(i1), N.get(i2)};
     if (rec) {
         put3(rec, p[0]);
         put3(rec + 3, p[1]);
         put3(rec + 6, p[2]);
-        rec[9] = v.mlx[k];
-        rec[10] = v.mly[k];
-        rec[11] = v.mlz[k];
+        rec[9] = it.lambda.x;
+        rec[10] = it.lambda.y;
+        rec[11] = it.lambda.z;
     }
-    const double w[3] = {N.w[i0], N.w[i1], N.w[i2]};
-    const double preW = v.anyPreTension ? v.preW[k] : 0.0;
-    const double preT = v.anyPreTension ? v.preT[k] : 0.0;
-    const MembraneEval e = evaluateMembrane(p[0], p[1], p[2], v.m00[k], v.m01[k], v.m10[k], v.m11[k], preW, preT);
-    const double areaScale = v.mscale[k];
+    const double* w = it.w;
+    const MembraneEval e = evaluateMembrane(p[0], p[1], p[2], it.m00, it.m01, it.m10, it.m11, it.preW, it.preT);
+    const double areaScale = it.areaScale;
     d3 J[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -313,20 +380,19 @@ __device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRef
         M.xz += wc * dot(J[0][corner], J[2][corner]);
         M.yz += wc * dot(J[1][corner], J[2][corner]);
     }
-    const s3 at{a.invDt2 * v.cxx[k], a.invDt2 * v.cyy[k], a.invDt2 * v.czz[k],
-                a.invDt2 * v.cxy[k], a.invDt2 * v.cxz[k], a.invDt2 * v.cyz[k]};
-    const d3 lambda{v.mlx[k], v.mly[k], v.mlz[k]};
+    const s3 at{a.invDt2 * it.cxx, a.invDt2 * it.cyy, a.invDt2 * it.czz,
+                a.invDt2 * it.cxy, a.invDt2 * it.cxz, a.invDt2 * it.cyz};
+    const d3 lambda = it.lambda;
     const d3 cv = scale(e.strain, areaScale);
     const d3 atl = mul(at, lambda);
     s3 S;
     d3 rhs;
     if (v.anyDamping) {
-        const double gamma = v.mdamp[k] / a.dt;
+        const double gamma = it.damping / a.dt;
         d3 dm{0.0, 0.0, 0.0};
-        const int idx[3] = {i0, i1, i2};
 #pragma unroll
         for (int corner = 0; corner < 3; ++corner) {
-            const d3 disp = sub(p[corner], d3{v.qx[idx[corner]], v.qy[idx[corner]], v.qz[idx[corner]]});
+            const d3 disp = sub(p[corner], it.prev[corner]);
             dm.x += dot(J[0][corner], disp);
             dm.y += dot(J[1][corner], disp);
             dm.z += dot(J[2][corner], disp);
@@ -360,17 +426,42 @@ __device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRef
     N.put(i2, add(p[2], corr[2]));
 }
 
+__device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRefs& N, int k, const StepArgs& a,
+                                                double* rec = nullptr) {
+    projectMembrane(v, N, k, membraneItem(v, N, k), a, rec);
+}
+
+struct HingeItem {
+    int idx[4];
+    double w[4];
+    double compliance, rest, lambda;
+};
+
+__device__ __forceinline__ HingeItem hingeItem(const WingView& v, const NodeRefs& N, int k) {
+    HingeItem it;
+    it.idx[0] = v.ha[k];
+    it.idx[1] = v.hb[k];
+    it.idx[2] = v.hc[k];
+    it.idx[3] = v.hd[k];
+#pragma unroll
+    for (int corner = 0; corner < 4; ++corner) it.w[corner] = N.w[it.idx[corner]];
+    it.compliance = v.hcomp[k];
+    it.rest = v.hrest[k];
+    it.lambda = v.hlam[k];
+    return it;
+}
+
 // Dihedral hinge (soft_body.cpp:1648-1711 and 1754-1792).
-__device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& N, int k, double invDt2,
-                                             double* rec = nullptr) {
-    const int idx[4] = {v.ha[k], v.hb[k], v.hc[k], v.hd[k]};
+__device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& N, int k, const HingeItem& it,
+                                             double invDt2, double* rec) {
+    const int* idx = it.idx;
     const d3 a = N.get(idx[0]), b = N.get(idx[1]), c = N.get(idx[2]), d = N.get(idx[3]);
     if (rec) {
         put3(rec, a);
         put3(rec + 3, b);
         put3(rec + 6, c);
         put3(rec + 9, d);
-        rec[12] = v.hlam[k];
+        rec[12] = it.lambda;
     }
     const d3 edge = sub(b, a);
     const d3 toC = sub(c, a);
@@ -400,23 +491,25 @@ __device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& 
     const d3 g[4] = {scale(add(add(edgeAdj, cAdj), dAdj), -1.0), edgeAdj, cAdj, dAdj};
     if (!isfinite(angle) || !fin3(g[0]) || !fin3(g[1]) || !fin3(g[2]) || !fin3(g[3])) return;
     double wgrad = 0.0;
-    double w[4];
+    const double* w = it.w;
 #pragma unroll
-    for (int corner = 0; corner < 4; ++corner) {
-        w[corner] = N.w[idx[corner]];
-        wgrad += w[corner] * dot(g[corner], g[corner]);
-    }
-    const double alpha = v.hcomp[k] * invDt2;
+    for (int corner = 0; corner < 4; ++corner) wgrad += w[corner] * dot(g[corner], g[corner]);
+    const double alpha = it.compliance * invDt2;
     const double den = wgrad + alpha;
     if (!(den > 0.0) || !isfinite(den)) return;
-    const double value = remainder(angle - v.hrest[k], 6.283185307179586476925286766559);
-    const double lambda = v.hlam[k];
+    const double value = remainder(angle - it.rest, 6.283185307179586476925286766559);
+    const double lambda = it.lambda;
     const double dl = (-value - alpha * lambda) / den;
     if (!isfinite(dl)) return;
     const d3 p[4] = {a, b, c, d};
 #pragma unroll
     for (int corner = 0; corner < 4; ++corner) N.put(idx[corner], add(p[corner], scale(g[corner], w[corner] * dl)));
     v.hlam[k] = lambda + dl;
+}
+
+__device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& N, int k, double invDt2,
+                                             double* rec = nullptr) {
+    projectHinge(v, N, k, hingeItem(v, N, k), invDt2, rec);
 }
 
 // Payload state held in registers by the lane running a chain of
@@ -940,6 +1033,20 @@ __device__ __forceinline__ void waitStagedDeps(const WingView& v, const StagedPr
     __syncthreads();
 }
 
+// A phase of the point-to-point loop whose first item's fixed values are
+// loaded before the dependency wait, so that load overlaps the wait and the
+// projection after it waits only on node positions.
+template <class Item, class Load, class Project>
+__device__ __forceinline__ void prefetchedPhase(const WingView& v, const StagedProgram& S, int ph, int step,
+                                                int first, int end, int stride, Load load, Project project) {
+    Item item{};
+    const bool have = first < end;
+    if (have) item = load(first);
+    waitStagedDeps(v, S, ph, step);
+    if (have) project(first, item);
+    for (int k = first + stride; k < end; k += stride) project(k, load(k));
+}
+
 // Publishes `value` on a progress word after releasing this workgroup's
 // earlier writes (call after a workgroup barrier).
 __device__ __forceinline__ void publishCounter(unsigned* counter, unsigned value) {
@@ -1249,9 +1356,38 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                 }
                 if (b >= e) continue;
                 const int step = it * m + ph;
-                waitStagedDeps(v, S, ph, step);
-                runPhase(v, N, P, b + static_cast<int>(threadIdx.x), e, static_cast<int>(blockDim.x), payload, a,
-                         trace ? trace + layout.iterBase + it * layout.iterStride : nullptr, layout);
+                double* rec = trace ? trace + layout.iterBase + it * layout.iterStride : nullptr;
+                const int first = b + static_cast<int>(threadIdx.x), stride = static_cast<int>(blockDim.x);
+                switch (P.type) {
+                case kPhaseDistance:
+                    prefetchedPhase<DistanceItem>(
+                        v, S, ph, step, first, e, stride, [&](int k) { return distanceItem(v, N, k); },
+                        [&](int k, const DistanceItem& item) {
+                            projectDistance(v, N, k, item, a.invDt2,
+                                            rec ? rec + layout.distBase + kRecDist * static_cast<long>(k) : nullptr);
+                        });
+                    break;
+                case kPhaseMembrane:
+                    prefetchedPhase<MembraneItem>(
+                        v, S, ph, step, first, e, stride, [&](int k) { return membraneItem(v, N, k); },
+                        [&](int k, const MembraneItem& item) {
+                            projectMembrane(v, N, k, item, a,
+                                            rec ? rec + layout.membBase + kRecMemb * static_cast<long>(k) : nullptr);
+                        });
+                    break;
+                case kPhaseHinge:
+                    prefetchedPhase<HingeItem>(
+                        v, S, ph, step, first, e, stride, [&](int k) { return hingeItem(v, N, k); },
+                        [&](int k, const HingeItem& item) {
+                            projectHinge(v, N, k, item, a.invDt2,
+                                         rec ? rec + layout.hingeBase + kRecHinge * static_cast<long>(k) : nullptr);
+                        });
+                    break;
+                default:
+                    waitStagedDeps(v, S, ph, step);
+                    runPhase(v, N, P, first, e, stride, payload, a, rec, layout);
+                    break;
+                }
                 __syncthreads();
                 publishCounter(v.flags + 32 * p, static_cast<unsigned>(step + 1));
             }
