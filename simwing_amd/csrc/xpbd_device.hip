@@ -1158,6 +1158,30 @@ __device__ __forceinline__ void atomicMaxNonNegative(double* address, double val
     atomicMax(reinterpret_cast<unsigned long long*>(address), static_cast<unsigned long long>(__double_as_longlong(value)));
 }
 
+// Items [first, end) of a parallel level of suspension segments.
+__device__ __forceinline__ void segmentLevel(const WingView& v, const NodeRefs& N, int first, int end, int stride,
+                                             PayloadDev& payload, const StepArgs& a, double* rec,
+                                             const TraceLayout& L) {
+    for (int k = first; k < end; k += stride) {
+        Payload state = loadPayload(payload);
+        const m3 iinv = loadInverseInertia(payload);
+        const int fe = v.sfrom[k], te = v.sto[k];
+        double lambda = v.slam[k];
+        if (rec)
+            recordSegment(rec + L.segBase + kRecSeg * static_cast<long>(k), fe >= 0 ? N.get(fe) : d3{0.0, 0.0, 0.0},
+                          te >= 0 ? N.get(te) : d3{0.0, 0.0, 0.0}, lambda, v.srest[k], state, iinv);
+        if (!projectSegment(v, N, fe, te, v.srest[k], v.sstiff[k], lambda, v.sruntime[k], state, payload, iinv,
+                            finiteM(iinv), a))
+            break;
+        v.slam[k] = lambda;
+        if (fe < 0 || te < 0) {
+            storePayload(payload, state);
+            if (v.slastPayload[k] && !refreshInverseInertia(payload, state.q))
+                raise(v, SW_SUSPENSION_ERROR, 1, 2, 0);
+        }
+    }
+}
+
 // Items [first, end) with the given stride of one phase of the iteration
 // program; a serial run (kPhaseSegmentRun) is staged and projected by the
 // calling workgroup as a whole. `rec`: the iteration's trace records
@@ -1178,24 +1202,7 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
             projectHinge(v, N, k, a.invDt2, rec ? rec + L.hingeBase + kRecHinge * static_cast<long>(k) : nullptr);
         break;
     case kPhaseSegments:
-        for (int k = first; k < end; k += stride) {
-            Payload state = loadPayload(payload);
-            const m3 iinv = loadInverseInertia(payload);
-            const int fe = v.sfrom[k], te = v.sto[k];
-            double lambda = v.slam[k];
-            if (rec)
-                recordSegment(rec + L.segBase + kRecSeg * static_cast<long>(k), fe >= 0 ? N.get(fe) : d3{0.0, 0.0, 0.0},
-                              te >= 0 ? N.get(te) : d3{0.0, 0.0, 0.0}, lambda, v.srest[k], state, iinv);
-            if (!projectSegment(v, N, fe, te, v.srest[k], v.sstiff[k], lambda, v.sruntime[k], state, payload, iinv,
-                                finiteM(iinv), a))
-                break;
-            v.slam[k] = lambda;
-            if (fe < 0 || te < 0) {
-                storePayload(payload, state);
-                if (v.slastPayload[k] && !refreshInverseInertia(payload, state.q))
-                    raise(v, SW_SUSPENSION_ERROR, 1, 2, 0);
-            }
-        }
+        segmentLevel(v, N, first, end, stride, payload, a, rec, L);
         break;
     case kPhaseSegmentRun:
         runChain(v, N, P, payload, a, rec ? rec + L.segBase : nullptr);
@@ -1343,13 +1350,29 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         for (int e = threadIdx.x; e < depCount; e += blockDim.x)
             S.dep[e] = int2{v.depPart[depBase + e], v.depLag[depBase + e]};
         __syncthreads();
-        for (int it = 0; it < a.iterations; ++it) {
+        if (chainGroup) {
+            // The dedicated chain workgroup owns only the serial runs: a loop
+            // of its own, away from the code of the other phases.
+            for (int it = 0; it < a.iterations; ++it) {
+                double* rec = trace ? trace + layout.iterBase + it * layout.iterStride : nullptr;
+                for (int ph = 0; ph < m; ++ph) {
+                    const Phase P = S.phase[ph];
+                    if (P.type != kPhaseSegmentRun) continue;
+                    const int step = it * m + ph;
+                    waitStagedDeps(v, S, ph, step);
+                    runChain(v, N, P, payload, a, rec ? rec + layout.segBase : nullptr);
+                    __syncthreads();
+                    publishCounter(v.flags + 32 * p, static_cast<unsigned>(step + 1));
+                }
+            }
+        }
+        for (int it = 0; it < (chainGroup ? 0 : a.iterations); ++it) {
             for (int ph = 0; ph < m; ++ph) {
                 const Phase P = S.phase[ph];
                 int b = P.begin, e = P.begin;
                 if (P.type == kPhaseSegmentRun) {
                     if (p == (dedicated ? workers : 0)) e = P.end;
-                } else if (!chainGroup) {
+                } else {
                     const long count = P.end - P.begin;
                     b = P.begin + static_cast<int>(count * p / workers);
                     e = P.begin + static_cast<int>(count * (p + 1) / workers);
@@ -1383,9 +1406,22 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                                          rec ? rec + layout.hingeBase + kRecHinge * static_cast<long>(k) : nullptr);
                         });
                     break;
+                case kPhaseSegments:
+                    waitStagedDeps(v, S, ph, step);
+                    segmentLevel(v, N, first, e, stride, payload, a, rec, layout);
+                    break;
+                case kPhaseSegmentRun:  // no dedicated chain workgroup
+                    waitStagedDeps(v, S, ph, step);
+                    runChain(v, N, P, payload, a, rec ? rec + layout.segBase : nullptr);
+                    break;
+                case kPhaseSpheres:
+                    waitStagedDeps(v, S, ph, step);
+                    for (int k = first; k < e; k += stride)
+                        projectSphere(v, N, k, a.invDt2,
+                                      rec ? rec + layout.sphereBase + kRecSphere * static_cast<long>(k) : nullptr);
+                    break;
                 default:
                     waitStagedDeps(v, S, ph, step);
-                    runPhase(v, N, P, first, e, stride, payload, a, rec, layout);
                     break;
                 }
                 __syncthreads();
