@@ -873,7 +873,7 @@ __device__ __forceinline__ void recordSegment(double* r, const d3& pa, const d3&
 }
 
 // `rec`: the iteration's segment records (recording only).
-__device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, const Phase& P, PayloadDev& payload,
+__device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, const Phase P, PayloadDev& payload,
                                       const StepArgs& a, double* rec) {
     const int slots = P.slotEnd - P.slotBegin;
     const int items = P.end - P.begin;
