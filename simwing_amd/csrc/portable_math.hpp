@@ -16,6 +16,7 @@
 //   atan(t), t in [0, 1]:  atan(t) = atan(c) + atan((t - c) / (1 + t c)),
 //       c = k/64; the reduced argument |u| <= 2^-7 goes through the Maclaurin
 //       series with the u^3 term in double-double.
+//   sin/cos(x), |x| < 2^-9: the Maclaurin series in double (see sincos).
 //   sin/cos(x): x = n pi/2 + r (three-part Cody-Waite constant, exact for
 //       |n| < 2^20), r = j/64 + d, |d| <= 2^-7, and the angle-addition
 //       formulas with tabulated sin(j/64), cos(j/64).
@@ -310,6 +311,22 @@ SWX_PM double acos(double x) {
 SWX_PM void sincos(double x, double* sine, double* cosine) {
     if (!(fabs(x) <= kMaxFinite)) {  // NaN or infinite
         *sine = *cosine = (x - x) / (x - x);
+        return;
+    }
+    if (fabs(x) < 0x1p-9) {
+        // Small arguments (the rotation increments of the payload): the
+        // Maclaurin series in double with one final rounding each,
+        // sin x = x + x p(x^2) by a fused multiply-add and
+        // cos x = 1 + (x^4 q(x^2) - x^2 / 2). The terms after the leading one
+        // are below 2^-19 of it, so the error before the final rounding is
+        // below 2^-72 (relative): correctly rounded except with probability
+        // about 2^-19 per call. sin keeps the sign of zero.
+        const double x2 = x * x;
+        const double p = x2 * (-1.0 / 6.0 + x2 * (1.0 / 120.0 + x2 * (-1.0 / 5040.0 + x2 * (1.0 / 362880.0))));
+        const double q =
+            (x2 * x2) * (1.0 / 24.0 + x2 * (-1.0 / 720.0 + x2 * (1.0 / 40320.0 + x2 * (-1.0 / 3628800.0))));
+        *sine = x == 0.0 ? x : fma(x, p, x);
+        *cosine = 1.0 + (q - 0.5 * x2);
         return;
     }
     // x = n pi/2 + r. Exact reduction needs |n| < 2^20; beyond that the
