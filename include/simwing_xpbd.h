@@ -352,6 +352,20 @@ void sw_scene_destroy(sw_scene* scene);
 /* Per-triangle pressure generated with the scene (ascending triangle id). */
 sw_status sw_scene_face_pressure(const sw_scene* scene, const double** pressure, size_t* count);
 
+/* readScene (scene.cpp:1225-1418): a scene from its compact binary stream
+ * (binary versions 2 and 3, little-endian). Rejections carry readScene's
+ * messages (magic header, binary version, count and string limits,
+ * truncation, trailing data, junction fixed flag) as SW_INVALID_ARGUMENT;
+ * the decoded scene is then checked with the structural subset of
+ * validateScene that sw_scene_assemble applies. A read scene has zero face
+ * pressure. */
+sw_status sw_scene_read(const uint8_t* bytes, size_t size, sw_scene** out);
+
+/* writeScene (scene.cpp:1051-1223): the scene's binary stream (version 3,
+ * every collection in stable-ID order). bytes NULL: only *size is set; a
+ * capacity below *size fails with SW_INVALID_ARGUMENT and *size set. */
+sw_status sw_scene_write(const sw_scene* scene, uint8_t* bytes, size_t capacity, size_t* size);
+
 /* SceneStructureSettings (scene_structure.h:27-39) subset. */
 typedef struct sw_assembly_settings {
     double seam_coincidence_tolerance;

@@ -34,6 +34,7 @@
 // Usage: ref_driver <preset> <out-prefix> <steps> <workers> [snapshot ...]
 //        ref_driver --compare-workers <preset> <steps>
 //        ref_driver --frame <preset> <out-file> <steps>
+//        ref_driver --scene <preset> <out-file>
 #include "fixture_io.hpp"
 #include "oracle_colouring.hpp"
 
@@ -50,7 +51,9 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <memory>
+#include <sstream>
 #include <set>
 #include <string>
 #include <vector>
@@ -689,6 +692,27 @@ int main(int argc, char** argv) {
         }
         if (argc >= 5 && std::strcmp(argv[1], "--frame") == 0)
             return frameMode(argv[2], argv[3], std::atoi(argv[4]));
+        if (argc >= 4 && std::strcmp(argv[1], "--scene") == 0) {
+            // simwing::fsi::writeScene (scene.cpp:1051-1223) of the seeded
+            // wing, and readScene of those bytes back (the stream must
+            // decode and re-encode to itself).
+            const swx::GeneratedWing wing = swx::generateWing(swx::wingPreset(argv[2]));
+            const fsi::Scene scene = toReferenceScene(wing.scene);
+            std::ostringstream out;
+            std::string error;
+            if (!fsi::writeScene(scene, out, &error)) throw std::runtime_error("writeScene: " + error);
+            const std::string bytes = out.str();
+            std::istringstream in(bytes);
+            fsi::Scene decoded;
+            if (!fsi::readScene(in, decoded, &error)) throw std::runtime_error("readScene: " + error);
+            std::ostringstream again;
+            if (!fsi::writeScene(decoded, again, &error) || again.str() != bytes)
+                throw std::runtime_error("scene stream does not round-trip");
+            std::ofstream file(argv[3], std::ios::binary);
+            file.write(bytes.data(), static_cast<std::streamsize>(bytes.size()));
+            std::printf("scene preset=%s bytes=%zu\n", argv[2], bytes.size());
+            return file ? 0 : 1;
+        }
         if (argc >= 5 && std::strcmp(argv[1], "--bench") == 0) {
             // Timing in the softwing-bench manner (tools/softwing_bench.cpp:1783-1801):
             // warm-up steps, then steady_clock around each step until the budget is spent.

@@ -253,14 +253,26 @@ def definition_colouring(definition: DefinitionArrays, family: int):
 
 
 class Scene:
-    """A seeded synthetic wing (scene-v2 entities held by the library)."""
+    """Scene-v2 entities held by the library: a seeded synthetic wing
+    (``Scene(preset)``) or a decoded binary stream (``read_scene``)."""
 
-    def __init__(self, preset: str):
-        lib = native.library()
-        handle = C.c_void_p()
-        check(lib.sw_scene_generate(preset.encode(), C.byref(handle)))
-        self._handle = handle
+    def __init__(self, preset: str | None, _handle=None):
+        if _handle is None:
+            lib = native.library()
+            handle = C.c_void_p()
+            check(lib.sw_scene_generate(preset.encode(), C.byref(handle)))
+            _handle = handle
+        self._handle = _handle
         self.preset = preset
+
+    def to_bytes(self) -> bytes:
+        """writeScene's compact binary stream (sw_scene_write)."""
+        lib = native.library()
+        size = C.c_size_t()
+        check(lib.sw_scene_write(self._handle, None, 0, C.byref(size)))
+        out = np.zeros(size.value, np.uint8)
+        check(lib.sw_scene_write(self._handle, u8p(out), out.size, C.byref(size)))
+        return out.tobytes()
 
     def __del__(self):
         if getattr(self, "_handle", None):
@@ -279,6 +291,14 @@ class Scene:
 
 def generate_scene(preset: str) -> Scene:
     return Scene(preset)
+
+
+def read_scene(data: bytes) -> Scene:
+    """readScene: a scene from its compact binary stream (sw_scene_read)."""
+    buf = np.frombuffer(bytes(data), np.uint8).copy()
+    handle = C.c_void_p()
+    check(native.library().sw_scene_read(u8p(buf), buf.size, C.byref(handle)))
+    return Scene(None, _handle=handle)
 
 
 class Assembly:

@@ -50,6 +50,7 @@ SOURCES = [
     ("assembly.cpp", False),
     ("schedule.cpp", False),
     ("frame.cpp", False),
+    ("scene_io.cpp", False),
     ("wing_generator.cpp", False),
 ]
 

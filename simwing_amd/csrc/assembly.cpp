@@ -496,6 +496,8 @@ void assembleSeams(const std::vector<const Seam*>& seams,
 
 }  // namespace
 
+void validateSceneStructure(const Scene& scene) { validate(scene); }
+
 Assembly assembleScene(const Scene& scene, const AssemblySettings& settings,
                        const std::vector<ContactSphere>& spheres) {
     validate(scene);

@@ -34,4 +34,9 @@ struct Assembly {
 Assembly assembleScene(const Scene& scene, const AssemblySettings& settings,
                        const std::vector<ContactSphere>& spheres = {});
 
+// The scene checks assembleScene applies first (the subset of validateScene,
+// scene.cpp:552-1040, that the structural conversion depends on). Throws
+// std::invalid_argument.
+void validateSceneStructure(const Scene& scene);
+
 }  // namespace swx

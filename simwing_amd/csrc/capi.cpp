@@ -8,6 +8,7 @@
 #include "frame.hpp"
 #include "device_wing.hpp"
 #include "scene.hpp"
+#include "scene_io.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -707,6 +708,28 @@ sw_status sw_scene_generate(const char* preset, sw_scene** out) {
 }
 
 void sw_scene_destroy(sw_scene* scene) { delete scene; }
+
+sw_status sw_scene_read(const uint8_t* bytes, size_t size, sw_scene** out) {
+    return guarded([&] {
+        requirePointer(out, "out");
+        auto s = std::make_unique<sw_scene>();
+        s->wing.scene = swx::readSceneBinary(bytes, size);
+        s->wing.facePressure.assign(s->wing.scene.triangles.size(), 0.0);
+        *out = s.release();
+    });
+}
+
+sw_status sw_scene_write(const sw_scene* scene, uint8_t* bytes, size_t capacity, size_t* size) {
+    return guarded([&] {
+        requirePointer(scene, "scene");
+        requirePointer(size, "size");
+        const std::vector<std::uint8_t> data = swx::writeSceneBinary(scene->wing.scene);
+        *size = data.size();
+        if (!bytes) return;
+        if (capacity < data.size()) throw std::invalid_argument("scene buffer is smaller than the stream");
+        std::copy(data.begin(), data.end(), bytes);
+    });
+}
 
 sw_status sw_scene_face_pressure(const sw_scene* scene, const double** pressure, size_t* count) {
     return guarded([&] {

@@ -138,12 +138,38 @@ struct Line {
     LineRole role = LineRole::Suspension;
 };
 
+// Scene provenance (SceneMetadata, scene.h:52-64); the defaults are the
+// synthetic generator's. Geometry is always SI.
+struct SceneMetadata {
+    std::uint32_t schemaMajor = 2;
+    std::uint32_t schemaMinor = 2;
+    std::uint8_t lengthUnit = 1;  // metre
+    std::uint8_t handedness = 1;  // right-handed
+    std::uint8_t upAxis = 1;      // +z
+    double sourceLengthToMeters = 1.0;
+    std::string designChecksum = "synthetic-wing";
+    std::string exporterVersion = "simwing-amd-generator-1";
+};
+
+// A fluid opening (scene.h:129-145). Carried through the binary codec; the
+// structural path does not read it.
+struct Opening {
+    StableId id = 0;
+    std::vector<StableId> orderedVertexIds;
+    StableId negativeRegion = 0;
+    StableId positiveRegion = 0;
+    std::uint8_t role = 1;  // intake
+    std::vector<std::array<StableId, 3>> capTriangles;
+};
+
 struct Scene {
+    SceneMetadata metadata;
     std::vector<Region> regions;
     std::vector<Vertex> vertices;
     std::vector<FabricMaterial> fabricMaterials;
     std::vector<SeamMaterial> seamMaterials;
     std::vector<Triangle> triangles;
+    std::vector<Opening> openings;
     std::vector<Seam> seams;
     std::vector<LineMaterial> lineMaterials;
     std::vector<Pilot> pilots;

@@ -14,6 +14,9 @@ wings, and writes into tests/golden/:
   loads pending again after the last step) and the diagnostic-frame fields
   simwing::viewer::buildStructureFrame makes of it (tiny: with the frame's
   serialized bytes; small: their SHA-256 digest and size);
+* ``<preset>.scene.bin``      -- ``ref_driver --scene``: the reference's
+  writeScene stream of the seeded wing (tiny: the stream itself; small:
+  ``small.scene.digest.swfx`` with its SHA-256 digest and size);
 * ``export.digest.swfx``      -- for the ~70k-triangle wing: SHA-256 digests
   of every assembly and colouring array, and strided samples plus digests of
   the state after 1 and 240 steps (the full arrays are too large to commit;
@@ -113,6 +116,20 @@ def run_frame(preset: str) -> None:
     swfx.write(str(HERE / f"{preset}.frame.swfx"), frame)
 
 
+def run_scene(preset: str) -> None:
+    WORK.mkdir(parents=True, exist_ok=True)
+    path = WORK / f"{preset}.scene.bin"
+    cmd = [str(DRIVER), "--scene", preset, str(path)]
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    data = np.frombuffer(path.read_bytes(), np.uint8)
+    if preset == "tiny":
+        (HERE / "tiny.scene.bin").write_bytes(data.tobytes())
+    else:
+        swfx.write(str(HERE / f"{preset}.scene.digest.swfx"),
+                   {"digest": digest(data), "size": np.array([data.size], np.uint64)})
+
+
 def main(argv: list[str]) -> int:
     if not DRIVER.exists():
         print(f"{DRIVER} is missing: run make -C oracle ref (needs /root/reference)", file=sys.stderr)
@@ -121,6 +138,7 @@ def main(argv: list[str]) -> int:
         run(preset)
         if preset in FRAME_STEPS:
             run_frame(preset)
+            run_scene(preset)
     return 0
 
 
