@@ -308,6 +308,15 @@ private:
     std::size_t marksUsed_ = 0;
     std::vector<std::pair<std::size_t, int>> spans_;  // first mark of a pair, piece (0 replay, 1 jacobian, 2 sweep)
     std::size_t mark();
+    std::size_t mark(hipStream_t stream);
+    // Pipelined reverse pass of a solo run: the replay of the next interval
+    // on the wing's stream, beside the Jacobian passes and sweeps of the
+    // current one on `reverseStream_`, over two trace buffers.
+    bool pipelineFits() const;
+    hipStream_t reverseStream_ = nullptr;
+    double* traceB_ = nullptr;
+    unsigned* barrier_ = nullptr;  // the sweep's own grid barrier words
+    hipEvent_t replayDone_[2] = {nullptr, nullptr}, sweepDone_[2] = {nullptr, nullptr};
 };
 
 // The adjoint of many wings stepped together: the forward pass and the
