@@ -1056,6 +1056,47 @@ __device__ __forceinline__ void publishCounter(unsigned* counter, unsigned value
     }
 }
 
+// A run of consecutive phases of one item type [ph0, ph1) of iteration `it`
+// for participant p of `workers`: each thread's first item of the next phase
+// is loaded (fixed values and own multiplier, as prefetchedPhase does) while
+// this phase's writes drain and its progress is published. A participant
+// with no item in a phase neither waits nor publishes for it.
+template <class Item, class Load, class Project>
+__device__ __forceinline__ void pipelinedRun(const WingView& v, const StagedProgram& S, int it, int ph0, int ph1,
+                                             int p, int workers, Load load, Project project) {
+    const int m = v.iterPhaseCount;
+    const int stride = static_cast<int>(blockDim.x);
+    const auto share = [&](int ph, int& b, int& e) {
+        const Phase& Q = S.phase[ph];
+        const long count = Q.end - Q.begin;
+        b = Q.begin + static_cast<int>(count * p / workers);
+        e = Q.begin + static_cast<int>(count * (p + 1) / workers);
+    };
+    int b, e;
+    share(ph0, b, e);
+    int first = b + static_cast<int>(threadIdx.x);
+    Item cur{};
+    if (first < e) cur = load(first);
+    for (int ph = ph0; ph < ph1; ++ph) {
+        const int step = it * m + ph;
+        if (b < e) {
+            waitStagedDeps(v, S, ph, step);
+            if (first < e) project(first, cur);
+            for (int k = first + stride; k < e; k += stride) project(k, load(k));
+        }
+        int nb = 0, ne = 0;
+        if (ph + 1 < ph1) share(ph + 1, nb, ne);
+        const int nfirst = nb + static_cast<int>(threadIdx.x);
+        Item next{};
+        if (nfirst < ne) next = load(nfirst);
+        if (b < e) {
+            __syncthreads();
+            publishCounter(v.flags + 32 * p, static_cast<unsigned>(step + 1));
+        }
+        b = nb, e = ne, first = nfirst, cur = next;
+    }
+}
+
 // Contact sphere pair (build-defined): node held off the centre node to the
 // radius; the centre is not moved by contact.
 __device__ __forceinline__ void projectSphere(const WingView& v, const NodeRefs& N, int k, double invDt2,
@@ -1367,8 +1408,40 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
             }
         }
         for (int it = 0; it < (chainGroup ? 0 : a.iterations); ++it) {
+            double* const irec = trace ? trace + layout.iterBase + it * layout.iterStride : nullptr;
             for (int ph = 0; ph < m; ++ph) {
                 const Phase P = S.phase[ph];
+                if (P.type == kPhaseDistance || P.type == kPhaseMembrane || P.type == kPhaseHinge) {
+                    int end = ph + 1;
+                    while (end < m && S.phase[end].type == P.type) ++end;
+                    if (P.type == kPhaseDistance) {
+                        pipelinedRun<DistanceItem>(
+                            v, S, it, ph, end, p, workers, [&](int k) { return distanceItem(v, N, k); },
+                            [&](int k, const DistanceItem& item) {
+                                projectDistance(v, N, k, item, a.invDt2,
+                                                irec ? irec + layout.distBase + kRecDist * static_cast<long>(k)
+                                                     : nullptr);
+                            });
+                    } else if (P.type == kPhaseMembrane) {
+                        pipelinedRun<MembraneItem>(
+                            v, S, it, ph, end, p, workers, [&](int k) { return membraneItem(v, N, k); },
+                            [&](int k, const MembraneItem& item) {
+                                projectMembrane(v, N, k, item, a,
+                                                irec ? irec + layout.membBase + kRecMemb * static_cast<long>(k)
+                                                     : nullptr);
+                            });
+                    } else {
+                        pipelinedRun<HingeItem>(
+                            v, S, it, ph, end, p, workers, [&](int k) { return hingeItem(v, N, k); },
+                            [&](int k, const HingeItem& item) {
+                                projectHinge(v, N, k, item, a.invDt2,
+                                             irec ? irec + layout.hingeBase + kRecHinge * static_cast<long>(k)
+                                                  : nullptr);
+                            });
+                    }
+                    ph = end - 1;
+                    continue;
+                }
                 int b = P.begin, e = P.begin;
                 if (P.type == kPhaseSegmentRun) {
                     if (p == (dedicated ? workers : 0)) e = P.end;
