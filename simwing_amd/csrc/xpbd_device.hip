@@ -994,12 +994,13 @@ struct StagedProgram {
     Phase* phase;
     int* offset;
     int2* dep;
+    int2* share;  // this participant's items [x, y) of every phase (forward loop only)
 };
 
 __host__ __device__ constexpr std::size_t alignTo8(std::size_t bytes) { return (bytes + 7) & ~std::size_t{7}; }
 __host__ __device__ constexpr std::size_t stagedProgramBytes(int m, int deps) {
     return alignTo8(static_cast<std::size_t>(m) * sizeof(Phase)) + alignTo8(static_cast<std::size_t>(m + 1) * sizeof(int)) +
-           static_cast<std::size_t>(deps) * sizeof(int2);
+           static_cast<std::size_t>(m + deps) * sizeof(int2);
 }
 constexpr std::size_t kMaxStagedProgramBytes = 48 * 1024;
 
@@ -1007,8 +1008,9 @@ __device__ __forceinline__ StagedProgram stagedProgram(char* base, int m) {
     StagedProgram s;
     s.phase = reinterpret_cast<Phase*>(base);
     s.offset = reinterpret_cast<int*>(base + alignTo8(static_cast<std::size_t>(m) * sizeof(Phase)));
-    s.dep = reinterpret_cast<int2*>(base + alignTo8(static_cast<std::size_t>(m) * sizeof(Phase)) +
-                                    alignTo8(static_cast<std::size_t>(m + 1) * sizeof(int)));
+    s.share = reinterpret_cast<int2*>(base + alignTo8(static_cast<std::size_t>(m) * sizeof(Phase)) +
+                                      alignTo8(static_cast<std::size_t>(m + 1) * sizeof(int)));
+    s.dep = s.share + m;
     return s;
 }
 
@@ -1063,14 +1065,12 @@ __device__ __forceinline__ void publishCounter(unsigned* counter, unsigned value
 // with no item in a phase neither waits nor publishes for it.
 template <class Item, class Load, class Project>
 __device__ __forceinline__ void pipelinedRun(const WingView& v, const StagedProgram& S, int it, int ph0, int ph1,
-                                             int p, int workers, Load load, Project project) {
+                                             int p, Load load, Project project) {
     const int m = v.iterPhaseCount;
     const int stride = static_cast<int>(blockDim.x);
     const auto share = [&](int ph, int& b, int& e) {
-        const Phase& Q = S.phase[ph];
-        const long count = Q.end - Q.begin;
-        b = Q.begin + static_cast<int>(count * p / workers);
-        e = Q.begin + static_cast<int>(count * (p + 1) / workers);
+        const int2 x = S.share[ph];
+        b = x.x, e = x.y;
     };
     int b, e;
     share(ph0, b, e);
@@ -1386,7 +1386,19 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         const StagedProgram S = stagedProgram(reinterpret_cast<char*>(lds + kMaxThreads), m);
         const int depBase = v.depOffsets[p * m];
         const int depCount = v.depOffsets[p * m + m] - depBase;
-        for (int k = threadIdx.x; k < m; k += blockDim.x) S.phase[k] = v.iterPhases[k];
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            const Phase Q = v.iterPhases[k];
+            S.phase[k] = Q;
+            int b = Q.begin, e = Q.begin;
+            if (Q.type == kPhaseSegmentRun) {
+                if (p == (dedicated ? workers : 0)) e = Q.end;
+            } else {
+                const long count = Q.end - Q.begin;
+                b = Q.begin + static_cast<int>(count * p / workers);
+                e = Q.begin + static_cast<int>(count * (p + 1) / workers);
+            }
+            S.share[k] = int2{b, e};
+        }
         for (int k = threadIdx.x; k <= m; k += blockDim.x) S.offset[k] = v.depOffsets[p * m + k] - depBase;
         for (int e = threadIdx.x; e < depCount; e += blockDim.x)
             S.dep[e] = int2{v.depPart[depBase + e], v.depLag[depBase + e]};
@@ -1416,7 +1428,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     while (end < m && S.phase[end].type == P.type) ++end;
                     if (P.type == kPhaseDistance) {
                         pipelinedRun<DistanceItem>(
-                            v, S, it, ph, end, p, workers, [&](int k) { return distanceItem(v, N, k); },
+                            v, S, it, ph, end, p, [&](int k) { return distanceItem(v, N, k); },
                             [&](int k, const DistanceItem& item) {
                                 projectDistance(v, N, k, item, a.invDt2,
                                                 irec ? irec + layout.distBase + kRecDist * static_cast<long>(k)
@@ -1424,7 +1436,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                             });
                     } else if (P.type == kPhaseMembrane) {
                         pipelinedRun<MembraneItem>(
-                            v, S, it, ph, end, p, workers, [&](int k) { return membraneItem(v, N, k); },
+                            v, S, it, ph, end, p, [&](int k) { return membraneItem(v, N, k); },
                             [&](int k, const MembraneItem& item) {
                                 projectMembrane(v, N, k, item, a,
                                                 irec ? irec + layout.membBase + kRecMemb * static_cast<long>(k)
@@ -1432,7 +1444,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                             });
                     } else {
                         pipelinedRun<HingeItem>(
-                            v, S, it, ph, end, p, workers, [&](int k) { return hingeItem(v, N, k); },
+                            v, S, it, ph, end, p, [&](int k) { return hingeItem(v, N, k); },
                             [&](int k, const HingeItem& item) {
                                 projectHinge(v, N, k, item, a.invDt2,
                                              irec ? irec + layout.hingeBase + kRecHinge * static_cast<long>(k)
@@ -1442,14 +1454,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     ph = end - 1;
                     continue;
                 }
-                int b = P.begin, e = P.begin;
-                if (P.type == kPhaseSegmentRun) {
-                    if (p == (dedicated ? workers : 0)) e = P.end;
-                } else {
-                    const long count = P.end - P.begin;
-                    b = P.begin + static_cast<int>(count * p / workers);
-                    e = P.begin + static_cast<int>(count * (p + 1) / workers);
-                }
+                const int b = S.share[ph].x, e = S.share[ph].y;
                 if (b >= e) continue;
                 const int step = it * m + ph;
                 double* rec = trace ? trace + layout.iterBase + it * layout.iterStride : nullptr;
