@@ -452,6 +452,22 @@ __device__ __forceinline__ HingeItem hingeItem(const WingView& v, const NodeRefs
 }
 
 // Dihedral hinge (soft_body.cpp:1648-1711 and 1754-1792).
+// remainder(x, 2π) (the double nearest 2π): for |x| < 3π the quotient is
+// -1, 0 or 1, and |x| - 2π is exact (Sterbenz), so the result equals the
+// library's, ties included (|x| = π keeps n = 0, the even quotient) and a
+// zero result carrying the sign of x; other arguments (and NaN) take the
+// library call.
+__device__ __forceinline__ double remainderTwoPi(double x) {
+    constexpr double y = 6.283185307179586476925286766559;
+    const double ax = fabs(x);
+    if (ax <= 0.5 * y) return x;
+    if (ax < 1.5 * y) {
+        const double r = ax - y;
+        return x < 0.0 ? -r : r;
+    }
+    return remainder(x, y);
+}
+
 __device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& N, int k, const HingeItem& it,
                                              double invDt2, double* rec) {
     const int* idx = it.idx;
@@ -497,7 +513,7 @@ __device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& 
     const double alpha = it.compliance * invDt2;
     const double den = wgrad + alpha;
     if (!(den > 0.0) || !isfinite(den)) return;
-    const double value = remainder(angle - it.rest, 6.283185307179586476925286766559);
+    const double value = remainderTwoPi(angle - it.rest);
     const double lambda = it.lambda;
     const double dl = (-value - alpha * lambda) / den;
     if (!isfinite(dl)) return;
@@ -3024,7 +3040,7 @@ __global__ void mathKernel(int function, const double* a, const double* b, doubl
     case 0: r = sqrt(x); break;
     case 1: r = x / y; break;
     case 2: r = pm::atan2(x, y); break;
-    case 3: r = remainder(x, 6.283185307179586476925286766559); break;
+    case 3: r = remainderTwoPi(x); break;
     case 4: r = pm::sin(x); break;
     case 5: r = pm::cos(x); break;
     case 6: r = pm::acos(x); break;

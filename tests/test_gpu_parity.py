@@ -86,8 +86,21 @@ def test_device_math_conformance():
     assert bits_equal(device(1, a, b), a / b), "division is not correctly rounded on the device"
     angle = rng.uniform(-math.pi, math.pi, n)
     r = 10.0 ** rng.uniform(-3, 3, n)
-    remainder = np.array([libm.remainder(v, 6.283185307179586) for v in angle * 3.0])
-    assert bits_equal(device(3, angle * 3.0, angle), remainder)
+    # Quotients -1, 0, 1 and beyond, the ties at |x| = pi and 3 pi (half and
+    # three halves of the double 2 pi), their neighbours, signed zeros and
+    # non-finite values.
+    two_pi = 6.283185307179586
+    edges = [0.5 * two_pi, 1.5 * two_pi, two_pi, 0.0, 1e300, 1e-300]
+    edges = [e2 for e in edges for e1 in (e, np.nextafter(e, 0.0), np.nextafter(e, np.inf)) for e2 in (e1, -e1)]
+    edges += [np.inf, -np.inf, np.nan]
+    x = np.concatenate([angle * 3.0, angle * 1.5, np.array(edges)])
+    remainder = np.array([libm.remainder(v, two_pi) for v in x])
+    got = np.empty_like(x)
+    native.check(native.library().sw_device_math(0, 3, native.f64p(x), native.f64p(x), native.f64p(got), x.size))
+    finite = np.isfinite(remainder)
+    assert np.array_equal(np.isnan(got), ~finite)  # inf and nan give nan (its sign bit is not compared)
+    wrong = np.flatnonzero(got[finite].view(np.uint64) != remainder[finite].view(np.uint64))
+    assert wrong.size == 0, [(x[finite][i], got[finite][i], remainder[finite][i]) for i in wrong[:8]]
     half = rng.standard_normal(n) * 10.0 ** rng.integers(-9, 1, n)
     near_one = 1.0 - np.abs(rng.standard_normal(n)) * 10.0 ** rng.integers(-16, 0, n)
     probes = {2: ("atan2", np.sin(angle) * r, np.cos(angle) * r), 4: ("sin", half, half), 5: ("cos", half, half),
