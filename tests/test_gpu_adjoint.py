@@ -118,6 +118,32 @@ def test_adjoint_leaves_structure_and_repeats():
     fresh.close()
 
 
+def test_adjoint_sequential_and_pipelined_reverse_passes_agree():
+    """Three checkpoint intervals, so the pipelined reverse pass reuses its
+    first trace buffer. 200 sweep workgroups fit beside the forward grid
+    (pipelined), 250 do not (the intervals are replayed and reversed in
+    sequence over one buffer); both agree with the default launch to
+    rounding (sphere-centre adjoints are accumulated atomically, so other
+    partitions may differ in the last bits) and each repeats bit for bit."""
+    d, s, adj, pressure, rest = setup("tiny", 6, 2)
+    target = np.asarray(d.arrays["node_position"]).reshape(-1, 3) + 0.01
+    runs = {}
+    for workgroups in (None, 200, 250):
+        if workgroups is not None:
+            adj.set_workgroups(workgroups)
+        first, second = adj.run(target), adj.run(target)
+        for key in FAMILIES:
+            assert np.array_equal(first[key], second[key]), (workgroups, key)
+        runs[workgroups] = first
+    for workgroups in (200, 250):
+        for key in FAMILIES:
+            a, b = runs[None][key], runs[workgroups][key]
+            assert np.all(np.isfinite(b)), (workgroups, key)
+            assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(a)), (workgroups, key)
+    adj.close()
+    s.close()
+
+
 def test_adjoint_matches_device_differences_tiny():
     """Directional derivatives against central differences of the device
     forward pass (h = 1e-5): the pressure and rest-length families are far
