@@ -644,8 +644,10 @@ __device__ bool projectSegment(const WingView& v, const Store& N, int fe, int te
         if (e.fixed) continue;
         const d3 step = scale(grads[s], e.invMass * increment);
         if (ids[s] >= 0) {
-            const d3 p = N.get(ids[s]);
-            N.put(ids[s], add(p, step));
+            // The endpoint's position as read above: the two endpoints are
+            // distinct nodes (a zero length was rejected), so the first
+            // correction's store does not change the second's node.
+            N.put(ids[s], add(e.position, step));
         } else {
             P.com = add(P.com, step);
             const d3 rotation = scale(mulM(iinv, cross(e.offset, grads[s])), increment);
