@@ -1,0 +1,63 @@
+"""The CPU oracle against the reference's own run of the ~70k-triangle wing
+(CPU only).
+
+`export.digest.swfx` holds SHA-256 digests of every state array the
+reference's solver (src/softwing + src/fsi, compiled by oracle/Makefile and
+run by oracle/ref_driver) produced after 1 and 240 flight steps of the
+export-scale wing (43,180 nodes, 70,940 membranes, 98,689 hinges, 300 line
+segments, rigid pilot, no contact spheres: the reference has no sphere
+family). The oracle in its C-library transcendental mode must reproduce
+every digest: positions, previous positions, velocities, every multiplier,
+membrane residuals, the line multipliers and the payload.
+
+Step 1 runs in the default CPU suite (about 2 s). The 240-step check takes
+about 5 minutes of one core, so it runs when SIMWING_SLOW_TESTS=1 is set:
+
+    SIMWING_SLOW_TESTS=1 python -m pytest tests/test_oracle_export_reference.py
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import simwing_amd as sw
+from helpers import flight_case
+
+DIGEST_KEYS = ("position", "previous", "velocity", "membrane_residual", "membrane_lambda", "dihedral_lambda",
+               "constraint_lambda")
+SLOW = os.environ.get("SIMWING_SLOW_TESTS") == "1"
+
+
+def digest(array: np.ndarray) -> np.ndarray:
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(array).tobytes()).digest(), np.uint64)
+
+
+def check_step(reference: dict, state: dict, hinge_order: np.ndarray, step: int) -> None:
+    for key in DIGEST_KEYS:
+        value = state[key][hinge_order] if key == "dihedral_lambda" else state[key]
+        assert np.array_equal(digest(value), reference[f"step{step}/{key}/digest"]), \
+            f"export step {step}: {key} differs from the reference"
+    for key in ("segment_lambda", "payload", "suspension_residual"):
+        got = np.ascontiguousarray(state[key], np.float64).view(np.uint64)
+        want = np.ascontiguousarray(reference[f"step{step}/{key}"], np.float64).view(np.uint64)
+        assert np.array_equal(got, want), f"export step {step}: {key} differs from the reference"
+
+
+@pytest.mark.parametrize("steps", [pytest.param((1,), id="step1"),
+                                   pytest.param((1, 240), id="step240", marks=pytest.mark.skipif(
+                                       not SLOW, reason="240 export steps take ~5 min; set SIMWING_SLOW_TESTS=1"))])
+def test_oracle_reproduces_reference_export_digests(golden, steps):
+    reference = golden("export.digest.swfx")
+    definition, pressure, loads = flight_case("export", strip_spheres=True)
+    hinge_order, _ = sw.definition_colouring(definition, 2)
+    assert np.array_equal(digest(hinge_order.astype(np.uint64)), reference["dihedral_colour_order_digest"])
+    body = orc.OracleBody(definition)  # C-library transcendentals, as the reference
+    body.set_face_pressure(pressure)
+    settings = sw.flight_step_settings()
+    for step in range(1, max(steps) + 1):
+        body.set_external_forces(loads)
+        body.step(settings)
+        if step in steps:
+            check_step(reference, body.state(), hinge_order.astype(np.int64), step)
