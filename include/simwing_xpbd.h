@@ -230,6 +230,23 @@ sw_status sw_structure_payload_state(const sw_structure* s, double* payload13);
 sw_status sw_structure_frame_fields(const sw_structure* s, double* vertex_position, double* vertex_velocity,
                                     double* vertex_pending_force, double* line_length, double* line_violation);
 
+/* The same fields as sw_structure_frame_fields, from an entry point that
+ * allocates nothing: no hipMalloc, hipFree or other allocating routine on
+ * any call. No counterpart in the reference. It is for callers that cannot
+ * allocate device memory: a frame publisher inside a tight stepping loop,
+ * code under HIP graph capture, hosts that forbid allocation after set-up.
+ * The fields are staged in a static device array of 2,097,152 doubles
+ * (16 MiB per GPU, held until the process ends); a structure needs
+ * 6 (N + H) + 2 (C + S) doubles of it, and a larger one is rejected with
+ * SW_INVALID_ARGUMENT. The array's address on a device is looked up on the
+ * first call on that device. All calls on one GPU share the array: calls
+ * that overlap on one GPU race and must be serialised by the caller. Every
+ * staged value is written before it is read, so nothing carries over from
+ * an earlier call. Results equal sw_structure_frame_fields bit for bit. */
+sw_status sw_structure_frame_fields_noalloc(const sw_structure* s, double* vertex_position,
+                                            double* vertex_velocity, double* vertex_pending_force,
+                                            double* line_length, double* line_violation);
+
 /* Stable identity of a frame: StructureFrameMappingDefinition
  * (structure_frame.h:35-39). vertex_count = N + H, triangle_count = T,
  * line_count = C + S. Ids nonzero and unique per kind, side regions

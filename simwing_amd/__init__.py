@@ -477,10 +477,12 @@ class Structure:
     def definition(self) -> DefinitionArrays:
         return self._definition
 
-    def frame_fields(self) -> dict:
-        """Diagnostic-frame fields (sw_structure_frame_fields): vertices are
-        the nodes then the harness points, lines the constraints then the
-        suspension segments (definition order)."""
+    def frame_fields(self, noalloc: bool = False) -> dict:
+        """Diagnostic-frame fields (sw_structure_frame_fields, or with
+        noalloc=True sw_structure_frame_fields_noalloc, which stages them in
+        the library's static device array): vertices are the nodes then the
+        harness points, lines the constraints then the suspension segments
+        (definition order)."""
         d = self._definition
         h = 0 if d.suspension is None else int(np.asarray(d.suspension["harness_ids"]).size)
         lines = int(d.arrays["constraint_kind"].size) + (
@@ -489,7 +491,8 @@ class Structure:
         out = {"vertex_position": np.zeros(3 * v), "vertex_velocity": np.zeros(3 * v),
                "vertex_pending_force": np.zeros(3 * v), "line_length": np.zeros(lines),
                "line_violation": np.zeros(lines)}
-        check(native.library().sw_structure_frame_fields(self._handle, *[f64p(out[k]) for k in (
+        entry = "sw_structure_frame_fields_noalloc" if noalloc else "sw_structure_frame_fields"
+        check(getattr(native.library(), entry)(self._handle, *[f64p(out[k]) for k in (
             "vertex_position", "vertex_velocity", "vertex_pending_force", "line_length", "line_violation")]))
         for k in ("vertex_position", "vertex_velocity", "vertex_pending_force"):
             out[k] = out[k].reshape(-1, 3)

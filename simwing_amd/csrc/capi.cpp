@@ -92,6 +92,7 @@ struct sw_structure {
     swx::Definition definition;
     std::uint64_t fingerprint = 0;
     std::unique_ptr<swx::DeviceWing> wing;
+    int device = 0;
     std::vector<double> pending;
     bool pendingDirty = true;
     double lastApplied[3] = {0.0, 0.0, 0.0};
@@ -286,6 +287,7 @@ sw_status sw_structure_create(const sw_definition* definition, int device, sw_st
             s->freeMembrane = !d.suspension && (d.membraneCount() > 0 || d.dihedralCount() > 0) && allDynamic;
         }
         s->wing = std::make_unique<swx::DeviceWing>(s->definition, device);
+        s->device = device;
         s->pending.assign(3 * s->definition.nodeCount(), 0.0);
         if (s->definition.suspension) {
             const auto& layout = s->wing->suspension();
@@ -413,6 +415,27 @@ sw_status sw_structure_frame_fields(const sw_structure* s, double* vertex_positi
         }
         if (line_length) std::copy(f.lineLength.begin(), f.lineLength.end(), line_length);
         if (line_violation) std::copy(f.lineViolation.begin(), f.lineViolation.end(), line_violation);
+    });
+}
+
+sw_status sw_structure_frame_fields_noalloc(const sw_structure* s, double* vertex_position,
+                                            double* vertex_velocity, double* vertex_pending_force,
+                                            double* line_length, double* line_violation) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        const std::size_t doubles = s->wing->frameFieldsDoubles();
+        if (doubles > swx::kFrameScratchDoubles)
+            throw std::invalid_argument("frame fields of this structure (" + std::to_string(doubles) +
+                                        " doubles) exceed the static staging maximum of " +
+                                        std::to_string(swx::kFrameScratchDoubles) + " doubles");
+        double* staging = swx::frameScratchPointer(s->device);
+        s->wing->frameFields(staging, vertex_position, vertex_velocity, line_length, line_violation);
+        if (vertex_pending_force) {
+            const std::size_t n = s->definition.nodeCount();
+            const std::size_t h = s->wing->suspension().hangOrder.size();
+            std::copy(s->pending.begin(), s->pending.end(), vertex_pending_force);
+            std::fill(vertex_pending_force + 3 * n, vertex_pending_force + 3 * (n + h), 0.0);
+        }
     });
 }
 

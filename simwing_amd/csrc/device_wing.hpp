@@ -125,6 +125,14 @@ public:
     };
     Frame frame();
 
+    // The frame fields of sw_structure_frame_fields_noalloc, computed into
+    // `staging` (device memory of at least frameFieldsDoubles() doubles that
+    // the caller provides) in frame order and copied from there into the
+    // non-null host outputs: positions and velocities 3 (N + H), line
+    // lengths and violations C + S. Allocates nothing.
+    std::size_t frameFieldsDoubles() const;
+    void frameFields(double* staging, double* position, double* velocity, double* length, double* violation);
+
     // Reductions for StructureDiagnostics (see capi.cpp for the fields).
     struct Reductions {
         double mass, weighted[3], momentum[3], kinetic;
@@ -208,11 +216,21 @@ private:
     std::size_t nodes_ = 0, triangles_ = 0, constraints_ = 0, membranes_ = 0, hinges_ = 0, segments_ = 0,
                 pairs_ = 0;
     void* partials_ = nullptr;  // diagnostics partial records
+    int* frameLineSlot_ = nullptr;  // definition line -> device item (frameFieldsKernel)
+    int* frameHangSlot_ = nullptr;  // definition harness point -> device hang index
     int workgroups_ = 1;
     int threads_ = 512;
     bool sharedNodes_ = false;
     std::size_t snapshotBytes_ = 0;
 };
+
+// Capacity of the static device staging array of
+// sw_structure_frame_fields_noalloc, in doubles (16 MiB per GPU).
+constexpr std::size_t kFrameScratchDoubles = std::size_t{1} << 21;
+
+// Device address of that array on `device` (hipGetSymbolAddress, looked up
+// once per device with the device current, then cached).
+double* frameScratchPointer(int device);
 
 // A failed step inside an adjoint run (the device error word), thrown after
 // the wing's state has been restored.

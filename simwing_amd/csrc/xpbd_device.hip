@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 
@@ -1816,11 +1817,107 @@ __global__ void frameKernel(WingView v, double* out) {
     }
 }
 
+// The diagnostic-frame fields of the committed state directly in frame
+// order: positions, velocities (3 per vertex: the nodes, then the harness
+// points in definition order), then per line in definition order the
+// current length and its violation (the quantities frameKernel computes).
+// lineSlot maps a definition line (constraints, then segments) to its
+// device item (distance slot, or distCount + segment slot); hangSlot maps a
+// definition harness point to its device index.
+__global__ void frameFieldsKernel(WingView v, const int* __restrict__ lineSlot, const int* __restrict__ hangSlot,
+                                  double* out) {
+    const long n = v.nodeCount;
+    const int h = v.hangCount;
+    const long vertices = n + h;
+    const long lines = static_cast<long>(v.distCount) + v.segCount;
+    double* position = out;
+    double* velocity = out + 3 * vertices;
+    double* length = out + 6 * vertices;
+    double* violation = length + lines;
+    const PayloadDev& p = *v.payload;
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    const long first = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    m3 r{};
+    bool rigid = false;
+    if (v.segCount > 0) rigid = rotationMatrix(quat{p.q[0], p.q[1], p.q[2], p.q[3]}, r);
+    const d3 com{p.com[0], p.com[1], p.com[2]};
+    const auto hang = [&](int i, d3& point, d3& offset) {
+        const d3 rel = sub(d3{v.hangLocal[3 * i], v.hangLocal[3 * i + 1], v.hangLocal[3 * i + 2]},
+                           d3{p.comLocal[0], p.comLocal[1], p.comLocal[2]});
+        offset = mulM(r, rel);
+        point = add(com, offset);
+    };
+    for (long i = first; i < n; i += stride) {
+        position[3 * i] = v.px[i];
+        position[3 * i + 1] = v.py[i];
+        position[3 * i + 2] = v.pz[i];
+        velocity[3 * i] = v.vx[i];
+        velocity[3 * i + 1] = v.vy[i];
+        velocity[3 * i + 2] = v.vz[i];
+    }
+    for (long j = first; j < h; j += stride) {
+        d3 point, offset;
+        hang(hangSlot[j], point, offset);
+        const d3 pv = add(d3{p.v[0], p.v[1], p.v[2]}, cross(d3{p.w[0], p.w[1], p.w[2]}, offset));
+        const double nan = __builtin_nan("");
+        const long o = 3 * (n + j);
+        position[o] = rigid ? point.x : nan;
+        position[o + 1] = rigid ? point.y : nan;
+        position[o + 2] = rigid ? point.z : nan;
+        velocity[o] = rigid ? pv.x : nan;
+        velocity[o + 1] = rigid ? pv.y : nan;
+        velocity[o + 2] = rigid ? pv.z : nan;
+    }
+    for (long l = first; l < lines; l += stride) {
+        const int k = lineSlot[l];
+        if (k < v.distCount) {
+            const int a = v.da[k], b = v.db[k];
+            const double len_ = len(sub(d3{v.px[b], v.py[b], v.pz[b]}, d3{v.px[a], v.py[a], v.pz[a]}));
+            const double extension = len_ - v.drest[k];
+            length[l] = len_;
+            violation[l] = v.dcable[k] ? stdmax(0.0, extension) : fabs(extension);
+        } else {
+            const int s = k - v.distCount;
+            d3 ends[2], offset;
+            const int codes[2] = {v.sfrom[s], v.sto[s]};
+            for (int e = 0; e < 2; ++e) {
+                if (codes[e] >= 0) ends[e] = d3{v.px[codes[e]], v.py[codes[e]], v.pz[codes[e]]};
+                else hang(-codes[e] - 1, ends[e], offset);
+            }
+            const double len_ = len(sub(ends[1], ends[0]));
+            length[l] = len_;
+            violation[l] = stdmax(0.0, len_ - v.srest[s]);
+        }
+    }
+}
+
 void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
 }
 
+// Staging for sw_structure_frame_fields_noalloc: a file-scope device array,
+// one copy per GPU, held until the process ends. Host code never uses its
+// name as an address; it uses the per-device pointer from
+// hipGetSymbolAddress (frameScratchPointer).
+static __device__ __align__(256) double scratch[kFrameScratchDoubles];
+
 }  // namespace
+
+double* frameScratchPointer(int device) {
+    static std::mutex guard;
+    static std::vector<double*> pointers;
+    std::lock_guard<std::mutex> lock(guard);
+    if (device < 0) throw std::invalid_argument("device index must not be negative");
+    if (pointers.size() <= static_cast<std::size_t>(device)) pointers.resize(static_cast<std::size_t>(device) + 1);
+    double*& ptr = pointers[static_cast<std::size_t>(device)];
+    if (!ptr) {
+        check(hipSetDevice(device), "hipSetDevice");
+        void* address = nullptr;
+        check(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        ptr = static_cast<double*>(address);
+    }
+    return ptr;
+}
 
 // ---------------------------------------------------------------------------
 // DeviceWing host side.
@@ -2412,6 +2509,19 @@ void DeviceWing::build(const Definition& d) {
     buildSync();
     check(hipMalloc(&partials_, 64 * sizeof(Partial)), "diagnostics partials");
     allocations_.push_back(partials_);
+    {
+        // Frame-order maps of frameFieldsKernel.
+        std::vector<int> lineSlot(constraints_ + segments_, 0);
+        std::vector<int> hangSlot(layout_.hangOrder.size(), 0);
+        for (std::size_t k = 0; k < constraints_; ++k) lineSlot[distance_.order[k]] = static_cast<int>(k);
+        for (std::size_t r = 0; r < segments_; ++r)
+            lineSlot[constraints_ + layout_.segmentOrder[r]] =
+                static_cast<int>(constraints_ + segmentDevicePosition_[r]);
+        for (std::size_t r = 0; r < layout_.hangOrder.size(); ++r)
+            hangSlot[layout_.hangOrder[r]] = static_cast<int>(r);
+        frameLineSlot_ = deviceCopy(allocations_, lineSlot);
+        frameHangSlot_ = deviceCopy(allocations_, hangSlot);
+    }
     check(hipStreamSynchronize(stream_), "sync build");
     check(hipDeviceSynchronize(), "sync build");
 }
@@ -2871,6 +2981,37 @@ DeviceWing::Frame DeviceWing::frame() {
         f.lineViolation[constraints_ + layout_.segmentOrder[r]] = violation[constraints_ + slot];
     }
     return f;
+}
+
+std::size_t DeviceWing::frameFieldsDoubles() const {
+    const std::size_t vertices = nodes_ + static_cast<std::size_t>(view_->hangCount);
+    return 6 * vertices + 2 * (constraints_ + segments_);
+}
+
+void DeviceWing::frameFields(double* staging, double* position, double* velocity, double* length,
+                             double* violation) {
+    const WingView& v = *view_;
+    const std::size_t vertices = nodes_ + static_cast<std::size_t>(v.hangCount);
+    const std::size_t lines = constraints_ + segments_;
+    check(hipSetDevice(device_), "hipSetDevice");
+    const long work = static_cast<long>(std::max({vertices, lines, std::size_t{1}}));
+    const int blocks = static_cast<int>(std::min<long>(1024, (work + 255) / 256));
+    hipLaunchKernelGGL(frameFieldsKernel, dim3(blocks), dim3(256), 0, stream_, v, frameLineSlot_, frameHangSlot_,
+                       staging);
+    check(hipGetLastError(), "frame fields");
+    const struct {
+        double* host;
+        std::size_t offset, count;
+    } copies[] = {{position, 0, 3 * vertices},
+                  {velocity, 3 * vertices, 3 * vertices},
+                  {length, 6 * vertices, lines},
+                  {violation, 6 * vertices + lines, lines}};
+    for (const auto& c : copies)
+        if (c.host && c.count)
+            check(hipMemcpyAsync(c.host, staging + c.offset, c.count * sizeof(double), hipMemcpyDeviceToHost,
+                                 stream_),
+                  "frame fields");
+    check(hipStreamSynchronize(stream_), "frame fields");
 }
 
 WingView DeviceWing::launchView() const {
