@@ -349,6 +349,19 @@ sw_status sw_structure_get_launch(const sw_structure* s, int* workgroups, int* t
  * reference (launch configuration). */
 sw_status sw_structure_set_cooperative(sw_structure* s, int cooperative);
 
+/* One-XCD synchronisation (no counterpart in the reference; launch
+ * configuration). When enabled, a multi-workgroup substep launch carries 8
+ * workgroups per participant and only the participants that land on one XCD
+ * (chosen at run time from the hardware XCD id) run; they share that XCD's
+ * L2, so a phase publishes its writes by draining them to the L2 and a
+ * waiter invalidates only its own L1, instead of an agent-scope L2
+ * write-back and invalidation per phase. Results are identical. *active is
+ * 0 when the current launch shape does not allow it (single workgroup, or
+ * 8 x the workgroups not resident at once, or more workgroups than one
+ * XCD's compute units hold); the request is kept and re-checked by
+ * sw_structure_set_launch. Enabled by default. */
+sw_status sw_structure_set_xcd_local(sw_structure* s, int enabled, int* active);
+
 /* Substep-kernel timing (no counterpart in the reference; measurement only).
  * While enabled, each substep kernel launch is bracketed by HIP events on the
  * structure's own stream. sw_structure_kernel_time waits for the stream and

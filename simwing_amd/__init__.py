@@ -605,6 +605,13 @@ class Structure:
     def set_cooperative(self, cooperative: bool) -> None:
         check(native.library().sw_structure_set_cooperative(self._handle, int(cooperative)))
 
+    def set_xcd_local(self, enabled: bool) -> bool:
+        """One-XCD synchronisation of multi-workgroup launches
+        (sw_structure_set_xcd_local); returns whether it is active."""
+        active = C.c_int()
+        check(native.library().sw_structure_set_xcd_local(self._handle, int(enabled), C.byref(active)))
+        return bool(active.value)
+
     def set_kernel_timing(self, enabled: bool) -> None:
         check(native.library().sw_structure_set_kernel_timing(self._handle, int(enabled)))
 

@@ -654,6 +654,14 @@ sw_status sw_structure_set_cooperative(sw_structure* s, int cooperative) {
     });
 }
 
+sw_status sw_structure_set_xcd_local(sw_structure* s, int enabled, int* active) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        const bool on = s->wing->setXcdLocal(enabled != 0);
+        if (active) *active = on && enabled ? 1 : 0;
+    });
+}
+
 sw_status sw_structure_set_kernel_timing(sw_structure* s, int enabled) {
     return guarded([&] {
         requirePointer(s, "structure");

@@ -164,6 +164,15 @@ public:
     // otherwise idle device and lets tools that do not follow cooperative
     // launches observe the kernel. Every wait in the kernel is bounded.
     void setCooperative(bool cooperative) { cooperative_ = cooperative; }
+    // One-XCD synchronisation for multi-workgroup launches: the launch
+    // carries 8 workgroups per participant and only those on one XCD take
+    // part, so the participants share that XCD's L2 and synchronise through
+    // it (no agent-scope L2 write-back or invalidation per phase). Returns
+    // false (and stays off) when the launch shape cannot hold 8 x the
+    // workgroups on the device at once. On by default wherever the launch
+    // shape allows it; the request is kept across setLaunch.
+    bool setXcdLocal(bool enabled);
+    bool xcdLocal() const { return xcdLocal_; }
     int workgroups() const { return workgroups_; }
     int threads() const { return threads_; }
     bool sharedNodes() const { return sharedNodes_; }
@@ -203,6 +212,9 @@ private:
     std::size_t sharedNodeBudget() const;
     std::size_t stageBytes_ = 0;  // dynamic LDS of the staged point-to-point program
     bool hasRun_ = false, neighbourSync_ = false, dedicatedChain_ = false, cooperative_ = true;
+    bool xcdLocal_ = false;      // one-XCD synchronisation active for the current shape
+    bool xcdRequested_ = true;   // ... and wanted (the default: on where the shape allows it)
+    unsigned epoch_ = 0;
     int participants_ = 0, maxParticipants_ = 0;
     unsigned long long launchGeneration_ = 0;
     void* depAlloc_[3] = {nullptr, nullptr, nullptr};
@@ -223,6 +235,10 @@ private:
     bool sharedNodes_ = false;
     std::size_t snapshotBytes_ = 0;
 };
+
+// Workgroups launched per participant in a one-XCD launch (the XCD count of
+// an MI355X).
+constexpr int kXcdLaunchFactor = 8;
 
 // Capacity of the static device staging array of
 // sw_structure_frame_fields_noalloc, in doubles (16 MiB per GPU).

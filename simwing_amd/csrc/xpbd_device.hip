@@ -84,6 +84,14 @@ struct WingView {
     // point-to-point synchronisation, dedicated chain workgroup, nodes in
     // LDS, and the wing's workgroup count in a batched launch.
     int neighbourSync, dedicatedChain, sharedNodes, launchBlocks;
+    // One-XCD synchronisation (set for launches whose participants all run
+    // on one XCD, so they share one L2): published writes are drained to
+    // that L2 and a waiter invalidates only its compute unit's L1, instead of
+    // writing back and invalidating the L2 at agent scope. `epoch` tags the
+    // launch's participant tickets (nonzero, different from the previous
+    // launch's).
+    int xcdLocal;
+    unsigned epoch;
     // Trajectory recording for the adjoint (null otherwise): every item
     // writes the values its projection reads (its input vector), laid out as
     // traceLayout() describes; any launch mode records the same values.
@@ -159,34 +167,111 @@ __device__ __forceinline__ bool syncTimedOut(const WingView& v, unsigned spins) 
 // leaving make every workgroup's earlier global writes visible to every
 // workgroup's later reads; the counter and generation words are touched only
 // with relaxed atomics in between.
+// One-XCD release half: every wave waits until its own earlier global
+// stores are complete in the XCD's L2. Called by all threads before the
+// workgroup barrier that precedes a publication.
+__device__ __forceinline__ void drainStores(const WingView& v) {
+    if (v.xcdLocal) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// Acquire after a wait: the agent-scope acquire (buffer_inv sc1: this
+// compute unit's L1, and the L2 lines other XCDs may have changed). With
+// every participant on one XCD the writers did not write their L2 lines
+// back, which the invalidation keeps (dirty lines are not dropped), so the
+// same invalidation is the acquire; measured on gfx950, the workgroup-scope
+// form (sc0) leaves stale L1 lines of other compute units' writes.
+__device__ __forceinline__ void acquireAfterWait(const WingView& v) {
+    (void)v;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 __device__ __forceinline__ void gridSync(const WingView& v, int workers) {
     if (workers == 1) {
         __syncthreads();
         return;
     }
+    drainStores(v);
     __syncthreads();
     if (threadIdx.x == 0) {
+        const bool local = v.xcdLocal != 0;
         unsigned* count = v.barrier;
         unsigned* generation = v.barrier + 32;
         const unsigned gen = __hip_atomic_load(generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // The generation must be read before this workgroup arrives: once it
+        // has arrived, the last arrival may already advance it.
+        if (!local) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == static_cast<unsigned>(workers) - 1u) {
             // Every other workgroup has arrived: reset the counter, then
-            // publish the next generation (release orders the reset first).
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            // publish the next generation (release orders the reset first;
+            // one XCD: the reset is an earlier atomic at the same L2).
+            acquireAfterWait(v);
             __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(generation, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (local) {
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __hip_atomic_store(generation, gen + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(generation, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
         } else {
             unsigned spins = 0;
             while (__hip_atomic_load(generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 __builtin_amdgcn_s_sleep(1);
                 if (syncTimedOut(v, ++spins)) break;
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            acquireAfterWait(v);
         }
     }
     __syncthreads();
+}
+
+// XCD (accelerator complex die) of the calling wave: hardware register
+// XCC_ID (20), bits [3:0].
+__device__ __forceinline__ unsigned xccId() { return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 0xfu; }
+
+// Participant index of this workgroup in a one-XCD launch (thread 0 only),
+// or -1 if it does not take part. Each XCD's workgroups draw tickets from
+// the XCD's own counter (barrier word 96 + XCD); the workgroup drawing
+// ticket `participants - 1` claims the launch for its XCD (word 112) unless
+// another XCD did first; workgroups of the claiming XCD with a ticket below
+// `participants` take part under that index. Counters and the claim carry
+// the launch's epoch in their upper 20 bits, so words left by an earlier
+// launch count as empty. Participants therefore all run on one XCD,
+// whatever the dispatcher's placement, provided one XCD receives at least
+// `participants` workgroups (the host launches 8 x participants).
+__device__ int xcdParticipant(const WingView& v, int participants) {
+    const unsigned xcc = xccId();
+    unsigned* counter = v.barrier + 96 + xcc;
+    unsigned* claim = v.barrier + 112;
+    const unsigned tag = (v.epoch & 0xfffffu) << 12;
+    unsigned old = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned ticket = 0;
+    for (;;) {
+        const bool current = (old & ~0xfffu) == tag;
+        const unsigned next = current ? old + 1u : (tag | 1u);
+        if (__hip_atomic_compare_exchange_strong(counter, &old, next, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) {
+            ticket = (next & 0xfffu) - 1u;
+            break;
+        }
+    }
+    if (ticket >= static_cast<unsigned>(participants)) return -1;
+    if (ticket == static_cast<unsigned>(participants) - 1u) {
+        unsigned seen = __hip_atomic_load(claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while ((seen & ~0xfffu) != tag) {
+            if (__hip_atomic_compare_exchange_strong(claim, &seen, tag | (xcc + 1u), __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                break;
+        }
+    }
+    unsigned spins = 0;
+    unsigned seen = 0;
+    while (((seen = __hip_atomic_load(claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & ~0xfffu) != tag) {
+        __builtin_amdgcn_s_sleep(1);
+        if (syncTimedOut(v, ++spins)) return -1;
+    }
+    return (seen & 0xfffu) == xcc + 1u ? static_cast<int>(ticket) : -1;
 }
 
 // Node position/inverse-mass store: LDS copies or the HBM arrays. A staged
@@ -997,7 +1082,7 @@ __device__ __forceinline__ void waitDeps(const WingView& v, int index, int step)
                 if (syncTimedOut(v, ++spins)) break;
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        acquireAfterWait(v);
     }
     __syncthreads();
 }
@@ -1049,7 +1134,7 @@ __device__ __forceinline__ void waitStagedDeps(const WingView& v, const StagedPr
                 if (syncTimedOut(v, ++spins)) break;
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        acquireAfterWait(v);
     }
     __syncthreads();
 }
@@ -1070,9 +1155,11 @@ __device__ __forceinline__ void prefetchedPhase(const WingView& v, const StagedP
 
 // Publishes `value` on a progress word after releasing this workgroup's
 // earlier writes (call after a workgroup barrier).
-__device__ __forceinline__ void publishCounter(unsigned* counter, unsigned value) {
+// One XCD: every wave drained its stores before that barrier
+// (drainStores), so only the progress word itself is written.
+__device__ __forceinline__ void publishCounter(const WingView& v, unsigned* counter, unsigned value) {
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (!v.xcdLocal) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_store(counter, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -1109,8 +1196,9 @@ __device__ __forceinline__ void pipelinedRun(const WingView& v, const StagedProg
         Item next{};
         if (nfirst < ne) next = load(nfirst);
         if (b < e) {
+            drainStores(v);
             __syncthreads();
-            publishCounter(v.flags + 32 * p, static_cast<unsigned>(step + 1));
+            publishCounter(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
         }
         b = nb, e = ne, first = nfirst, cur = next;
     }
@@ -1433,8 +1521,9 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     const int step = it * m + ph;
                     waitStagedDeps(v, S, ph, step);
                     runChain(v, N, P, payload, a, rec ? rec + layout.segBase : nullptr);
+                    drainStores(v);
                     __syncthreads();
-                    publishCounter(v.flags + 32 * p, static_cast<unsigned>(step + 1));
+                    publishCounter(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
                 }
             }
         }
@@ -1521,8 +1610,9 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     waitStagedDeps(v, S, ph, step);
                     break;
                 }
+                drainStores(v);
                 __syncthreads();
-                publishCounter(v.flags + 32 * p, static_cast<unsigned>(step + 1));
+                publishCounter(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
             }
         }
         gridSync(v, blocks);
@@ -1643,7 +1733,19 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
 
 template <bool kShared>
 __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArgs a) {
-    substepBody<kShared ? 1 : 0>(v, a, static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), v.trace);
+    int block = static_cast<int>(blockIdx.x), blocks = static_cast<int>(gridDim.x);
+    if (v.xcdLocal) {
+        // One-XCD launch: the grid holds several workgroups per XCD; those
+        // holding the first launchBlocks tickets of the XCD that fills them
+        // first are the participants, the rest leave.
+        __shared__ int role;
+        if (threadIdx.x == 0) role = xcdParticipant(v, v.launchBlocks);
+        __syncthreads();
+        if (role < 0) return;
+        block = role;
+        blocks = v.launchBlocks;
+    }
+    substepBody<kShared ? 1 : 0>(v, a, block, blocks, v.trace);
 }
 
 // Batched substep: workgroup b runs workgroup blockLocal[b] of wing
@@ -2507,6 +2609,7 @@ void DeviceWing::build(const Definition& d) {
         v.flags = deviceCopy(allocations_, std::vector<unsigned>(32 * static_cast<std::size_t>(maxParticipants_), 0u));
     }
     buildSync();
+    setXcdLocal(xcdRequested_);
     check(hipMalloc(&partials_, 64 * sizeof(Partial)), "diagnostics partials");
     allocations_.push_back(partials_);
     {
@@ -2557,6 +2660,26 @@ void DeviceWing::setLaunch(int workgroups, int threads) {
               "shared memory attribute");
     }
     buildSync();
+    setXcdLocal(xcdRequested_);
+}
+
+bool DeviceWing::setXcdLocal(bool enabled) {
+    xcdRequested_ = enabled;
+    xcdLocal_ = false;
+    if (!enabled || sharedNodes_ || workgroups_ < 2) return !enabled;
+    check(hipSetDevice(device_), "hipSetDevice");
+    int cus = 0, perCu = 0;
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
+    const std::size_t scratch = kMaxThreads * sizeof(double) + (neighbourSync_ ? stageBytes_ : 0);
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(&substepKernel<false>),
+                                                       threads_, scratch),
+          "occupancy");
+    // Every workgroup of the grid resident at once, and one XCD's share of
+    // the compute units able to hold all participants.
+    const long grid = static_cast<long>(kXcdLaunchFactor) * workgroups_;
+    if (grid > static_cast<long>(perCu) * cus || workgroups_ > perCu * (cus / kXcdLaunchFactor)) return false;
+    xcdLocal_ = true;
+    return true;
 }
 
 void DeviceWing::setExternalForces(const std::vector<double>& forces) {
@@ -2600,13 +2723,25 @@ void DeviceWing::launchSubstep(const StepArgs& stepArgs, double* trace) {
         hipLaunchKernelGGL(substepKernel<true>, dim3(1), dim3(threads_), bytes, stream_, v, args);
     } else if (workgroups_ == 1) {
         hipLaunchKernelGGL(substepKernel<false>, dim3(1), dim3(threads_), scratch, stream_, v, args);
-    } else if (!cooperative_) {
-        hipLaunchKernelGGL(substepKernel<false>, dim3(workgroups_), dim3(threads_), scratch, stream_, v, args);
     } else {
-        void* params[] = {&v, &args};
-        check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&substepKernel<false>), dim3(workgroups_),
-                                         dim3(threads_), params, scratch, stream_),
-              "cooperative launch");
+        int grid = workgroups_;
+        if (xcdLocal_) {
+            // One-XCD launch: 8 workgroups per participant, so that one XCD
+            // receives `workgroups_` of them (xcdParticipant).
+            v.xcdLocal = 1;
+            epoch_ = (epoch_ + 1u) & 0xfffffu;
+            if (epoch_ == 0) epoch_ = 1;
+            v.epoch = epoch_;
+            grid = kXcdLaunchFactor * workgroups_;
+        }
+        if (!cooperative_) {
+            hipLaunchKernelGGL(substepKernel<false>, dim3(grid), dim3(threads_), scratch, stream_, v, args);
+        } else {
+            void* params[] = {&v, &args};
+            check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&substepKernel<false>), dim3(grid),
+                                             dim3(threads_), params, scratch, stream_),
+                  "cooperative launch");
+        }
     }
     check(hipGetLastError(), "substep launch");
     if (timing_) {

@@ -247,6 +247,7 @@ _SIGNATURES = {
     "sw_assembly_flight_loads": (C.c_int, [C.c_void_p, _F64P]),
     "sw_structure_frame_fields": (C.c_int, [C.c_void_p, _F64P, _F64P, _F64P, _F64P, _F64P]),
     "sw_structure_frame_fields_noalloc": (C.c_int, [C.c_void_p, _F64P, _F64P, _F64P, _F64P, _F64P]),
+    "sw_structure_set_xcd_local": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "sw_scene_read": (C.c_int, [_U8P, C.c_size_t, C.POINTER(C.c_void_p)]),
     "sw_scene_write": (C.c_int, [C.c_void_p, _U8P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "sw_structure_frame": (C.c_int, [C.c_void_p, C.POINTER(FrameMapping), C.POINTER(FrameContext), _U8P,

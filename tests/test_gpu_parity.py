@@ -154,19 +154,24 @@ def test_device_bit_equal_to_portable_oracle(preset, steps):
 
 def test_launch_shape_does_not_change_results():
     """Single-workgroup launches with LDS-resident nodes (256 and 128 lanes)
-    and multi-workgroup grid launches give bit-identical states."""
+    and multi-workgroup grid launches, synchronised at agent scope or within
+    one XCD, give bit-identical states."""
     d, pressure, loads = flight_case("small")
     st = sw.flight_step_settings()
     states = []
-    for shape in (None, (1, 128), (16, 256), (64, 128)):
+    for shape, local in ((None, True), ((1, 128), True), ((16, 256), False), ((64, 128), False), ((14, 256), True),
+                         ((16, 256), True), ((32, 256), True), ((3, 256), True)):
         structure = sw.Structure(d)
         if shape:
             structure.set_launch(*shape)
+        active = structure.set_xcd_local(local)
+        # The small wing's default shape is 16 workgroups of 256 lanes.
+        assert active == (local and (shape is None or shape[0] > 1)), (shape, local, active)
         structure.set_face_pressure(pressure)
         for _ in range(5):
             structure.set_external_forces(loads)
             structure.step(st)
-        states.append((shape, device_state(structure)))
+        states.append(((shape, local), device_state(structure)))
     for shape, state in states[1:]:
         for key in state:
             assert bits_equal(state[key], states[0][1][key]), f"launch {shape}: {key} differs"
