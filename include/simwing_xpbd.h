@@ -133,10 +133,13 @@ typedef struct sw_definition {
     const uint64_t* sphere_nodes;
 } sw_definition;
 
-/* StructureStepSettings (structure.h:208-219). worker_threads is accepted
- * for interface parity and ignored: the device sweep is the reference's
- * coloured sweep (workerThreads >= 1) at every setting except 0, which the
- * device does not implement (the serial index-order sweep) and rejects. */
+/* StructureStepSettings (structure.h:208-219). worker_threads selects the
+ * sweep order as in the reference: 0 the serial sweep (distance constraints
+ * and membranes in definition order, soft_body.cpp:1316-1345), any other
+ * count the coloured sweep (the count itself does not change results). The
+ * device runs the serial order as dependency levels over the definition
+ * order. Batched steps and adjoint runs take the coloured sweep only
+ * (worker_threads = 0 is rejected there). */
 typedef struct sw_step_settings {
     double time_step;
     int substeps;
@@ -187,6 +190,9 @@ void sw_structure_destroy(sw_structure* structure);
 /* definitionFingerprint() (structure.cpp:326-431), acceptedStepCount(),
  * simulationTimeSeconds(). */
 sw_status sw_structure_fingerprint(const sw_structure* s, uint64_t* fingerprint);
+/* The same fingerprint computed on the host from a definition (validated as
+ * sw_structure_create validates it; no device involved). */
+sw_status sw_definition_fingerprint(const sw_definition* definition, uint64_t* fingerprint);
 sw_status sw_structure_progress(const sw_structure* s, uint64_t* accepted_steps, double* time);
 
 /* clearExternalForces / addExternalForce / setExternalForces
@@ -204,8 +210,13 @@ sw_status sw_structure_step(sw_structure* s, const sw_step_settings* settings, s
 sw_status sw_structure_diagnostics(sw_structure* s, sw_diagnostics* diagnostics);
 
 /* Steps `count` times back to back on the device without reading back
- * diagnostics between steps (the benchmark loop). Same transactional rule
- * per step; stops at the first failure. */
+ * diagnostics between steps (the benchmark loop). The loads pending at the
+ * call apply to every one of the steps (as a caller setting the same loads
+ * before each step) and are consumed at the end. Same transactional rule
+ * per step: a failure in step k restores the state at the start of step k,
+ * the k steps before it stay accepted (step count, time and last applied
+ * load advance by k; with k > 0 the pending loads are consumed) and the
+ * failure is reported. */
 sw_status sw_structure_step_many(sw_structure* s, const sw_step_settings* settings, int count);
 
 /* nodeStates() (structure.cpp:715-724): 3 doubles per node per array; any
@@ -319,6 +330,31 @@ sw_status sw_structure_solver_state(const sw_structure* s, double* constraint_la
 sw_status sw_structure_checkpoint(const sw_structure* s, sw_checkpoint** out);
 sw_status sw_structure_restore(sw_structure* s, const sw_checkpoint* checkpoint);
 void sw_checkpoint_release(sw_checkpoint* checkpoint);
+
+/* StructureCheckpoint's public fields (structure.h:254-271): version
+ * (structureCheckpointVersion = 2), definitionFingerprint,
+ * acceptedStepCount, simulationTimeSeconds and the node count; then the
+ * nodes (position, previousPosition, velocity: 3 doubles per node each),
+ * pendingExternalForcesNewtons (3 per node) and
+ * lastAppliedExternalForceNewtons (3). Any pointer may be NULL. */
+sw_status sw_checkpoint_info(const sw_checkpoint* checkpoint, uint32_t* version, uint64_t* fingerprint,
+                             uint64_t* accepted_steps, double* simulation_time, size_t* node_count);
+sw_status sw_checkpoint_nodes(const sw_checkpoint* checkpoint, double* position, double* previous,
+                              double* velocity, double* pending_forces, double* last_applied_force);
+
+/* suspensionState() (structure.cpp:718-776): *present is 0 for a structure
+ * without a suspension (std::nullopt) and nothing else is written. Else:
+ * payload13 = centre of mass, body-to-world quaternion (w, x, y, z), linear
+ * and angular velocity; harness positions and velocities (3 per harness
+ * point, definition order); per segment in definition order its stable id,
+ * current length, commanded rest length and tension (multiplier / dt^2 of
+ * the last substep, suspension.cpp:2513-2517). Before the first accepted
+ * substep the segments report their rest length as the current length and
+ * zero tension, as the reference does. Any array pointer may be NULL. */
+sw_status sw_structure_suspension_state(const sw_structure* s, int* present, double* payload13,
+                                        double* harness_position, double* harness_velocity, uint64_t* segment_ids,
+                                        double* segment_length, double* segment_commanded_rest,
+                                        double* segment_tension);
 
 /* Colour schedule of a family, colour-major: family 0 = distance
  * constraints (= SoftBody::constraintColouringView, soft_body.h:462-474),

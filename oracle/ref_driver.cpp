@@ -1,6 +1,7 @@
 // ORACLE HARNESS (test infrastructure only). Built by oracle/Makefile against
 // the reference's own sources (src/softwing, src/fsi) into oracle/_ref/; it is
-// never shipped, and it never runs on the GPU host.
+// never shipped. Fixture generation runs it where the reference sources are;
+// the benchmark's CPU baseline leg runs the prebuilt binary (--bench).
 //
 // It drives the reference's CPU XPBD path on the build's seeded synthetic
 // wings and writes golden fixtures:
@@ -35,6 +36,15 @@
 //        ref_driver --compare-workers <preset> <steps>
 //        ref_driver --frame <preset> <out-file> <steps>
 //        ref_driver --scene <preset> <out-file>
+//        ref_driver --structure <preset> <out-file> <steps> <workers>
+//
+// The --structure mode drives simwing::fsi::Structure like --frame (colour-
+// major hinges, flight loads before every step and again after the last, no
+// face pressure) with `workers` worker threads (0: the serial index-order
+// sweep) and writes what the adapter's public surface reports:
+// definitionFingerprint(), suspensionState() before the first step and
+// after the last, the committed node states, the checkpoint's public fields
+// and serializeStructureCheckpoint's bytes of that checkpoint.
 #include "fixture_io.hpp"
 #include "oracle_colouring.hpp"
 
@@ -42,6 +52,7 @@
 
 #include "scene_structure.h"
 #include "structure.h"
+#include "structure_checkpoint_persistence.h"
 #include "structure_frame.h"
 #include "viewer_protocol.h"
 #include <softwing/soft_body.h>
@@ -671,6 +682,89 @@ int frameMode(const std::string& preset, const std::string& out, int steps) {
     return 0;
 }
 
+void writeSuspensionState(swfx::Writer& w, const std::string& tag, const fsi::Structure& structure) {
+    const auto ss = structure.suspensionState();
+    if (!ss) return;
+    const auto& q = ss->payload;
+    w.f64(tag + "/payload", {q.centerOfMassWorldMeters.x, q.centerOfMassWorldMeters.y, q.centerOfMassWorldMeters.z,
+                             q.bodyToWorld.w, q.bodyToWorld.x, q.bodyToWorld.y, q.bodyToWorld.z,
+                             q.linearVelocityMetersPerSecond.x, q.linearVelocityMetersPerSecond.y,
+                             q.linearVelocityMetersPerSecond.z, q.angularVelocityRadiansPerSecond.x,
+                             q.angularVelocityRadiansPerSecond.y, q.angularVelocityRadiansPerSecond.z});
+    std::vector<double> hp, hv, length, commanded, tension;
+    for (const auto& h : ss->harnessPositionsMeters) hp.insert(hp.end(), {h.x, h.y, h.z});
+    for (const auto& h : ss->harnessVelocitiesMetersPerSecond) hv.insert(hv.end(), {h.x, h.y, h.z});
+    std::vector<std::uint64_t> ids;
+    for (const auto& g : ss->segments) {
+        ids.push_back(g.stableId);
+        length.push_back(g.currentLengthMeters);
+        commanded.push_back(g.commandedRestLengthMeters);
+        tension.push_back(g.tensionNewtons);
+    }
+    w.f64(tag + "/harness_position", hp);
+    w.f64(tag + "/harness_velocity", hv);
+    w.u64(tag + "/segment_ids", ids);
+    w.f64(tag + "/segment_length", length);
+    w.f64(tag + "/segment_commanded_rest", commanded);
+    w.f64(tag + "/segment_tension", tension);
+}
+
+int structureMode(const std::string& preset, const std::string& out, int steps, unsigned workers) {
+    Prepared p = prepare(preset);
+    fsi::StructureDefinition def = p.assembly.definition;
+    std::vector<fsi::StructureDihedralDefinition> hinges;
+    for (const std::size_t index : p.hinges.order) hinges.push_back(def.dihedrals[index]);
+    def.dihedrals = std::move(hinges);
+    const std::vector<fsi::StructureVector3> loads = structureLoads(def);
+    fsi::Structure structure(def);
+    swfx::Writer w(out);
+    w.u64("steps", {static_cast<std::uint64_t>(steps)});
+    w.u64("workers", {workers});
+    w.u64("fingerprint", {structure.definitionFingerprint()});
+    writeSuspensionState(w, "initial", structure);
+    fsi::StructureStepSettings settings;
+    settings.timeStepSeconds = 1.0 / 120.0;
+    settings.substeps = 1;
+    settings.constraintIterations = 48;
+    settings.cableConstraintSweepPairs = 3;
+    settings.gravityMetersPerSecondSquared = {0.0, 0.0, -9.80665};
+    settings.velocityDampingPerSecond = 8.0;
+    settings.workerThreads = workers;
+    for (int s = 0; s < steps; ++s) {
+        structure.setExternalForces(loads);
+        static_cast<void>(structure.step(settings));
+    }
+    structure.setExternalForces(loads);
+    writeSuspensionState(w, "final", structure);
+    const fsi::StructureCheckpoint cp = structure.checkpoint();
+    w.u64("checkpoint/version", {cp.version});
+    w.u64("checkpoint/fingerprint", {cp.definitionFingerprint});
+    w.u64("checkpoint/accepted_steps", {cp.acceptedStepCount});
+    w.f64("checkpoint/time", {cp.simulationTimeSeconds});
+    std::vector<double> x, prev, v, f;
+    for (const auto& n : cp.nodes) {
+        x.insert(x.end(), {n.positionMeters.x, n.positionMeters.y, n.positionMeters.z});
+        prev.insert(prev.end(), {n.previousPositionMeters.x, n.previousPositionMeters.y,
+                                 n.previousPositionMeters.z});
+        v.insert(v.end(), {n.velocityMetersPerSecond.x, n.velocityMetersPerSecond.y, n.velocityMetersPerSecond.z});
+    }
+    for (const auto& g : cp.pendingExternalForcesNewtons) f.insert(f.end(), {g.x, g.y, g.z});
+    w.f64("checkpoint/position", x);
+    w.f64("checkpoint/previous", prev);
+    w.f64("checkpoint/velocity", v);
+    w.f64("checkpoint/pending_force", f);
+    w.f64("checkpoint/last_applied", {cp.lastAppliedExternalForceNewtons.x, cp.lastAppliedExternalForceNewtons.y,
+                                      cp.lastAppliedExternalForceNewtons.z});
+    std::vector<std::uint8_t> bytes;
+    fsi::StructureCheckpointPersistenceError error;
+    if (!fsi::serializeStructureCheckpoint(structure, cp, bytes, &error))
+        throw std::runtime_error("serializeStructureCheckpoint: " + error.message);
+    w.u8("checkpoint_bytes", bytes);
+    std::printf("structure preset=%s steps=%d workers=%u fingerprint=%016llx checkpoint_bytes=%zu\n", preset.c_str(),
+                steps, workers, static_cast<unsigned long long>(structure.definitionFingerprint()), bytes.size());
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -690,6 +784,8 @@ int main(int argc, char** argv) {
                         static_cast<unsigned long long>(hb), ha == hb ? "IDENTICAL" : "DIFFERENT");
             return ha == hb ? 0 : 1;
         }
+        if (argc >= 6 && std::strcmp(argv[1], "--structure") == 0)
+            return structureMode(argv[2], argv[3], std::atoi(argv[4]), static_cast<unsigned>(std::atoi(argv[5])));
         if (argc >= 5 && std::strcmp(argv[1], "--frame") == 0)
             return frameMode(argv[2], argv[3], std::atoi(argv[4]));
         if (argc >= 4 && std::strcmp(argv[1], "--scene") == 0) {

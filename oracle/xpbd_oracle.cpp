@@ -1,8 +1,9 @@
 // ORACLE (test infrastructure only; never linked into the product library).
 //
 // Plain serial C++ restatement of the reference's coupled XPBD substep, in
-// the reference's own data layout (array of node records) and sweep order
-// for workerThreads >= 1. Each function cites the reference lines it
+// the reference's own data layout (array of node records) and sweep orders
+// (coloured for workerThreads >= 1, definition order for workerThreads = 0).
+// Each function cites the reference lines it
 // restates. Compiled with g++ -O2 -ffp-contract=off, like the reference.
 //
 // Pinned by: bit-identical states against the fixtures the reference itself
@@ -771,7 +772,8 @@ std::vector<std::size_t> loadPathOrder(const oracle_body& b) {
     return order;
 }
 
-// soft_body.cpp:1175-1460 (coupled, workerThreads >= 1) for one substep.
+// soft_body.cpp:1175-1460 (coupled; the coloured sweep for workerThreads >= 1,
+// the serial definition-order sweep for workerThreads = 0) for one substep.
 void substep(oracle_body& b, Real dt, const sw_step_settings& st) {
     // Pressure gather in ascending triangle order (soft_body.cpp:1492-1534).
     {
@@ -831,8 +833,15 @@ void substep(oracle_body& b, Real dt, const sw_step_settings& st) {
     for (int it = 0; it < iters; ++it) {
         const int share = interleaved / std::max(1, iters) + (it < interleaved % std::max(1, iters) ? 1 : 0);
         for (int k = 0; k < share; ++k) cablePair();
-        for (auto k : b.distance.order) projectDistance(b, b.constraints[k], invDt2);
-        for (auto k : b.membrane.order) projectMembrane(b, b.membranes[k], dt);
+        if (st.worker_threads == 0) {
+            // workerThreads = 0: the serial sweep in definition order
+            // (soft_body.cpp:1326-1335, 1347-1351).
+            for (auto& c : b.constraints) projectDistance(b, c, invDt2);
+            for (auto& m : b.membranes) projectMembrane(b, m, dt);
+        } else {
+            for (auto k : b.distance.order) projectDistance(b, b.constraints[k], invDt2);
+            for (auto k : b.membrane.order) projectMembrane(b, b.membranes[k], dt);
+        }
         for (Hinge& h : b.hinges) projectHinge(b, h, invDt2);
         if (b.hasSuspension) lineIteration(b, dt);
         for (Pair& p : b.pairs) projectPair(b, p, invDt2);

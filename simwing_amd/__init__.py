@@ -289,6 +289,14 @@ class Scene:
         return Assembly(self, settings or native.flight_assembly_settings())
 
 
+def definition_fingerprint(definition: "DefinitionArrays") -> int:
+    """definitionFingerprint() of a definition, on the host (sw_definition_fingerprint)."""
+    view = definition.to_ctypes()
+    out = C.c_uint64()
+    check(native.library().sw_definition_fingerprint(C.byref(view), C.byref(out)))
+    return out.value
+
+
 def generate_scene(preset: str) -> Scene:
     return Scene(preset)
 
@@ -582,6 +590,26 @@ class Structure:
             f64p(out["segment_lambda"])))
         return out
 
+    def suspension_state(self) -> dict | None:
+        """suspensionState() (sw_structure_suspension_state): None without a
+        suspension."""
+        d = self._definition
+        present = C.c_int()
+        if d.suspension is None:
+            check(native.library().sw_structure_suspension_state(self._handle, C.byref(present), None, None, None,
+                                                                  None, None, None, None))
+            return None
+        h = int(np.asarray(d.suspension["harness_ids"]).size)
+        sn = int(np.asarray(d.suspension["segment_ids"]).size)
+        out = {"payload": np.zeros(13), "harness_position": np.zeros(3 * h), "harness_velocity": np.zeros(3 * h),
+               "segment_ids": np.zeros(sn, np.uint64), "segment_length": np.zeros(sn),
+               "segment_commanded_rest": np.zeros(sn), "segment_tension": np.zeros(sn)}
+        check(native.library().sw_structure_suspension_state(
+            self._handle, C.byref(present), f64p(out["payload"]), f64p(out["harness_position"]),
+            f64p(out["harness_velocity"]), u64p(out["segment_ids"]), f64p(out["segment_length"]),
+            f64p(out["segment_commanded_rest"]), f64p(out["segment_tension"])))
+        return out if present.value else None
+
     def checkpoint(self) -> "Checkpoint":
         handle = C.c_void_p()
         check(native.library().sw_structure_checkpoint(self._handle, C.byref(handle)))
@@ -634,6 +662,25 @@ class Checkpoint:
 
     def __init__(self, handle):
         self._handle = handle
+
+    def info(self) -> dict:
+        """The public fields: version, definition fingerprint, accepted step
+        count, simulation time and node count (sw_checkpoint_info)."""
+        version, fingerprint, steps = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        time, nodes = C.c_double(), C.c_size_t()
+        check(native.library().sw_checkpoint_info(self._handle, C.byref(version), C.byref(fingerprint),
+                                                  C.byref(steps), C.byref(time), C.byref(nodes)))
+        return {"version": version.value, "fingerprint": fingerprint.value, "accepted_steps": steps.value,
+                "time": time.value, "node_count": nodes.value}
+
+    def nodes(self) -> dict:
+        """Node states, pending loads and the last applied load (sw_checkpoint_nodes)."""
+        n = self.info()["node_count"]
+        out = {k: np.zeros(3 * n) for k in ("position", "previous", "velocity", "pending_force")}
+        out["last_applied"] = np.zeros(3)
+        check(native.library().sw_checkpoint_nodes(self._handle, *[f64p(out[k]) for k in (
+            "position", "previous", "velocity", "pending_force", "last_applied")]))
+        return out
 
     def __del__(self):
         if getattr(self, "_handle", None):

@@ -86,6 +86,8 @@ struct sw_checkpoint {
     std::vector<double> pending;
     double lastApplied[3] = {0.0, 0.0, 0.0};
     double maxResidual = 0.0;
+    double lastSubstepDt = 0.0;
+    bool suspensionStepped = false;
 };
 
 struct sw_structure {
@@ -99,6 +101,12 @@ struct sw_structure {
     std::uint64_t acceptedSteps = 0;
     double time = 0.0;
     double maxResidual = 0.0;
+    // Substep length of the last accepted step, and whether the suspension
+    // has finished a substep (its per-segment diagnostics exist,
+    // suspension.cpp:2481-2540; before that suspensionState() reports rest
+    // lengths and zero tension, structure.cpp:765-768).
+    double lastSubstepDt = 0.0;
+    bool suspensionStepped = false;
     std::vector<std::string> segmentNames;  // runtime order
     bool hasLoadPath = false;
     // An isolated free membrane (no suspension, every node dynamic, membranes
@@ -111,7 +119,9 @@ struct sw_structure {
 namespace {
 
 // SoftBody::stepCoupled / step setting checks (soft_body.cpp:981-1050).
-swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st) {
+// `serialAllowed`: the caller can run the serial sweep (workerThreads = 0);
+// the batch and the adjoints differentiate or batch the coloured sweep.
+swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st, bool serialAllowed = false) {
     const bool coupled = s.definition.suspension.has_value();
     const auto finite3 = [](const double* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); };
     if (!(st.time_step > 0.0) || !std::isfinite(st.time_step) || st.substeps <= 0 || st.constraint_iterations < 0 ||
@@ -122,10 +132,13 @@ swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st) {
     if (coupled && st.constraint_iterations != s.definition.suspension->solverIterations)
         throw std::invalid_argument(
             "Coupled structural iterations must equal the registered suspension lineIterations contract");
-    if (st.worker_threads == 0)
+    if (st.worker_threads == 0 && !serialAllowed)
         throw std::invalid_argument(
-            "workerThreads 0 selects the serial index-order sweep, which the device path does not run; "
-            "use workerThreads >= 1 (the coloured sweep)");
+            "workerThreads 0 selects the serial index-order sweep, which batched steps and adjoint runs do not "
+            "take; use workerThreads >= 1 (the coloured sweep)");
+    // workerThreads selects the sweep order (structure.h:216-218): 0 the
+    // serial definition order, any other count the coloured order.
+    s.wing->setSerialSweep(st.worker_threads == 0);
     if (s.hasLoadPath && st.cable_sweep_pairs > 0)
         throw std::invalid_argument("load-path cable sweep pairs are not supported by the device path");
     if (s.freeMembrane)
@@ -229,7 +242,7 @@ void fillDiagnostics(sw_structure& s, sw_diagnostics* out) {
 
 void runSteps(sw_structure& s, const sw_step_settings& settings, int count, sw_diagnostics* out) {
     if (count < 1) throw std::invalid_argument("step count must be positive");
-    const swx::StepArgs args = prepare(s, settings);
+    const swx::StepArgs args = prepare(s, settings, true);
     if (s.pendingDirty) {
         s.wing->setExternalForces(s.pending);
         s.pendingDirty = false;
@@ -241,20 +254,31 @@ void runSteps(sw_structure& s, const sw_step_settings& settings, int count, sw_d
     }
     double residual = 0.0;
     const swx::DeviceError e = s.wing->readError(&residual);
+    // Steps accepted: all of them, or (after a failure) those before the
+    // failing one, whose pre-step snapshot is the last one taken.
+    int accepted = count;
+    if (e.code != 0) accepted = std::max(0, s.wing->snapshotsTaken() - 1);
+    const auto commit = [&](int steps) {
+        if (steps <= 0) return;
+        double applied[3] = {0.0, 0.0, 0.0};
+        for (std::size_t i = 0; i < s.pending.size() / 3; ++i)
+            for (int k = 0; k < 3; ++k) applied[k] += s.pending[3 * i + k];
+        for (int k = 0; k < 3; ++k) s.lastApplied[k] = applied[k];
+        std::fill(s.pending.begin(), s.pending.end(), 0.0);
+        s.pendingDirty = true;
+        s.acceptedSteps += static_cast<std::uint64_t>(steps);
+        for (int step = 0; step < steps; ++step) s.time += settings.time_step;
+        s.lastSubstepDt = args.dt;
+        s.suspensionStepped = s.suspensionStepped || s.definition.suspension.has_value();
+    };
     if (e.code != 0) {
         s.wing->restoreSnapshot();
         s.wing->clearError();
         s.wing->synchronize();
+        commit(accepted);
         throwDeviceError(s, e);
     }
-    double applied[3] = {0.0, 0.0, 0.0};
-    for (std::size_t i = 0; i < s.pending.size() / 3; ++i)
-        for (int k = 0; k < 3; ++k) applied[k] += s.pending[3 * i + k];
-    for (int k = 0; k < 3; ++k) s.lastApplied[k] = applied[k];
-    std::fill(s.pending.begin(), s.pending.end(), 0.0);
-    s.pendingDirty = true;
-    s.acceptedSteps += static_cast<std::uint64_t>(count);
-    for (int step = 0; step < count; ++step) s.time += settings.time_step;
+    commit(count);
     s.maxResidual = residual;
     fillDiagnostics(s, out);
 }
@@ -544,6 +568,8 @@ sw_status sw_structure_checkpoint(const sw_structure* s, sw_checkpoint** out) {
         c->pending = s->pending;
         for (int k = 0; k < 3; ++k) c->lastApplied[k] = s->lastApplied[k];
         c->maxResidual = s->maxResidual;
+        c->lastSubstepDt = s->lastSubstepDt;
+        c->suspensionStepped = s->suspensionStepped;
         *out = c.release();
     });
 }
@@ -566,10 +592,71 @@ sw_status sw_structure_restore(sw_structure* s, const sw_checkpoint* c) {
         s->acceptedSteps = c->acceptedSteps;
         s->time = c->time;
         s->maxResidual = c->maxResidual;
+        s->lastSubstepDt = c->lastSubstepDt;
+        s->suspensionStepped = c->suspensionStepped;
     });
 }
 
 void sw_checkpoint_release(sw_checkpoint* checkpoint) { delete checkpoint; }
+
+sw_status sw_checkpoint_info(const sw_checkpoint* c, uint32_t* version, uint64_t* fingerprint,
+                             uint64_t* accepted_steps, double* simulation_time, size_t* node_count) {
+    return guarded([&] {
+        requirePointer(c, "checkpoint");
+        if (version) *version = 2;  // structureCheckpointVersion (structure.h:248)
+        if (fingerprint) *fingerprint = c->fingerprint;
+        if (accepted_steps) *accepted_steps = c->acceptedSteps;
+        if (simulation_time) *simulation_time = c->time;
+        if (node_count) *node_count = c->pending.size() / 3;
+    });
+}
+
+sw_status sw_checkpoint_nodes(const sw_checkpoint* c, double* position, double* previous, double* velocity,
+                              double* pending_forces, double* last_applied_force) {
+    return guarded([&] {
+        requirePointer(c, "checkpoint");
+        if (position) std::copy(c->state.position.begin(), c->state.position.end(), position);
+        if (previous) std::copy(c->state.previous.begin(), c->state.previous.end(), previous);
+        if (velocity) std::copy(c->state.velocity.begin(), c->state.velocity.end(), velocity);
+        if (pending_forces) std::copy(c->pending.begin(), c->pending.end(), pending_forces);
+        if (last_applied_force) std::copy(c->lastApplied, c->lastApplied + 3, last_applied_force);
+    });
+}
+
+sw_status sw_structure_suspension_state(const sw_structure* s, int* present, double* payload13,
+                                        double* harness_position, double* harness_velocity, uint64_t* segment_ids,
+                                        double* segment_length, double* segment_commanded_rest,
+                                        double* segment_tension) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        const swx::Definition& d = s->definition;
+        if (present) *present = d.suspension ? 1 : 0;
+        if (!d.suspension) return;
+        const swx::SuspensionDef& sd = *d.suspension;
+        const std::size_t segments = sd.segmentIds.size();
+        const swx::DeviceWing::State st = s->wing->download();
+        if (payload13) {
+            // com, q (w, x, y, z), v, w of the payload (State::payload holds
+            // com, q, v, w, then the previous pose).
+            std::copy(st.payload.begin(), st.payload.begin() + 13, payload13);
+        }
+        const swx::DeviceWing::Frame f = s->wing->frame();
+        if (harness_position) std::copy(f.harnessPosition.begin(), f.harnessPosition.end(), harness_position);
+        if (harness_velocity) std::copy(f.harnessVelocity.begin(), f.harnessVelocity.end(), harness_velocity);
+        std::vector<double> lambda(segments, 0.0);
+        const auto& order = s->wing->suspension().segmentOrder;  // runtime position -> definition index
+        for (std::size_t r = 0; r < order.size(); ++r) lambda[order[r]] = st.segmentLambda[r];
+        const std::size_t constraints = d.constraintCount();
+        const double dt = s->lastSubstepDt;
+        for (std::size_t j = 0; j < segments; ++j) {
+            const double rest = sd.segmentRest[j];
+            if (segment_ids) segment_ids[j] = sd.segmentIds[j];
+            if (segment_commanded_rest) segment_commanded_rest[j] = rest;
+            if (segment_length) segment_length[j] = s->suspensionStepped ? f.lineLength[constraints + j] : rest;
+            if (segment_tension) segment_tension[j] = s->suspensionStepped ? lambda[j] / (dt * dt) : 0.0;
+        }
+    });
+}
 
 sw_status sw_structure_colouring(const sw_structure* s, int family, uint64_t* order, size_t* orderCount,
                                  uint64_t* offsets, size_t* offsetCount) {
@@ -584,6 +671,16 @@ sw_status sw_structure_colouring(const sw_structure* s, int family, uint64_t* or
         if (offsetCount) *offsetCount = c->offsets.size();
         if (order) std::copy(c->order.begin(), c->order.end(), order);
         if (offsets) std::copy(c->offsets.begin(), c->offsets.end(), offsets);
+    });
+}
+
+sw_status sw_definition_fingerprint(const sw_definition* definition, uint64_t* fingerprint) {
+    return guarded([&] {
+        requirePointer(definition, "definition");
+        requirePointer(fingerprint, "fingerprint");
+        const swx::Definition d = swx::Definition::fromView(*definition);
+        swx::validateDefinition(d);
+        *fingerprint = swx::definitionFingerprint(d);
     });
 }
 

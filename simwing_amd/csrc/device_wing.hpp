@@ -25,6 +25,11 @@ enum PhaseType : int {
     kPhaseSegmentRun = 4, // consecutive single-item levels run by one lane
     kPhaseSpheres = 5,    // one contact-sphere dependency level
     kPhaseStitches = 6,   // one level of the seam-stitch closing pass
+    // Serial sweep (workerThreads = 0): one dependency level over the
+    // definition order of the distance constraints or the membranes; the
+    // phase's [begin, end) indexes the level-ordered list of device slots.
+    kPhaseDistanceList = 7,
+    kPhaseMembraneList = 8,
 };
 
 struct Phase {
@@ -101,8 +106,11 @@ public:
     // and restore from it.
     void snapshot();
     void restoreSnapshot();
-    // Clears the device error word.
+    // Clears the device error word (and the snapshot count).
     void clearError();
+    // Snapshots taken since the last clearError (one per step that started
+    // without a pending failure).
+    int snapshotsTaken();
     DeviceError readError(double* maxResidual);
     void synchronize();
 
@@ -172,6 +180,15 @@ public:
     // workgroups on the device at once. On by default wherever the launch
     // shape allows it; the request is kept across setLaunch.
     bool setXcdLocal(bool enabled);
+    // The reference's serial sweep (StepSettings::workerThreads = 0:
+    // distance constraints and membranes in definition order,
+    // soft_body.cpp:1316-1345) or its coloured sweep (workerThreads >= 1).
+    // The serial sweep runs as dependency levels over the definition order
+    // (items of one level touch disjoint nodes, so the level's parallel
+    // projection equals the serial one); hinges, segments and spheres are
+    // the same in both.
+    void setSerialSweep(bool serial);
+    bool serialSweep() const { return serialSweep_; }
     bool xcdLocal() const { return xcdLocal_; }
     int workgroups() const { return workgroups_; }
     int threads() const { return threads_; }
@@ -198,6 +215,11 @@ private:
     ColourOrder distance_, membrane_, hinge_;
     SuspensionLayout layout_;
     std::vector<Phase> iterPhases_, closePhases_;
+    std::vector<Phase> colouredPhases_, serialPhases_;  // the two iteration programs
+    Phase* colouredPhasesDev_ = nullptr;
+    Phase* serialPhasesDev_ = nullptr;
+    std::vector<int> distList_, membList_;  // serial-sweep level order (device slots)
+    bool serialSweep_ = false;
     // Nodes of each item in device slot order (the payload is node nodes_),
     // kept on the host for the synchronisation analysis.
     struct ItemNodes {

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
@@ -66,6 +67,7 @@ struct WingView {
     double *spRadius, *spComp, *splam;
     int stitchCount;
     int* stitchItems;
+    int *distList, *membList;  // serial-sweep levels: device slots in level order
     Phase* iterPhases;
     int iterPhaseCount;
     Phase* closePhases;
@@ -1359,6 +1361,18 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
         for (int k = first; k < end; k += stride)
             projectSphere(v, N, k, a.invDt2, rec ? rec + L.sphereBase + kRecSphere * static_cast<long>(k) : nullptr);
         break;
+    case kPhaseDistanceList:
+        for (int j = first; j < end; j += stride) {
+            const int k = v.distList[j];
+            projectDistance(v, N, k, a.invDt2, rec ? rec + L.distBase + kRecDist * static_cast<long>(k) : nullptr);
+        }
+        break;
+    case kPhaseMembraneList:
+        for (int j = first; j < end; j += stride) {
+            const int k = v.membList[j];
+            projectMembrane(v, N, k, a, rec ? rec + L.membBase + kRecMemb * static_cast<long>(k) : nullptr);
+        }
+        break;
     default:
         break;
     }
@@ -1606,6 +1620,11 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                         projectSphere(v, N, k, a.invDt2,
                                       rec ? rec + layout.sphereBase + kRecSphere * static_cast<long>(k) : nullptr);
                     break;
+                case kPhaseDistanceList:
+                case kPhaseMembraneList:
+                    waitStagedDeps(v, S, ph, step);
+                    runPhase(v, N, P, first, e, stride, payload, a, rec, layout);
+                    break;
                 default:
                     waitStagedDeps(v, S, ph, step);
                     break;
@@ -1761,8 +1780,13 @@ __global__ void __launch_bounds__(kMaxThreads) batchSubstepKernel(const WingView
 
 // Copies the mutable state block into the snapshot while no error is
 // pending (so a failing step keeps its own pre-step snapshot).
+// Each snapshot taken increments barrier word 120 (cleared with the error
+// word), so the host can tell how many steps of a queued run started before
+// a failure.
 __global__ void snapshotKernel(WingView v, int restore) {
     if (!restore && __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    if (!restore && blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_fetch_add(v.barrier + 120, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const long stride = static_cast<long>(gridDim.x) * blockDim.x;
     for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < v.stateDoubles; i += stride) {
         if (restore) v.state[i] = v.snapshot[i];
@@ -2542,7 +2566,52 @@ void DeviceWing::build(const Definition& d) {
     hasRun_ = false;
     for (const Phase& P : iterPhases_) hasRun_ = hasRun_ || P.type == kPhaseSegmentRun;
     v.hasRun = hasRun_ ? 1 : 0;
-    v.iterPhases = deviceCopy(allocations_, iterPhases_);
+    // The serial sweep's program: dependency levels over the definition
+    // order of the distance constraints, then of the membranes, then the
+    // coloured program's hinge, segment and sphere phases.
+    {
+        const auto levelList = [&](const ColourOrder& colours, std::size_t count, int width,
+                                   const std::function<std::uint32_t(std::size_t, int)>& node, int type,
+                                   std::vector<int>& list) {
+            std::vector<std::uint32_t> slotOf(count);
+            for (std::size_t k = 0; k < colours.order.size(); ++k) slotOf[colours.order[k]] = static_cast<std::uint32_t>(k);
+            std::vector<std::vector<std::uint32_t>> writes(count);
+            for (std::size_t i = 0; i < count; ++i)
+                for (int c = 0; c < width; ++c) writes[i].push_back(node(i, c));
+            const auto level = dependencyLevels(writes, std::vector<std::vector<std::uint32_t>>(count),
+                                                std::vector<std::uint8_t>(count, 0), nodes_);
+            std::vector<std::uint32_t> byLevel(count);
+            std::iota(byLevel.begin(), byLevel.end(), 0u);
+            std::stable_sort(byLevel.begin(), byLevel.end(),
+                             [&](std::uint32_t x, std::uint32_t y) { return level[x] < level[y]; });
+            list.clear();
+            for (std::uint32_t i : byLevel) list.push_back(static_cast<int>(slotOf[i]));
+            std::size_t s = 0;
+            while (s < byLevel.size()) {
+                std::size_t e = s;
+                while (e < byLevel.size() && level[byLevel[e]] == level[byLevel[s]]) ++e;
+                serialPhases_.push_back({type, static_cast<int>(s), static_cast<int>(e), 0});
+                s = e;
+            }
+        };
+        serialPhases_.clear();
+        levelList(distance_, constraints_, 2,
+                  [&](std::size_t i, int c) { return static_cast<std::uint32_t>(d.constraintNodes[2 * i + c]); },
+                  kPhaseDistanceList, distList_);
+        levelList(membrane_, membranes_, 3,
+                  [&](std::size_t i, int c) {
+                      return static_cast<std::uint32_t>(d.triangleNodes[3 * d.membraneTriangle[i] + c]);
+                  },
+                  kPhaseMembraneList, membList_);
+        for (const Phase& P : iterPhases_)
+            if (P.type != kPhaseDistance && P.type != kPhaseMembrane) serialPhases_.push_back(P);
+        v.distList = deviceCopy(allocations_, distList_);
+        v.membList = deviceCopy(allocations_, membList_);
+        serialPhasesDev_ = deviceCopy(allocations_, serialPhases_);
+    }
+    colouredPhases_ = iterPhases_;
+    colouredPhasesDev_ = deviceCopy(allocations_, iterPhases_);
+    v.iterPhases = colouredPhasesDev_;
     v.iterPhaseCount = static_cast<int>(iterPhases_.size());
     v.closePhases = deviceCopy(allocations_, closePhases_);
     v.closePhaseCount = static_cast<int>(closePhases_.size());
@@ -2663,6 +2732,19 @@ void DeviceWing::setLaunch(int workgroups, int threads) {
     setXcdLocal(xcdRequested_);
 }
 
+void DeviceWing::setSerialSweep(bool serial) {
+    if (serial == serialSweep_) return;
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipStreamSynchronize(stream_), "synchronise before switching the sweep");
+    serialSweep_ = serial;
+    iterPhases_ = serial ? serialPhases_ : colouredPhases_;
+    view_->iterPhases = serial ? serialPhasesDev_ : colouredPhasesDev_;
+    view_->iterPhaseCount = static_cast<int>(iterPhases_.size());
+    ++launchGeneration_;
+    buildSync();
+    setXcdLocal(xcdRequested_);
+}
+
 bool DeviceWing::setXcdLocal(bool enabled) {
     xcdRequested_ = enabled;
     xcdLocal_ = false;
@@ -2761,6 +2843,14 @@ void DeviceWing::restoreSnapshot() {
     check(hipStreamSynchronize(stream_), "restore");
 }
 
+int DeviceWing::snapshotsTaken() {
+    unsigned taken = 0;
+    check(hipMemcpyAsync(&taken, view_->barrier + 120, sizeof(unsigned), hipMemcpyDeviceToHost, stream_),
+          "snapshot count");
+    check(hipStreamSynchronize(stream_), "snapshot count");
+    return static_cast<int>(taken);
+}
+
 void DeviceWing::clearError() {
     // Also resets the synchronisation words (they are zero after every
     // completed launch, but not after a launch whose waits timed out).
@@ -2779,6 +2869,12 @@ void DeviceWing::itemAccess(int type, int k, std::vector<int>& reads, std::vecto
         break;
     case kPhaseMembrane:
         writes.assign(items_.memb.begin() + 3 * k, items_.memb.begin() + 3 * k + 3);
+        break;
+    case kPhaseDistanceList:
+        writes.assign(items_.dist.begin() + 2 * distList_[k], items_.dist.begin() + 2 * distList_[k] + 2);
+        break;
+    case kPhaseMembraneList:
+        writes.assign(items_.memb.begin() + 3 * membList_[k], items_.memb.begin() + 3 * membList_[k] + 3);
         break;
     case kPhaseHinge:
         writes.assign(items_.hinge.begin() + 4 * k, items_.hinge.begin() + 4 * k + 4);

@@ -248,6 +248,12 @@ _SIGNATURES = {
     "sw_structure_frame_fields": (C.c_int, [C.c_void_p, _F64P, _F64P, _F64P, _F64P, _F64P]),
     "sw_structure_frame_fields_noalloc": (C.c_int, [C.c_void_p, _F64P, _F64P, _F64P, _F64P, _F64P]),
     "sw_structure_set_xcd_local": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "sw_definition_fingerprint": (C.c_int, [C.POINTER(Definition), C.POINTER(C.c_uint64)]),
+    "sw_checkpoint_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_size_t)]),
+    "sw_checkpoint_nodes": (C.c_int, [C.c_void_p, _F64P, _F64P, _F64P, _F64P, _F64P]),
+    "sw_structure_suspension_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), _F64P, _F64P, _F64P,
+                                                C.POINTER(C.c_uint64), _F64P, _F64P, _F64P]),
     "sw_scene_read": (C.c_int, [_U8P, C.c_size_t, C.POINTER(C.c_void_p)]),
     "sw_scene_write": (C.c_int, [C.c_void_p, _U8P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "sw_structure_frame": (C.c_int, [C.c_void_p, C.POINTER(FrameMapping), C.POINTER(FrameContext), _U8P,

@@ -17,6 +17,13 @@ wings, and writes into tests/golden/:
 * ``<preset>.scene.bin``      -- ``ref_driver --scene``: the reference's
   writeScene stream of the seeded wing (tiny: the stream itself; small:
   ``small.scene.digest.swfx`` with its SHA-256 digest and size);
+* ``<preset>.structure.w<N>.swfx`` -- ``ref_driver --structure``: the
+  reference's simwing::fsi::Structure stepped 20 times with the flight loads
+  (no face pressure) with workerThreads N (0: the serial sweep, 1: the
+  coloured sweep): definitionFingerprint(), suspensionState() before the
+  first step and after the last, the checkpoint's public fields and
+  serializeStructureCheckpoint's bytes (tiny: the bytes; small: their
+  SHA-256 digest and size);
 * ``export.digest.swfx``      -- for the ~70k-triangle wing: SHA-256 digests
   of every assembly and colouring array, and strided samples plus digests of
   the state after 1 and 240 steps (the full arrays are too large to commit;
@@ -30,6 +37,7 @@ from __future__ import annotations
 import hashlib
 import subprocess
 import sys
+import tempfile
 from pathlib import Path
 
 import numpy as np
@@ -40,7 +48,7 @@ sys.path.insert(0, str(HERE))
 import swfx  # noqa: E402
 
 DRIVER = ROOT / "oracle" / "_ref" / "ref_driver"
-WORK = ROOT / "_work" / "fixtures"
+WORK = Path(tempfile.mkdtemp(prefix="simwing-fixtures-"))
 
 PLAN = {
     "tiny": {"steps": 20, "snapshots": [1, 5, 20], "workers": 12},
@@ -130,6 +138,22 @@ def run_scene(preset: str) -> None:
                    {"digest": digest(data), "size": np.array([data.size], np.uint64)})
 
 
+STRUCTURE_STEPS = 20
+
+
+def run_structure(preset: str, workers: int) -> None:
+    path = WORK / f"{preset}.structure.w{workers}.swfx"
+    cmd = [str(DRIVER), "--structure", preset, str(path), str(STRUCTURE_STEPS), str(workers)]
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    fixture = swfx.read(str(path))
+    if preset != "tiny":
+        data = fixture.pop("checkpoint_bytes")
+        fixture["checkpoint_bytes_digest"] = digest(data)
+        fixture["checkpoint_bytes_size"] = np.array([data.size], np.uint64)
+    swfx.write(str(HERE / f"{preset}.structure.w{workers}.swfx"), fixture)
+
+
 def main(argv: list[str]) -> int:
     if not DRIVER.exists():
         print(f"{DRIVER} is missing: run make -C oracle ref (needs /root/reference)", file=sys.stderr)
@@ -139,6 +163,8 @@ def main(argv: list[str]) -> int:
         if preset in FRAME_STEPS:
             run_frame(preset)
             run_scene(preset)
+            for workers in (0, 1):
+                run_structure(preset, workers)
     return 0
 
 
