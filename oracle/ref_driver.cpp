@@ -262,7 +262,7 @@ struct Rig {
 
 // Structure's buildBody (structure.cpp:554-639) with hinges in `hingeOrder`.
 Rig buildRig(const fsi::StructureDefinition& d, const std::vector<std::size_t>& hingeOrder,
-             const std::vector<double>& facePressure) {
+             const std::vector<double>& facePressure, std::size_t vertexNodes) {
     Rig rig;
     rig.body = std::make_unique<sw::SoftBody>();
     sw::SoftBody& body = *rig.body;
@@ -410,21 +410,24 @@ Rig buildRig(const fsi::StructureDefinition& d, const std::vector<std::size_t>& 
     // Steady flight loads (swx::flightLoads): the system weight carried by
     // the fabric nodes, mass-weighted, radial from the pilot.
     double total = 0.0;
-    const std::size_t vertexNodes =
-        d.nodes.size() - (d.suspension ? d.suspension->junctions.size() : 0);
+
     std::vector<double> pos, mass;
     for (const auto& n : d.nodes) {
         total += n.massKg;
         pos.insert(pos.end(), {n.positionMeters.x, n.positionMeters.y, n.positionMeters.z});
         mass.push_back(n.massKg);
     }
-    swx::V3 centre{};
+    std::vector<std::uint8_t> fixed;
+    for (const auto& n : d.nodes) fixed.push_back(n.fixed ? 1 : 0);
+    swx::V3 pilot{};
     if (d.suspension) {
         total += d.suspension->pilotMassKg;
-        centre = {d.suspension->pilotInitialCenterOfMassWorldMeters.x,
-                  d.suspension->pilotInitialCenterOfMassWorldMeters.y,
-                  d.suspension->pilotInitialCenterOfMassWorldMeters.z};
+        pilot = {d.suspension->pilotInitialCenterOfMassWorldMeters.x,
+                 d.suspension->pilotInitialCenterOfMassWorldMeters.y,
+                 d.suspension->pilotInitialCenterOfMassWorldMeters.z};
     }
+    const swx::V3 centre =
+        swx::flightLoadCentre(pos.data(), fixed.data(), d.nodes.size(), d.suspension ? &pilot : nullptr);
     std::vector<double> loads(3 * d.nodes.size());
     swx::flightLoads(pos.data(), mass.data(), d.nodes.size(), vertexNodes, total * 9.80665, centre,
                      loads.data());
@@ -534,22 +537,25 @@ Prepared prepare(const std::string& preset) {
 namespace viewer = simwing::viewer;
 
 // The flight loads of buildRig, as Structure forces.
-std::vector<fsi::StructureVector3> structureLoads(const fsi::StructureDefinition& d) {
+std::vector<fsi::StructureVector3> structureLoads(const fsi::StructureDefinition& d, std::size_t vertexNodes) {
     double total = 0.0;
-    const std::size_t vertexNodes = d.nodes.size() - (d.suspension ? d.suspension->junctions.size() : 0);
     std::vector<double> pos, mass;
     for (const auto& n : d.nodes) {
         total += n.massKg;
         pos.insert(pos.end(), {n.positionMeters.x, n.positionMeters.y, n.positionMeters.z});
         mass.push_back(n.massKg);
     }
-    swx::V3 centre{};
+    std::vector<std::uint8_t> fixed;
+    for (const auto& n : d.nodes) fixed.push_back(n.fixed ? 1 : 0);
+    swx::V3 pilot{};
     if (d.suspension) {
         total += d.suspension->pilotMassKg;
-        centre = {d.suspension->pilotInitialCenterOfMassWorldMeters.x,
-                  d.suspension->pilotInitialCenterOfMassWorldMeters.y,
-                  d.suspension->pilotInitialCenterOfMassWorldMeters.z};
+        pilot = {d.suspension->pilotInitialCenterOfMassWorldMeters.x,
+                 d.suspension->pilotInitialCenterOfMassWorldMeters.y,
+                 d.suspension->pilotInitialCenterOfMassWorldMeters.z};
     }
+    const swx::V3 centre =
+        swx::flightLoadCentre(pos.data(), fixed.data(), d.nodes.size(), d.suspension ? &pilot : nullptr);
     std::vector<double> loads(3 * d.nodes.size());
     swx::flightLoads(pos.data(), mass.data(), d.nodes.size(), vertexNodes, total * 9.80665, centre, loads.data());
     std::vector<fsi::StructureVector3> out(d.nodes.size());
@@ -604,7 +610,7 @@ int frameMode(const std::string& preset, const std::string& out, int steps) {
     std::vector<fsi::StructureDihedralDefinition> hinges;
     for (const std::size_t index : p.hinges.order) hinges.push_back(def.dihedrals[index]);
     def.dihedrals = std::move(hinges);
-    const std::vector<fsi::StructureVector3> loads = structureLoads(def);
+    const std::vector<fsi::StructureVector3> loads = structureLoads(def, p.assembly.mappings.nodeVertexIds.size());
     fsi::Structure structure(def);
     fsi::StructureStepSettings settings;
     settings.timeStepSeconds = 1.0 / 120.0;
@@ -715,7 +721,7 @@ int structureMode(const std::string& preset, const std::string& out, int steps, 
     std::vector<fsi::StructureDihedralDefinition> hinges;
     for (const std::size_t index : p.hinges.order) hinges.push_back(def.dihedrals[index]);
     def.dihedrals = std::move(hinges);
-    const std::vector<fsi::StructureVector3> loads = structureLoads(def);
+    const std::vector<fsi::StructureVector3> loads = structureLoads(def, p.assembly.mappings.nodeVertexIds.size());
     fsi::Structure structure(def);
     swfx::Writer w(out);
     w.u64("steps", {static_cast<std::uint64_t>(steps)});
@@ -773,8 +779,8 @@ int main(int argc, char** argv) {
             const std::string preset = argv[2];
             const int steps = std::atoi(argv[3]);
             Prepared p = prepare(preset);
-            Rig a = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure);
-            Rig b = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure);
+            Rig a = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure, p.assembly.mappings.nodeVertexIds.size());
+            Rig b = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure, p.assembly.mappings.nodeVertexIds.size());
             for (int s = 0; s < steps; ++s) {
                 stepRig(a, stepSettings(1));
                 stepRig(b, stepSettings(12));
@@ -816,7 +822,7 @@ int main(int argc, char** argv) {
             const unsigned workers = static_cast<unsigned>(std::atoi(argv[3]));
             const double budget = std::atof(argv[4]);
             Prepared p = prepare(preset);
-            Rig rig = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure);
+            Rig rig = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure, p.assembly.mappings.nodeVertexIds.size());
             const sw::StepSettings settings = stepSettings(workers);
             const int warmup = 2;
             for (int s = 0; s < warmup; ++s) stepRig(rig, settings);
@@ -854,7 +860,7 @@ int main(int argc, char** argv) {
             writeAssembly(w, p.assembly, p.hinges.order);
             w.f64("face_pressure", p.wing.facePressure);
         }
-        Rig rig = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure);
+        Rig rig = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure, p.assembly.mappings.nodeVertexIds.size());
         {
             const auto view = rig.body->constraintColouringView();
             swfx::Writer w(prefix + ".colouring.swfx");

@@ -139,8 +139,6 @@ swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st, bool serialAl
     // workerThreads selects the sweep order (structure.h:216-218): 0 the
     // serial definition order, any other count the coloured order.
     s.wing->setSerialSweep(st.worker_threads == 0);
-    if (s.hasLoadPath && st.cable_sweep_pairs > 0)
-        throw std::invalid_argument("load-path cable sweep pairs are not supported by the device path");
     if (s.freeMembrane)
         throw std::invalid_argument(
             "free-membrane momentum restoration is not supported by the device path "
@@ -156,6 +154,14 @@ swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st, bool serialAl
     }
     a.maxLineResidual = coupled ? s.definition.suspension->maximumLineResidual : 0.0;
     a.iterations = st.constraint_iterations;
+    // Load-path cable sweep pairs (soft_body.cpp:1296-1315): a quarter of
+    // them interleaved before the iterations' general sweeps, the rest
+    // closing after them.
+    s.wing->setCablePairs(st.cable_sweep_pairs, st.constraint_iterations);
+    const int interleaved = s.wing->interleavedCablePairs();
+    const int iterations = std::max(1, st.constraint_iterations);
+    a.cableBase = interleaved / iterations;
+    a.cableRem = interleaved % iterations;
     return a;
 }
 
@@ -921,8 +927,10 @@ sw_status sw_assembly_flight_loads(const sw_assembly* a, double* forces) {
         double total = 0.0;
         for (double m : d.nodeMass) total += m;
         if (d.suspension) total += d.suspension->pilotMass;
-        swx::V3 centre{};
-        if (d.suspension) centre = {d.suspension->comWorld[0], d.suspension->comWorld[1], d.suspension->comWorld[2]};
+        swx::V3 pilot{};
+        if (d.suspension) pilot = {d.suspension->comWorld[0], d.suspension->comWorld[1], d.suspension->comWorld[2]};
+        const swx::V3 centre = swx::flightLoadCentre(d.nodePosition.data(), d.nodeFixed.data(), d.nodeCount(),
+                                                     d.suspension ? &pilot : nullptr);
         swx::flightLoads(d.nodePosition.data(), d.nodeMass.data(), d.nodeCount(), a->assembly.nodeVertexIds.size(),
                          total * 9.80665, centre, forces);
     });
@@ -1097,6 +1105,9 @@ sw_status sw_batch_adjoint_configure(sw_batch* b, const sw_step_settings* settin
         std::vector<swx::StepArgs> args;
         for (auto& s : b->wings) {
             args.push_back(prepare(*s, *settings));
+            if (s->wing->interleavedCablePairs() > 0)
+                throw std::invalid_argument(
+                    "the adjoint reverses closing load-path cable pairs only (cableConstraintSweepPairs <= 3)");
             definitions.push_back(&s->definition);
             a->loads.push_back(s->pending);
         }
@@ -1180,6 +1191,9 @@ sw_status sw_adjoint_configure(sw_structure* s, const sw_step_settings* settings
         const swx::StepArgs args = prepare(*s, *settings);
         if (steps < 1 || steps > (1ull << 40)) throw std::invalid_argument("adjoint step count must be positive");
         if (checkpoint_interval < 1) throw std::invalid_argument("checkpoint interval must be positive");
+        if (s->wing->interleavedCablePairs() > 0)
+            throw std::invalid_argument(
+                "the adjoint reverses closing load-path cable pairs only (cableConstraintSweepPairs <= 3)");
         auto a = std::make_unique<sw_adjoint>();
         a->structure = s;
         a->loads = s->pending;

@@ -30,6 +30,9 @@ enum PhaseType : int {
     // phase's [begin, end) indexes the level-ordered list of device slots.
     kPhaseDistanceList = 7,
     kPhaseMembraneList = 8,
+    // An interleaved load-path cable pair's level (flags: the pair's index
+    // within the iteration; [begin, end) indexes the pair's level list).
+    kPhaseCableList = 9,
 };
 
 struct Phase {
@@ -83,6 +86,7 @@ struct StepArgs {
     double reference[3];
     double maxLineResidual;
     int iterations;
+    int cableBase, cableRem;  // interleaved cable pairs: per iteration, and one more before iteration cableRem
 };
 
 class DeviceBatch;
@@ -188,6 +192,12 @@ public:
     // projection equals the serial one); hinges, segments and spheres are
     // the same in both.
     void setSerialSweep(bool serial);
+    // Load-path cable sweep pairs (StepSettings::cableConstraintSweepPairs)
+    // and the iteration count they are split over; rebuilds the programs
+    // when either changes. interleavedCablePairs() is the part run inside
+    // the iterations.
+    void setCablePairs(int pairs, int iterations);
+    int interleavedCablePairs() const { return cableInterleaved_; }
     bool serialSweep() const { return serialSweep_; }
     bool xcdLocal() const { return xcdLocal_; }
     int workgroups() const { return workgroups_; }
@@ -220,6 +230,14 @@ private:
     Phase* serialPhasesDev_ = nullptr;
     std::vector<int> distList_, membList_;  // serial-sweep level order (device slots)
     bool serialSweep_ = false;
+    std::vector<Phase> colouredBase_, serialBase_;   // the programs without cable phases
+    std::vector<int> stitchList_;                    // seam-stitch closing levels (device slots)
+    std::vector<Phase> stitchLevels_;
+    std::vector<int> cablePair_;                     // one cable pair's levels (device slots)
+    std::vector<std::pair<int, int>> cablePairLevels_;
+    int cablePairs_ = 0, cableIterations_ = 1, cableInterleaved_ = 0;
+    std::vector<void*> programAlloc_;                // device copies of the current programs
+    void applyCablePrograms(int pairs, int iterations);
     // Nodes of each item in device slot order (the payload is node nodes_),
     // kept on the host for the synchronisation analysis.
     struct ItemNodes {

@@ -219,6 +219,7 @@ struct WingParams {
     V3 pilotInertia{8.0, 8.0, 8.0};
     int contactSpheres = 0;
     double contactRadius = 0.05;
+    int tether = 0;  // 1: no pilot; lines end at two fixed riser junctions
 };
 
 struct GeneratedWing {
@@ -282,6 +283,11 @@ void applyOverrides(WingParams& params, const std::string& overrides);
 // `out` is 3*nodeCount. This is the surface-flight mass-weighted
 // preview-load pattern (surface_flight_case.cpp:187-214) with a radial
 // rather than a uniform direction.
+// Centre of the flight loads: the pilot's centre of mass when there is one
+// (`pilot` non-null), else the mean position of the fixed nodes (the
+// tethered wing's riser points), else the origin.
+V3 flightLoadCentre(const double* position, const std::uint8_t* fixed, std::size_t nodeCount, const V3* pilot);
+
 void flightLoads(const double* position, const double* mass, std::size_t nodeCount,
                  std::size_t surfaceNodes, double totalWeight, const V3& centre, double* out);
 

@@ -1,6 +1,7 @@
 #include "schedule.hpp"
 
 #include <algorithm>
+#include <limits>
 #include <bit>
 #include <map>
 #include <numeric>
@@ -110,6 +111,69 @@ std::vector<std::uint32_t> dependencyLevels(const std::vector<std::vector<std::u
         if (payload[i]) lastPayload = mine;
     }
     return level;
+}
+
+std::vector<std::uint32_t> loadPathOrder(const Definition& d) {
+    const std::size_t n = d.nodeCount();
+    std::vector<char> structural(n, 0);
+    std::vector<std::vector<std::uint32_t>> neighbours(n);
+    std::vector<std::uint32_t> path;
+    const auto inverseMass = [&](std::size_t i) { return d.nodeFixed[i] != 0 ? 0.0 : 1.0 / d.nodeMass[i]; };
+    for (std::size_t i = 0; i < n; ++i) structural[i] = inverseMass(i) == 0.0 ? 1 : 0;
+    for (std::uint64_t v : d.triangleNodes) structural[v] = 1;
+    for (std::size_t c = 0; c < d.constraintCount(); ++c) {
+        const std::uint32_t a = static_cast<std::uint32_t>(d.constraintNodes[2 * c]);
+        const std::uint32_t b = static_cast<std::uint32_t>(d.constraintNodes[2 * c + 1]);
+        const std::uint8_t kind = d.constraintKind[c];
+        if (kind == SW_CONSTRAINT_DISTANCE || kind == SW_CONSTRAINT_SEAM_STITCH) {
+            structural[a] = structural[b] = 1;
+            continue;
+        }
+        path.push_back(static_cast<std::uint32_t>(c));
+        neighbours[a].push_back(b);
+        neighbours[b].push_back(a);
+    }
+    for (std::uint64_t v : d.dihedralNodes) structural[v] = 1;
+    constexpr std::uint32_t unknown = 0xffffffffu;
+    std::vector<std::uint32_t> depth(n, unknown);
+    std::vector<std::uint32_t> queue;
+    const auto flood = [&](std::size_t head) {
+        for (; head < queue.size(); ++head) {
+            const std::uint32_t x = queue[head];
+            for (std::uint32_t y : neighbours[x])
+                if (depth[y] == unknown) {
+                    depth[y] = depth[x] + 1;
+                    queue.push_back(y);
+                }
+        }
+    };
+    for (std::size_t i = 0; i < n; ++i)
+        if (structural[i] && !neighbours[i].empty()) {
+            depth[i] = 0;
+            queue.push_back(static_cast<std::uint32_t>(i));
+        }
+    flood(0);
+    // Components with no structural node: rooted at the heaviest node (the
+    // smallest inverse mass; the first such node on ties).
+    for (;;) {
+        std::size_t root = n;
+        double lightest = std::numeric_limits<double>::infinity();
+        for (std::size_t i = 0; i < n; ++i)
+            if (depth[i] == unknown && !neighbours[i].empty() && inverseMass(i) < lightest) {
+                root = i;
+                lightest = inverseMass(i);
+            }
+        if (root == n) break;
+        depth[root] = 0;
+        const std::size_t head = queue.size();
+        queue.push_back(static_cast<std::uint32_t>(root));
+        flood(head);
+    }
+    const auto key = [&](std::uint32_t c) {
+        return std::max(depth[d.constraintNodes[2 * c]], depth[d.constraintNodes[2 * c + 1]]);
+    };
+    std::stable_sort(path.begin(), path.end(), [&](std::uint32_t x, std::uint32_t y) { return key(x) < key(y); });
+    return path;
 }
 
 SuspensionLayout suspensionLayout(const Definition& d) {

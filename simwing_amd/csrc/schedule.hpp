@@ -57,4 +57,14 @@ struct SuspensionLayout {
 };
 SuspensionLayout suspensionLayout(const Definition& d);
 
+// Load-path order of the Cable and SuspensionTie constraints
+// (SoftBody::loadPathConstraintOrder, soft_body.cpp:528-629): constraint
+// indices sorted stably by the larger breadth-first depth of their two
+// endpoints over the load-path graph, counted from the structural nodes
+// (fixed, or touched by a triangle, a distance or stitch constraint or a
+// hinge); a component without one is rooted at its node of smallest
+// inverse mass (first in node order). The cable sweep pairs project this
+// order backwards, then forwards (soft_body.cpp:1265-1288).
+std::vector<std::uint32_t> loadPathOrder(const Definition& d);
+
 }  // namespace swx

@@ -324,7 +324,10 @@ GeneratedWing generateWing(const WingParams& P) {
     pilot.mass = P.pilotMass;
     pilot.centerOfMass = V3{0.35 * P.rootChord, 0.0, -P.lineLength - 0.35};
     pilot.principalInertia = P.pilotInertia;
-    S.pilots.push_back(pilot);
+    // Tethered wing: no rigid pilot; the lines end at two fixed riser
+    // junctions where the harness points would be (the scene then carries
+    // its lines as load-path Cable constraints).
+    if (!P.tether) S.pilots.push_back(pilot);
     S.lineMaterials.push_back({1, "line", 0.0015, P.lineLinearDensity, P.lineAxialStiffness, 1.0});
 
     const StableId junctionBase = B.positionById.size() + 1000;
@@ -340,6 +343,18 @@ GeneratedWing generateWing(const WingParams& P) {
     Attachment hr = hl;
     hr.id = harnessRight;
     hr.pilotLocal = localRight;
+    const StableId riserBase =
+        junctionBase + static_cast<StableId>(P.lineCount) * static_cast<StableId>(P.segmentsPerLine) + 1;
+    if (P.tether) {
+        hl.kind = AttachmentKind::SuspensionJunction;
+        hl.pilotId = 0;
+        hl.pilotLocal = V3{};
+        hl.junctionId = riserBase;
+        hr.kind = AttachmentKind::SuspensionJunction;
+        hr.pilotId = 0;
+        hr.pilotLocal = V3{};
+        hr.junctionId = riserBase + 1;
+    }
     std::vector<Attachment> surfaceAttachments, junctionAttachments;
     StableId nextJunctionAttachment = static_cast<StableId>(P.lineCount) + 3;
     const int segs = P.segmentsPerLine;
@@ -432,6 +447,18 @@ GeneratedWing generateWing(const WingParams& P) {
             jn.mass = mass;
         }
     }
+    if (P.tether) {
+        for (int side = 0; side < 2; ++side) {
+            const V3& local = side == 0 ? localLeft : localRight;
+            Junction riser;
+            riser.id = riserBase + static_cast<StableId>(side);
+            riser.position = V3{pilot.centerOfMass.x + local.x, pilot.centerOfMass.y + local.y,
+                                pilot.centerOfMass.z + local.z};
+            riser.mass = 0.5 * P.pilotMass;
+            riser.fixed = true;
+            S.junctions.push_back(riser);
+        }
+    }
     for (const Attachment& a : surfaceAttachments) S.attachments.push_back(a);
     S.attachments.push_back(hl);
     S.attachments.push_back(hr);
@@ -509,6 +536,13 @@ WingParams wingPreset(const std::string& name) {
         p.contactSpheres = 8;
         return p;
     }
+    if (name == "tiny-tethered" || name == "small-tethered") {
+        // The same wings without a rigid pilot, their lines ending at two
+        // fixed riser junctions (load-path cables).
+        WingParams t = wingPreset(name == "tiny-tethered" ? "tiny" : "small");
+        t.tether = 1;
+        return t;
+    }
     if (name == "variant10k") {
         p.seed = 0x5EED0003ULL;
         p.spanNodes = 64;
@@ -540,9 +574,9 @@ void applyOverrides(WingParams& p, const std::string& overrides) {
                                    "pressureMean", "pressureStd", "seamAxialStiffness", "seamLinearDensity",
                                    "pilotMass", "contactRadius"};
         int* ints[] = {&p.spanNodes, &p.chordNodes, &p.ribEvery, &p.lineRows, &p.lineCount, &p.segmentsPerLine,
-                       &p.contactSpheres};
+                       &p.contactSpheres, &p.tether};
         const char* intNames[] = {"spanNodes", "chordNodes", "ribEvery", "lineRows", "lineCount", "segmentsPerLine",
-                                  "contactSpheres"};
+                                  "contactSpheres", "tether"};
         bool found = false;
         for (std::size_t k = 0; k < sizeof(reals) / sizeof(reals[0]); ++k)
             if (key == realNames[k]) {
@@ -606,6 +640,22 @@ std::vector<WingParams> batchPresets(std::uint64_t seed, int count, double minNo
 }  // namespace swx
 
 namespace swx {
+
+V3 flightLoadCentre(const double* position, const std::uint8_t* fixed, std::size_t nodeCount, const V3* pilot) {
+    if (pilot) return *pilot;
+    V3 sum{};
+    std::size_t count = 0;
+    for (std::size_t i = 0; i < nodeCount; ++i) {
+        if (!fixed[i]) continue;
+        sum.x += position[3 * i];
+        sum.y += position[3 * i + 1];
+        sum.z += position[3 * i + 2];
+        ++count;
+    }
+    if (count == 0) return V3{};
+    const double inv = 1.0 / static_cast<double>(count);
+    return V3{sum.x * inv, sum.y * inv, sum.z * inv};
+}
 
 void flightLoads(const double* position, const double* mass, std::size_t nodeCount,
                  std::size_t surfaceNodes, double totalWeight, const V3& centre, double* out) {

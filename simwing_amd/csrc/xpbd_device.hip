@@ -68,6 +68,7 @@ struct WingView {
     int stitchCount;
     int* stitchItems;
     int *distList, *membList;  // serial-sweep levels: device slots in level order
+    int* cableList;            // one load-path cable pair's levels (device slots)
     Phase* iterPhases;
     int iterPhaseCount;
     Phase* closePhases;
@@ -1336,6 +1337,13 @@ __device__ __forceinline__ void segmentLevel(const WingView& v, const NodeRefs& 
 // program; a serial run (kPhaseSegmentRun) is staged and projected by the
 // calling workgroup as a whole. `rec`: the iteration's trace records
 // (recording only).
+// Interleaved load-path cable pairs of iteration `it`
+// (soft_body.cpp:1308-1315): interleaved / iterations, one more for the
+// first interleaved % iterations iterations.
+__device__ __forceinline__ int cablePairsOf(const StepArgs& a, int it) {
+    return a.cableBase + (it < a.cableRem ? 1 : 0);
+}
+
 __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, int first, int end, int stride,
                          PayloadDev& payload, const StepArgs& a, double* rec, const TraceLayout& L) {
     switch (P.type) {
@@ -1372,6 +1380,9 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
             const int k = v.membList[j];
             projectMembrane(v, N, k, a, rec ? rec + L.membBase + kRecMemb * static_cast<long>(k) : nullptr);
         }
+        break;
+    case kPhaseCableList:  // the caller skips pairs beyond the iteration's share; never recorded
+        for (int j = first; j < end; j += stride) projectDistance(v, N, v.cableList[j], a.invDt2, nullptr);
         break;
     default:
         break;
@@ -1625,6 +1636,10 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     waitStagedDeps(v, S, ph, step);
                     runPhase(v, N, P, first, e, stride, payload, a, rec, layout);
                     break;
+                case kPhaseCableList:
+                    waitStagedDeps(v, S, ph, step);
+                    if (P.flags < cablePairsOf(a, it)) runPhase(v, N, P, first, e, stride, payload, a, rec, layout);
+                    break;
                 default:
                     waitStagedDeps(v, S, ph, step);
                     break;
@@ -1646,7 +1661,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                 const Phase P = v.iterPhases[ph];
                 if (P.type == kPhaseSegmentRun) {
                     if (block == 0) runPhase(v, N, P, P.begin, P.end, 1, payload, a, rec, layout);
-                } else {
+                } else if (P.type != kPhaseCableList || P.flags < cablePairsOf(a, it)) {
                     runPhase(v, N, P, P.begin + gtid, P.end, gstride, payload, a, rec, layout);
                 }
                 gridSync(v, workers);
@@ -2123,6 +2138,7 @@ DeviceWing::~DeviceWing() {
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : allocations_) (void)hipFree(p);
+    for (void* p : programAlloc_) (void)hipFree(p);
     for (void* p : depAlloc_)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
@@ -2552,16 +2568,44 @@ void DeviceWing::build(const Definition& d) {
         std::iota(slots.begin(), slots.end(), 0u);
         std::stable_sort(slots.begin(), slots.end(),
                          [&](std::uint32_t x, std::uint32_t y) { return level[x] < level[y]; });
-        v.stitchCount = static_cast<int>(items.size());
-        v.stitchItems = deviceCopy(allocations_, permute(items, slots));
-        closePhases_.clear();
+        stitchList_ = permute(items, slots);
+        stitchLevels_.clear();
         std::size_t s = 0;
         while (s < slots.size()) {
             std::size_t e = s;
             while (e < slots.size() && level[slots[e]] == level[slots[s]]) ++e;
-            closePhases_.push_back({kPhaseStitches, static_cast<int>(s), static_cast<int>(e), 0});
+            stitchLevels_.push_back({kPhaseStitches, static_cast<int>(s), static_cast<int>(e), 0});
             s = e;
         }
+    }
+    // One load-path cable pair (soft_body.cpp:1265-1288): the load-path
+    // order backwards, then forwards, as dependency levels of device slots.
+    {
+        std::vector<std::uint32_t> slotOf(constraints_);
+        for (std::size_t k = 0; k < distance_.order.size(); ++k) slotOf[distance_.order[k]] = static_cast<std::uint32_t>(k);
+        const std::vector<std::uint32_t> path = loadPathOrder(d);
+        std::vector<std::uint32_t> sequence(path.rbegin(), path.rend());
+        sequence.insert(sequence.end(), path.begin(), path.end());
+        std::vector<std::vector<std::uint32_t>> writes;
+        for (std::uint32_t c : sequence)
+            writes.push_back({static_cast<std::uint32_t>(d.constraintNodes[2 * c]),
+                              static_cast<std::uint32_t>(d.constraintNodes[2 * c + 1])});
+        const auto level = dependencyLevels(writes, writes, std::vector<std::uint8_t>(writes.size(), 0), nodes_);
+        std::vector<std::uint32_t> byLevel(sequence.size());
+        std::iota(byLevel.begin(), byLevel.end(), 0u);
+        std::stable_sort(byLevel.begin(), byLevel.end(),
+                         [&](std::uint32_t x, std::uint32_t y) { return level[x] < level[y]; });
+        cablePair_.clear();
+        cablePairLevels_.clear();
+        for (std::uint32_t i : byLevel) cablePair_.push_back(static_cast<int>(slotOf[sequence[i]]));
+        std::size_t s = 0;
+        while (s < byLevel.size()) {
+            std::size_t e = s;
+            while (e < byLevel.size() && level[byLevel[e]] == level[byLevel[s]]) ++e;
+            cablePairLevels_.push_back({static_cast<int>(s), static_cast<int>(e)});
+            s = e;
+        }
+        v.cableList = deviceCopy(allocations_, cablePair_);
     }
     hasRun_ = false;
     for (const Phase& P : iterPhases_) hasRun_ = hasRun_ || P.type == kPhaseSegmentRun;
@@ -2607,14 +2651,10 @@ void DeviceWing::build(const Definition& d) {
             if (P.type != kPhaseDistance && P.type != kPhaseMembrane) serialPhases_.push_back(P);
         v.distList = deviceCopy(allocations_, distList_);
         v.membList = deviceCopy(allocations_, membList_);
-        serialPhasesDev_ = deviceCopy(allocations_, serialPhases_);
     }
-    colouredPhases_ = iterPhases_;
-    colouredPhasesDev_ = deviceCopy(allocations_, iterPhases_);
-    v.iterPhases = colouredPhasesDev_;
-    v.iterPhaseCount = static_cast<int>(iterPhases_.size());
-    v.closePhases = deviceCopy(allocations_, closePhases_);
-    v.closePhaseCount = static_cast<int>(closePhases_.size());
+    colouredBase_ = iterPhases_;
+    serialBase_ = serialPhases_;
+    applyCablePrograms(0, 1);
 
     // Payload.
     {
@@ -2728,6 +2768,73 @@ void DeviceWing::setLaunch(int workgroups, int threads) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamicLds)),
               "shared memory attribute");
     }
+    buildSync();
+    setXcdLocal(xcdRequested_);
+}
+
+// The programs for `pairs` load-path cable sweep pairs and `iterations`
+// iterations (soft_body.cpp:1296-1315, 1380-1388): the interleaved pairs as
+// cable phases at the start of the iteration program (pair j of an
+// iteration runs when j < the iteration's share), the closing pairs and
+// then the seam-stitch levels as the closing program (with no iterations,
+// every pair closes and no stitch pass runs).
+void DeviceWing::applyCablePrograms(int pairs, int iterations) {
+    WingView& v = *view_;
+    cablePairs_ = pairs;
+    cableIterations_ = iterations;
+    int closing = pairs > 0 ? pairs - pairs / 4 : 0;
+    int interleaved = pairs - closing;
+    if (iterations == 0) {
+        closing = pairs;
+        interleaved = 0;
+    }
+    if (cablePair_.empty()) closing = interleaved = 0;
+    for (void* p : programAlloc_) (void)hipFree(p);
+    programAlloc_.clear();
+    // Closing program.
+    std::vector<int> closeList;
+    closePhases_.clear();
+    for (int c = 0; c < closing; ++c) {
+        const int base = static_cast<int>(closeList.size());
+        closeList.insert(closeList.end(), cablePair_.begin(), cablePair_.end());
+        for (const auto& L : cablePairLevels_) closePhases_.push_back({kPhaseStitches, base + L.first, base + L.second, 0});
+    }
+    if (iterations > 0) {
+        const int base = static_cast<int>(closeList.size());
+        closeList.insert(closeList.end(), stitchList_.begin(), stitchList_.end());
+        for (const Phase& P : stitchLevels_) closePhases_.push_back({kPhaseStitches, base + P.begin, base + P.end, 0});
+    }
+    v.stitchCount = static_cast<int>(closeList.size());
+    v.stitchItems = deviceCopy(programAlloc_, closeList);
+    v.closePhases = deviceCopy(programAlloc_, closePhases_);
+    v.closePhaseCount = static_cast<int>(closePhases_.size());
+    // Iteration programs.
+    std::vector<Phase> prefix;
+    const int iters = std::max(1, iterations);
+    const int share = interleaved / iters + (interleaved % iters > 0 ? 1 : 0);
+    for (int j = 0; j < share; ++j)
+        for (const auto& L : cablePairLevels_) prefix.push_back({kPhaseCableList, L.first, L.second, j});
+    cableInterleaved_ = interleaved;
+    colouredPhases_ = prefix;
+    colouredPhases_.insert(colouredPhases_.end(), colouredBase_.begin(), colouredBase_.end());
+    serialPhases_ = prefix;
+    serialPhases_.insert(serialPhases_.end(), serialBase_.begin(), serialBase_.end());
+    colouredPhasesDev_ = deviceCopy(programAlloc_, colouredPhases_);
+    serialPhasesDev_ = deviceCopy(programAlloc_, serialPhases_);
+    iterPhases_ = serialSweep_ ? serialPhases_ : colouredPhases_;
+    v.iterPhases = serialSweep_ ? serialPhasesDev_ : colouredPhasesDev_;
+    v.iterPhaseCount = static_cast<int>(iterPhases_.size());
+}
+
+void DeviceWing::setCablePairs(int pairs, int iterations) {
+    if (cablePair_.empty()) pairs = 0;  // no load-path constraint: the pairs do nothing
+    if (pairs == cablePairs_ && (iterations > 0) == (cableIterations_ > 0) &&
+        (pairs == 0 || iterations == cableIterations_))
+        return;
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipStreamSynchronize(stream_), "synchronise before rebuilding the programs");
+    applyCablePrograms(pairs, iterations);
+    ++launchGeneration_;
     buildSync();
     setXcdLocal(xcdRequested_);
 }
@@ -2872,6 +2979,9 @@ void DeviceWing::itemAccess(int type, int k, std::vector<int>& reads, std::vecto
         break;
     case kPhaseDistanceList:
         writes.assign(items_.dist.begin() + 2 * distList_[k], items_.dist.begin() + 2 * distList_[k] + 2);
+        break;
+    case kPhaseCableList:
+        writes.assign(items_.dist.begin() + 2 * cablePair_[k], items_.dist.begin() + 2 * cablePair_[k] + 2);
         break;
     case kPhaseMembraneList:
         writes.assign(items_.memb.begin() + 3 * membList_[k], items_.memb.begin() + 3 * membList_[k] + 3);

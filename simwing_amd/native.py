@@ -21,6 +21,12 @@ SW_LOGIC_ERROR = 4
 SW_DEVICE_ERROR = 5
 SW_BAD_ALLOC = 6
 
+# Constraint kinds (sw_definition.constraint_kind).
+SW_CONSTRAINT_DISTANCE = 1
+SW_CONSTRAINT_CABLE = 2
+SW_CONSTRAINT_SUSPENSION_TIE = 3
+SW_CONSTRAINT_SEAM_STITCH = 4
+
 _U64P = C.POINTER(C.c_uint64)
 _F64P = C.POINTER(C.c_double)
 _U8P = C.POINTER(C.c_uint8)

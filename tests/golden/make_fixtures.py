@@ -24,7 +24,8 @@ wings, and writes into tests/golden/:
   first step and after the last, the checkpoint's public fields and
   serializeStructureCheckpoint's bytes (tiny: the bytes; small: their
   SHA-256 digest and size);
-* ``export.digest.swfx``      -- for the ~70k-triangle wing: SHA-256 digests
+* ``<preset>.digest.swfx``    -- for the ~70k-triangle wing (export) and the
+  tethered small wing (small-tethered): SHA-256 digests
   of every assembly and colouring array, and strided samples plus digests of
   the state after 1 and 240 steps (the full arrays are too large to commit;
   dihedral arrays are in the reference's hinge insertion order, which is the
@@ -54,8 +55,14 @@ PLAN = {
     "tiny": {"steps": 20, "snapshots": [1, 5, 20], "workers": 12},
     "small": {"steps": 120, "snapshots": [1, 120], "workers": 12},
     "export": {"steps": 240, "snapshots": [1, 240], "workers": 8},
+    # The same wings tethered at two fixed riser junctions: no pilot, the
+    # lines are load-path Cable constraints swept in pairs.
+    "tiny-tethered": {"steps": 20, "snapshots": [1, 5, 20], "workers": 12},
+    "small-tethered": {"steps": 20, "snapshots": [1, 20], "workers": 12},
 }
 STRIDE = 101
+# Written as digests and strided samples (the full arrays are too large).
+DIGESTED = {"export", "small-tethered"}
 FRAME_STEPS = {"tiny": 20, "small": 40}
 
 
@@ -75,7 +82,7 @@ def run(preset: str) -> None:
     assembly = swfx.read(f"{prefix}.assembly.swfx")
     colouring = swfx.read(f"{prefix}.colouring.swfx")
     states = swfx.read(f"{prefix}.states.swfx")
-    if preset != "export":
+    if preset not in DIGESTED:
         swfx.write(str(HERE / f"{preset}.assembly.swfx"), assembly)
         swfx.write(str(HERE / f"{preset}.colouring.swfx"), colouring)
         keep = {k: v for k, v in states.items() if k != "state_hash"}
@@ -98,9 +105,10 @@ def run(preset: str) -> None:
             out[f"{tag}/{key}/samples"] = rows[picked].reshape(-1).copy()
             out[f"{tag}/{key}/maxabs"] = np.array([np.abs(value).max()])
         for key in ("segment_lambda", "payload", "suspension_residual"):
-            out[f"{tag}/{key}"] = states[f"{tag}/{key}"]
+            if f"{tag}/{key}" in states:  # absent without a pilot
+                out[f"{tag}/{key}"] = states[f"{tag}/{key}"]
     out["dihedral_colour_order_digest"] = digest(assembly["dihedral_colour_order"])
-    swfx.write(str(HERE / "export.digest.swfx"), out)
+    swfx.write(str(HERE / f"{preset}.digest.swfx"), out)
 
 
 def run_frame(preset: str) -> None:
