@@ -415,6 +415,7 @@ struct Segment {
     long from, to;  // node index, or -(hang + 1)
     Real rest, stiffness, lambda = 0.0;
     std::size_t definition = 0;  // index in the definition's segment arrays
+    Real damping = 0.0;          // axial damping, N s/m
 };
 struct Pair {
     std::size_t node, centre;
@@ -772,6 +773,119 @@ std::vector<std::size_t> loadPathOrder(const oracle_body& b) {
     return order;
 }
 
+// soft_body.cpp:118-150
+struct Momentum {
+    Real mass = 0.0;
+    V centre, linear, angular;
+};
+Momentum measureMomentum(const std::vector<Node>& nodes) {
+    Momentum r;
+    for (const Node& n : nodes) {
+        if (n.w <= 0.0) continue;
+        const Real m = 1.0 / n.w;
+        r.mass += m;
+        r.centre = r.centre + m * n.position;
+        r.linear = r.linear + m * n.velocity;
+    }
+    if (!(r.mass > 0.0)) return r;
+    r.centre = r.centre / r.mass;
+    for (const Node& n : nodes) {
+        if (n.w <= 0.0) continue;
+        const Real m = 1.0 / n.w;
+        r.angular = r.angular + m * cross(n.position - r.centre, n.velocity);
+    }
+    return r;
+}
+
+// soft_body.cpp:152-190
+void restoreMomentum(std::vector<Node>& nodes, const Momentum& target) {
+    if (!(target.mass > 0.0)) return;
+    Momentum current = measureMomentum(nodes);
+    const V translation = (target.linear - current.linear) / target.mass;
+    for (Node& n : nodes)
+        if (n.w > 0.0) n.velocity = n.velocity + translation;
+    current = measureMomentum(nodes);
+    Sym inertia;
+    for (const Node& n : nodes) {
+        if (n.w <= 0.0) continue;
+        const Real m = 1.0 / n.w;
+        const V r = n.position - current.centre;
+        inertia.xx += m * (r.y * r.y + r.z * r.z);
+        inertia.yy += m * (r.x * r.x + r.z * r.z);
+        inertia.zz += m * (r.x * r.x + r.y * r.y);
+        inertia.xy -= m * r.x * r.y;
+        inertia.xz -= m * r.x * r.z;
+        inertia.yz -= m * r.y * r.z;
+    }
+    if (!symPositiveDefinite(inertia)) return;
+    const V omega = checkedSolve(inertia, target.angular - current.angular);
+    for (Node& n : nodes)
+        if (n.w > 0.0) n.velocity = n.velocity + cross(omega, n.position - current.centre);
+}
+
+// suspension.cpp:1165-1185
+Real boundedAxialDampingImpulse(Real relative, Real damping, Real dt, Real effective) {
+    if (!isfinite(relative) || !(damping >= 0.0) || !isfinite(damping) || !(dt > 0.0) || !isfinite(dt) ||
+        !(effective >= 0.0) || !isfinite(effective))
+        fail(6, "line-damping", "invalid axial damping input");
+    if (damping == 0.0 || effective == 0.0 || relative == 0.0) return 0.0;
+    const Real requested = damping * abs(relative) * dt;
+    const Real stopping = abs(relative) / effective;
+    const Real bounded = std::min(requested, stopping);  // >= 0; copysign onto a nonzero `relative`
+    return relative < 0.0 ? -bounded : bounded;
+}
+
+// suspension.cpp:2413-2479 (free payload): axial damping impulses along every
+// taut, damped segment in runtime (string-id) order.
+void lineDamping(oracle_body& b, Real dt) {
+    bool any = false;
+    for (const Segment& s : b.segments) any = any || (s.lambda > 0.0 && s.damping > 0.0);
+    if (!any) return;
+    Payload& p = b.payload;
+    const M3 inverseInertia = inverse3(worldInertia(p, p.q));
+    const auto offsetOf = [&](long e) { return rotation(p.q) * (p.hang[-e - 1] - p.comLocal); };
+    const auto pointVelocity = [&](long e) {
+        if (e >= 0) return b.nodes[e].velocity;
+        return p.v + cross(p.w, offsetOf(e));
+    };
+    const auto apply = [&](long e, const V& point, const V& impulse) {
+        if (e >= 0) {
+            Node& n = b.nodes[e];
+            if (n.w > 0.0) n.velocity = n.velocity + impulse * n.w;
+        } else {
+            p.v = p.v + impulse / p.mass;
+            p.w = p.w + inverseInertia * cross(point - p.com, impulse);
+        }
+    };
+    for (const Segment& s : b.segments) {
+        if (!(s.lambda > 0.0) || !(s.damping > 0.0)) continue;
+        const V from = s.from >= 0 ? b.nodes[s.from].position : hangPosition(p, -s.from - 1);
+        const V to = s.to >= 0 ? b.nodes[s.to].position : hangPosition(p, -s.to - 1);
+        const V d = to - from;
+        const Real magnitude = length(d);
+        const V direction = magnitude > 0.0 ? d / magnitude : V{};
+        const Real relative = dot(pointVelocity(s.to) - pointVelocity(s.from), direction);
+        Real effective = 0.0;
+        if (s.from >= 0) {
+            effective += b.nodes[s.from].w;
+        } else {
+            const V rxn = cross(from - p.com, direction);
+            effective += 1.0 / p.mass + dot(rxn, inverseInertia * rxn);
+        }
+        if (s.to >= 0) {
+            effective += b.nodes[s.to].w;
+        } else {
+            const V rxn = cross(to - p.com, direction);
+            effective += 1.0 / p.mass + dot(rxn, inverseInertia * rxn);
+        }
+        if (!(effective > 0.0) || relative == 0.0) continue;
+        const Real impulse = boundedAxialDampingImpulse(relative, s.damping, dt, effective);
+        const V fromImpulse = direction * impulse;
+        apply(s.from, from, fromImpulse);
+        apply(s.to, to, -fromImpulse);
+    }
+}
+
 // soft_body.cpp:1175-1460 (coupled; the coloured sweep for workerThreads >= 1,
 // the serial definition-order sweep for workerThreads = 0) for one substep.
 void substep(oracle_body& b, Real dt, const sw_step_settings& st) {
@@ -806,11 +920,11 @@ void substep(oracle_body& b, Real dt, const sw_step_settings& st) {
         n.velocity = ref + (n.velocity + acc * dt - ref) * damping;
         n.position += n.velocity * dt;
     }
-    if (!b.hasSuspension && (!b.membranes.empty() || !b.hinges.empty())) {
-        bool allDynamic = true;
-        for (const Node& n : b.nodes) allDynamic = allDynamic && n.w > 0.0;
-        if (allDynamic) throw std::logic_error("the oracle does not restate free-membrane momentum restoration");
-    }
+    // soft_body.cpp:1193-1205: an isolated free membrane keeps its predicted
+    // momentum (contact spheres stand in for the reference's contact pairs).
+    bool preserveMomentum = !b.hasSuspension && (!b.membranes.empty() || !b.hinges.empty()) && b.pairs.empty();
+    for (const Node& n : b.nodes) preserveMomentum = preserveMomentum && n.w > 0.0;
+    const Momentum predicted = preserveMomentum ? measureMomentum(b.nodes) : Momentum{};
     for (auto& c : b.constraints) c.lambda = 0.0;
     for (auto& m : b.membranes) {
         m.lambda = {};
@@ -874,6 +988,7 @@ void substep(oracle_body& b, Real dt, const sw_step_settings& st) {
     }
     const Real invDt = 1.0 / dt;
     for (Node& n : b.nodes) n.velocity = n.w == 0.0 ? V{} : (n.position - n.previous) * invDt;
+    if (preserveMomentum) restoreMomentum(b.nodes, predicted);  // soft_body.cpp:1446-1448
     if (b.hasSuspension) {
         Payload& p = b.payload;
         // suspension.cpp:2386-2411
@@ -884,6 +999,7 @@ void substep(oracle_body& b, Real dt, const sw_step_settings& st) {
         const Real sine = mSin(half);
         if (abs(sine) < 1.0e-12) p.w = V{delta.x, delta.y, delta.z} * (2.0 / dt);
         else p.w = V{delta.x / sine, delta.y / sine, delta.z / sine} * (2.0 * half / dt);
+        lineDamping(b, dt);
         // suspension.cpp:2324-2384
         const Real n2 = p.q.w * p.q.w + p.q.x * p.q.x + p.q.y * p.q.y + p.q.z * p.q.z;
         if (!finite(p.com) || !finite(p.q) || !finite(p.v) || !finite(p.w) || abs(n2 - 1.0) > 1.0e-12)
@@ -1045,7 +1161,7 @@ int oracle_create(const sw_definition* d, oracle_body** out) {
                 b->segments.push_back({"segment:" + std::to_string(s.segment_ids[k]),
                                        endpoint.at(name(s.segment_from_kind[k], s.segment_from_id[k])),
                                        endpoint.at(name(s.segment_to_kind[k], s.segment_to_id[k])), s.segment_rest[k],
-                                       s.segment_stiffness[k], 0.0, k});
+                                       s.segment_stiffness[k], 0.0, k, s.segment_damping[k]});
             std::sort(b->segments.begin(), b->segments.end(),
                       [](const Segment& l, const Segment& r) { return l.id < r.id; });
         }

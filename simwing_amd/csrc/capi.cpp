@@ -109,14 +109,22 @@ struct sw_structure {
     bool suspensionStepped = false;
     std::vector<std::string> segmentNames;  // runtime order
     bool hasLoadPath = false;
-    // An isolated free membrane (no suspension, every node dynamic, membranes
-    // or hinges present): the reference restores its predicted momentum
-    // after each substep (soft_body.cpp:1196-1205, 1446-1448), which the
-    // device path does not compute.
-    bool freeMembrane = false;
 };
 
 namespace {
+
+// What the adjoints do not differentiate: load-path cable pairs interleaved
+// with the iterations, a free membrane's momentum restoration and axial
+// line damping.
+void requireDifferentiable(const sw_structure& s) {
+    if (s.wing->interleavedCablePairs() > 0)
+        throw std::invalid_argument(
+            "the adjoint reverses closing load-path cable pairs only (cableConstraintSweepPairs <= 3)");
+    if (s.wing->preservesMomentum())
+        throw std::invalid_argument("the adjoint does not differentiate free-membrane momentum restoration");
+    if (s.wing->dampedSegments() > 0)
+        throw std::invalid_argument("the adjoint does not differentiate axial line damping");
+}
 
 // SoftBody::stepCoupled / step setting checks (soft_body.cpp:981-1050).
 // `serialAllowed`: the caller can run the serial sweep (workerThreads = 0);
@@ -139,10 +147,6 @@ swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st, bool serialAl
     // workerThreads selects the sweep order (structure.h:216-218): 0 the
     // serial definition order, any other count the coloured order.
     s.wing->setSerialSweep(st.worker_threads == 0);
-    if (s.freeMembrane)
-        throw std::invalid_argument(
-            "free-membrane momentum restoration is not supported by the device path "
-            "(fix a node, or step the membrane with a suspension)");
     swx::StepArgs a{};
     a.dt = st.time_step / st.substeps;
     a.invDt = 1.0 / a.dt;
@@ -176,6 +180,8 @@ swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st, bool serialAl
                                           : "payload state became non-finite or degenerate";
         throw swx::SuspensionFailure(e.phase, entity, what);
     }
+    if (e.code == SW_INVALID_ARGUMENT && e.phase == 9)
+        throw std::invalid_argument("SymmetricMatrix3 is singular or ill-conditioned (momentum restoration)");
     if (e.code == SW_INVALID_ARGUMENT)
         throw std::invalid_argument("SymmetricMatrix3 is singular or ill-conditioned; membrane element slot=" +
                                     std::to_string(e.index));
@@ -309,12 +315,6 @@ sw_status sw_structure_create(const sw_definition* definition, int device, sw_st
         for (std::size_t c = 0; c < s->definition.constraintCount(); ++c) {
             const auto kind = s->definition.constraintKind[c];
             if (kind == SW_CONSTRAINT_CABLE || kind == SW_CONSTRAINT_SUSPENSION_TIE) s->hasLoadPath = true;
-        }
-        {
-            const swx::Definition& d = s->definition;
-            bool allDynamic = true;
-            for (std::size_t i = 0; i < d.nodeCount(); ++i) allDynamic = allDynamic && d.nodeFixed[i] == 0;
-            s->freeMembrane = !d.suspension && (d.membraneCount() > 0 || d.dihedralCount() > 0) && allDynamic;
         }
         s->wing = std::make_unique<swx::DeviceWing>(s->definition, device);
         s->device = device;
@@ -1105,9 +1105,7 @@ sw_status sw_batch_adjoint_configure(sw_batch* b, const sw_step_settings* settin
         std::vector<swx::StepArgs> args;
         for (auto& s : b->wings) {
             args.push_back(prepare(*s, *settings));
-            if (s->wing->interleavedCablePairs() > 0)
-                throw std::invalid_argument(
-                    "the adjoint reverses closing load-path cable pairs only (cableConstraintSweepPairs <= 3)");
+            requireDifferentiable(*s);
             definitions.push_back(&s->definition);
             a->loads.push_back(s->pending);
         }
@@ -1191,9 +1189,7 @@ sw_status sw_adjoint_configure(sw_structure* s, const sw_step_settings* settings
         const swx::StepArgs args = prepare(*s, *settings);
         if (steps < 1 || steps > (1ull << 40)) throw std::invalid_argument("adjoint step count must be positive");
         if (checkpoint_interval < 1) throw std::invalid_argument("checkpoint interval must be positive");
-        if (s->wing->interleavedCablePairs() > 0)
-            throw std::invalid_argument(
-                "the adjoint reverses closing load-path cable pairs only (cableConstraintSweepPairs <= 3)");
+        requireDifferentiable(*s);
         auto a = std::make_unique<sw_adjoint>();
         a->structure = s;
         a->loads = s->pending;

@@ -199,6 +199,10 @@ public:
     void setCablePairs(int pairs, int iterations);
     int interleavedCablePairs() const { return cableInterleaved_; }
     bool serialSweep() const { return serialSweep_; }
+    // A free membrane (momentum restored after each substep) and damped
+    // suspension lines: stepped, not differentiated.
+    bool preservesMomentum() const { return preserveMomentum_; }
+    std::size_t dampedSegments() const { return dampedSegments_; }
     bool xcdLocal() const { return xcdLocal_; }
     int workgroups() const { return workgroups_; }
     int threads() const { return threads_; }
@@ -230,6 +234,8 @@ private:
     Phase* serialPhasesDev_ = nullptr;
     std::vector<int> distList_, membList_;  // serial-sweep level order (device slots)
     bool serialSweep_ = false;
+    bool preserveMomentum_ = false;
+    std::size_t dampedSegments_ = 0;
     std::vector<Phase> colouredBase_, serialBase_;   // the programs without cable phases
     std::vector<int> stitchList_;                    // seam-stitch closing levels (device slots)
     std::vector<Phase> stitchLevels_;

@@ -219,7 +219,7 @@ struct WingParams {
     V3 pilotInertia{8.0, 8.0, 8.0};
     int contactSpheres = 0;
     double contactRadius = 0.05;
-    int tether = 0;  // 1: no pilot; lines end at two fixed riser junctions
+    int tether = 0;  // no pilot; lines end at two riser junctions, 1: fixed, 2: dynamic
 };
 
 struct GeneratedWing {

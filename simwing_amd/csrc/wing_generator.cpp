@@ -455,7 +455,7 @@ GeneratedWing generateWing(const WingParams& P) {
             riser.position = V3{pilot.centerOfMass.x + local.x, pilot.centerOfMass.y + local.y,
                                 pilot.centerOfMass.z + local.z};
             riser.mass = 0.5 * P.pilotMass;
-            riser.fixed = true;
+            riser.fixed = P.tether == 1;
             S.junctions.push_back(riser);
         }
     }
@@ -541,6 +541,14 @@ WingParams wingPreset(const std::string& name) {
         // fixed riser junctions (load-path cables).
         WingParams t = wingPreset(name == "tiny-tethered" ? "tiny" : "small");
         t.tether = 1;
+        return t;
+    }
+    if (name == "tiny-free" || name == "small-free") {
+        // ... ending at two dynamic riser junctions of half the pilot's
+        // mass each: every node is dynamic and there is no suspension, so
+        // the reference restores the free membrane's momentum.
+        WingParams t = wingPreset(name == "tiny-free" ? "tiny" : "small");
+        t.tether = 2;
         return t;
     }
     if (name == "variant10k") {

@@ -69,6 +69,11 @@ struct WingView {
     int* stitchItems;
     int *distList, *membList;  // serial-sweep levels: device slots in level order
     int* cableList;            // one load-path cable pair's levels (device slots)
+    int preserveMomentum;      // free membrane: restore the predicted momentum
+    double* momentum;          // mass, centre, linear, angular after prediction
+    int dampCount;             // segments with axial damping, runtime order:
+    int* dampSlot;             //   device slot
+    double* dampValue;         //   damping (N s/m)
     Phase* iterPhases;
     int iterPhaseCount;
     Phase* closePhases;
@@ -1277,9 +1282,9 @@ __device__ bool beginPayload(PayloadDev& p, const StepArgs& a) {
     return true;
 }
 
-// reconstructPayloadVelocity (suspension.cpp:2386-2411) and the payload
-// part of certifySubstep (suspension.cpp:2324-2333).
-__device__ bool finishPayload(PayloadDev& p, const StepArgs& a) {
+// reconstructPayloadVelocity (suspension.cpp:2386-2411). False when the
+// pose delta does not normalise.
+__device__ bool reconstructPayloadVelocity(PayloadDev& p, const StepArgs& a) {
     const d3 vel{(p.com[0] - p.prevCom[0]) / a.dt, (p.com[1] - p.prevCom[1]) / a.dt,
                  (p.com[2] - p.prevCom[2]) / a.dt};
     const quat q{p.q[0], p.q[1], p.q[2], p.q[3]};
@@ -1300,8 +1305,229 @@ __device__ bool finishPayload(PayloadDev& p, const StepArgs& a) {
     p.w[0] = omega.x;
     p.w[1] = omega.y;
     p.w[2] = omega.z;
+    return true;
+}
+
+// The payload part of certifySubstep (suspension.cpp:2324-2333), after the
+// line damping has changed the payload's velocities.
+__device__ bool payloadCertified(const PayloadDev& p) {
+    const quat q{p.q[0], p.q[1], p.q[2], p.q[3]};
     const double n2 = qnorm2(q);
-    return fin3(d3{p.com[0], p.com[1], p.com[2]}) && finQ(q) && fin3(vel) && fin3(omega) && !(fabs(n2 - 1.0) > 1.0e-12);
+    return fin3(d3{p.com[0], p.com[1], p.com[2]}) && finQ(q) && fin3(d3{p.v[0], p.v[1], p.v[2]}) &&
+           fin3(d3{p.w[0], p.w[1], p.w[2]}) && !(fabs(n2 - 1.0) > 1.0e-12);
+}
+
+// Lane j's value of `x` for a wave-uniform j.
+__device__ __forceinline__ double laneValue(double x, int j) {
+    const long long bits = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), j);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), j);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// measureMomentum (soft_body.cpp:118-149) over the dynamic nodes in index
+// order. Run by one whole wave: the lanes load 64 nodes at a time and every
+// lane adds them up in node order, so the sums are the serial ones.
+struct MomentumDev {
+    double mass;
+    d3 centre, linear, angular;
+};
+
+template <class Store>
+__device__ MomentumDev measureMomentum(const WingView& v, const Store& N) {
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    MomentumDev r{0.0, d3{0.0, 0.0, 0.0}, d3{0.0, 0.0, 0.0}, d3{0.0, 0.0, 0.0}};
+    const int n = v.nodeCount;
+    for (int base = 0; base < n; base += 64) {
+        const int i = base + lane;
+        const bool live = i < n;
+        const double w = live ? N.w[i] : 0.0;
+        const d3 x = live ? N.get(i) : d3{0.0, 0.0, 0.0};
+        const d3 u = live ? d3{v.vx[i], v.vy[i], v.vz[i]} : d3{0.0, 0.0, 0.0};
+        const int count = n - base < 64 ? n - base : 64;
+        for (int j = 0; j < count; ++j) {
+            const double wj = laneValue(w, j);
+            if (wj <= 0.0) continue;
+            const double m = 1.0 / wj;
+            r.mass += m;
+            r.centre = add(r.centre, scale(d3{laneValue(x.x, j), laneValue(x.y, j), laneValue(x.z, j)}, m));
+            r.linear = add(r.linear, scale(d3{laneValue(u.x, j), laneValue(u.y, j), laneValue(u.z, j)}, m));
+        }
+    }
+    if (!(r.mass > 0.0)) return r;
+    r.centre = divide(r.centre, r.mass);
+    for (int base = 0; base < n; base += 64) {
+        const int i = base + lane;
+        const bool live = i < n;
+        const double w = live ? N.w[i] : 0.0;
+        const d3 x = live ? N.get(i) : d3{0.0, 0.0, 0.0};
+        const d3 u = live ? d3{v.vx[i], v.vy[i], v.vz[i]} : d3{0.0, 0.0, 0.0};
+        const int count = n - base < 64 ? n - base : 64;
+        for (int j = 0; j < count; ++j) {
+            const double wj = laneValue(w, j);
+            if (wj <= 0.0) continue;
+            const double m = 1.0 / wj;
+            const d3 xj{laneValue(x.x, j), laneValue(x.y, j), laneValue(x.z, j)};
+            const d3 uj{laneValue(u.x, j), laneValue(u.y, j), laneValue(u.z, j)};
+            r.angular = add(r.angular, scale(cross(sub(xj, r.centre), uj), m));
+        }
+    }
+    return r;
+}
+
+// restoreMomentum (soft_body.cpp:151-190): a uniform velocity shift that
+// restores the linear momentum, then a rigid rotation about the centre of
+// mass that restores the angular momentum. One whole wave; false when the
+// inertia solve fails (the reference throws from checkedSolve).
+template <class Store>
+__device__ bool restoreMomentum(const WingView& v, const Store& N, const MomentumDev& target) {
+    if (!(target.mass > 0.0)) return true;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int n = v.nodeCount;
+    MomentumDev current = measureMomentum(v, N);
+    const d3 translation = divide(sub(target.linear, current.linear), target.mass);
+    for (int i = lane; i < n; i += 64) {
+        if (!(N.w[i] > 0.0)) continue;
+        v.vx[i] = v.vx[i] + translation.x;
+        v.vy[i] = v.vy[i] + translation.y;
+        v.vz[i] = v.vz[i] + translation.z;
+    }
+    current = measureMomentum(v, N);
+    s3 inertia{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int base = 0; base < n; base += 64) {
+        const int i = base + lane;
+        const bool live = i < n;
+        const double w = live ? N.w[i] : 0.0;
+        const d3 x = live ? N.get(i) : d3{0.0, 0.0, 0.0};
+        const int count = n - base < 64 ? n - base : 64;
+        for (int j = 0; j < count; ++j) {
+            const double wj = laneValue(w, j);
+            if (wj <= 0.0) continue;
+            const double m = 1.0 / wj;
+            const d3 r = sub(d3{laneValue(x.x, j), laneValue(x.y, j), laneValue(x.z, j)}, current.centre);
+            inertia.xx += m * (r.y * r.y + r.z * r.z);
+            inertia.yy += m * (r.x * r.x + r.z * r.z);
+            inertia.zz += m * (r.x * r.x + r.y * r.y);
+            inertia.xy -= m * r.x * r.y;
+            inertia.xz -= m * r.x * r.z;
+            inertia.yz -= m * r.y * r.z;
+        }
+    }
+    if (!positiveDefinite(inertia)) return true;
+    d3 omega;
+    if (!checkedSolve(inertia, sub(target.angular, current.angular), omega)) return false;
+    for (int i = lane; i < n; i += 64) {
+        if (!(N.w[i] > 0.0)) continue;
+        const d3 spin = cross(omega, sub(N.get(i), current.centre));
+        v.vx[i] = v.vx[i] + spin.x;
+        v.vy[i] = v.vy[i] + spin.y;
+        v.vz[i] = v.vz[i] + spin.z;
+    }
+    return true;
+}
+
+// boundedAxialDampingImpulse (suspension.cpp:1165-1185). False on an
+// invalid input (the reference throws).
+__device__ __forceinline__ bool boundedAxialDampingImpulse(double relative, double damping, double dt,
+                                                           double effective, double& out) {
+    if (!isfinite(relative) || !(damping >= 0.0) || !isfinite(damping) || !(dt > 0.0) || !isfinite(dt) ||
+        !(effective >= 0.0) || !isfinite(effective))
+        return false;
+    out = 0.0;
+    if (damping == 0.0 || effective == 0.0 || relative == 0.0) return true;
+    const double requested = damping * fabs(relative) * dt;
+    const double stopping = fabs(relative) / effective;
+    const double bounded = stdmin(requested, stopping);
+    out = relative < 0.0 ? -bounded : bounded;
+    return true;
+}
+
+// applyLineDamping for a free payload (suspension.cpp:2413-2479): an axial
+// impulse along every taut, damped segment, in runtime (string-id) order,
+// each seeing the velocities the earlier ones left. One lane; false after
+// raising a failure.
+template <class Store>
+__device__ bool dampLines(const WingView& v, const Store& N, PayloadDev& payload, const StepArgs& a) {
+    bool any = false;
+    for (int j = 0; j < v.dampCount; ++j) any = any || v.slam[v.dampSlot[j]] > 0.0;
+    if (!any) return true;
+    const Payload state = loadPayload(payload);
+    m3 world, inverseInertia;
+    if (!worldInertia(state.q, payload.inertia, world) || !inverseM(world, inverseInertia)) {
+        raise(v, SW_SUSPENSION_ERROR, 6, 2, 0);
+        return false;
+    }
+    m3 rot;
+    if (!rotationMatrix(state.q, rot)) {
+        raise(v, SW_SUSPENSION_ERROR, 6, 2, 0);
+        return false;
+    }
+    const d3 comLocal{payload.comLocal[0], payload.comLocal[1], payload.comLocal[2]};
+    d3 pv{payload.v[0], payload.v[1], payload.v[2]};
+    d3 pw{payload.w[0], payload.w[1], payload.w[2]};
+    const double mass = payload.mass;
+    for (int j = 0; j < v.dampCount; ++j) {
+        const int k = v.dampSlot[j];
+        if (!(v.slam[k] > 0.0)) continue;
+        const int ends[2] = {v.sfrom[k], v.sto[k]};
+        d3 point[2], velocity[2];
+        double effective = 0.0;
+        Endpoint e[2];
+        for (int s = 0; s < 2; ++s) {
+            if (!endpointState(v, N, ends[s], state, payload, e[s])) {
+                raise(v, SW_SUSPENSION_ERROR, 6, 2, 0);
+                return false;
+            }
+            point[s] = e[s].position;
+            if (ends[s] >= 0) {
+                velocity[s] = d3{v.vx[ends[s]], v.vy[ends[s]], v.vz[ends[s]]};
+            } else {
+                const d3 offset = mulM(rot, sub(N.hang(-ends[s] - 1), comLocal));
+                velocity[s] = add(pv, cross(pw, offset));
+            }
+        }
+        const d3 d = sub(point[1], point[0]);
+        const double magnitude = len(d);
+        const d3 direction = magnitude > 0.0 ? divide(d, magnitude) : d3{0.0, 0.0, 0.0};
+        const double relative = dot(sub(velocity[1], velocity[0]), direction);
+        for (int s = 0; s < 2; ++s) {
+            if (ends[s] >= 0) {
+                effective += N.w[ends[s]];
+            } else {
+                const d3 rxn = cross(sub(point[s], state.com), direction);
+                effective += 1.0 / mass + dot(rxn, mulM(inverseInertia, rxn));
+            }
+        }
+        if (!(effective > 0.0) || relative == 0.0) continue;
+        double impulse;
+        if (!boundedAxialDampingImpulse(relative, v.dampValue[j], a.dt, effective, impulse)) {
+            raise(v, SW_SUSPENSION_ERROR, 6, 1, v.sruntime[k]);
+            return false;
+        }
+        const d3 fromImpulse = scale(direction, impulse);
+        for (int s = 0; s < 2; ++s) {
+            const d3 J = s == 0 ? fromImpulse : neg(fromImpulse);
+            if (ends[s] >= 0) {
+                const double w = N.w[ends[s]];
+                if (w > 0.0) {
+                    const d3 dv = scale(J, w);
+                    v.vx[ends[s]] = v.vx[ends[s]] + dv.x;
+                    v.vy[ends[s]] = v.vy[ends[s]] + dv.y;
+                    v.vz[ends[s]] = v.vz[ends[s]] + dv.z;
+                }
+            } else {
+                pv = add(pv, divide(J, mass));
+                pw = add(pw, mulM(inverseInertia, cross(sub(point[s], state.com), J)));
+            }
+        }
+    }
+    payload.v[0] = pv.x;
+    payload.v[1] = pv.y;
+    payload.v[2] = pv.z;
+    payload.w[0] = pw.x;
+    payload.w[1] = pw.y;
+    payload.w[2] = pw.z;
+    return true;
 }
 
 __device__ __forceinline__ void atomicMaxNonNegative(double* address, double value) {
@@ -1502,6 +1728,26 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         N.put(i, add(N.get(i), scale(vel, a.dt)));
     }
     gridSync(v, blocks);
+    if (v.preserveMomentum) {
+        // The free membrane's momentum after prediction (soft_body.cpp:1196-1205),
+        // restored after finalisation.
+        if (block == 0 && threadIdx.x < 64) {
+            const MomentumDev m = measureMomentum(v, N);
+            if (threadIdx.x == 0) {
+                v.momentum[0] = m.mass;
+                v.momentum[1] = m.centre.x;
+                v.momentum[2] = m.centre.y;
+                v.momentum[3] = m.centre.z;
+                v.momentum[4] = m.linear.x;
+                v.momentum[5] = m.linear.y;
+                v.momentum[6] = m.linear.z;
+                v.momentum[7] = m.angular.x;
+                v.momentum[8] = m.angular.y;
+                v.momentum[9] = m.angular.z;
+            }
+        }
+        gridSync(v, blocks);
+    }
 
     // Constraint iterations.
     if (v.neighbourSync && !shared && blocks > 1) {
@@ -1752,8 +1998,19 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         if (threadIdx.x == 0) atomicMaxNonNegative(&payload.maxResidual, scratch[0]);
     }
     gridSync(v, workers);
+    if (v.preserveMomentum && block == 0 && threadIdx.x < 64) {
+        // soft_body.cpp:1446-1448
+        const MomentumDev target{v.momentum[0], d3{v.momentum[1], v.momentum[2], v.momentum[3]},
+                                 d3{v.momentum[4], v.momentum[5], v.momentum[6]},
+                                 d3{v.momentum[7], v.momentum[8], v.momentum[9]}};
+        if (!restoreMomentum(v, N, target) && threadIdx.x == 0) raise(v, SW_INVALID_ARGUMENT, 9, 2, 0);
+    }
     if (gtid == 0 && v.segCount > 0) {
-        if (!finishPayload(payload, a)) raise(v, SW_SUSPENSION_ERROR, 8, 2, 0);
+        // Payload velocity, line damping, then the payload and line-residual
+        // certification (suspension.cpp:2324-2411, 2413-2479).
+        if (!reconstructPayloadVelocity(payload, a)) raise(v, SW_SUSPENSION_ERROR, 8, 2, 0);
+        else if (v.dampCount > 0 && !dampLines(v, N, payload, a)) {
+        } else if (!payloadCertified(payload)) raise(v, SW_SUSPENSION_ERROR, 8, 2, 0);
         else if (payload.maxResidual > a.maxLineResidual) raise(v, SW_SUSPENSION_ERROR, 8, 3, 0);
     }
     if (shared) {
@@ -2184,10 +2441,6 @@ void DeviceWing::build(const Definition& d) {
         if (d.membraneMaterial[8 * m + 7] != 1.0)
             throw std::invalid_argument(
                 "compressionStiffnessRatio other than 1 is not supported by the device membrane sweep");
-    if (d.suspension)
-        for (double damping : d.suspension->segmentDamping)
-            if (damping != 0.0)
-                throw std::invalid_argument("axial line damping is not supported by the device suspension sweep");
 
     distance_ = colourDistance(d);
     membrane_ = colourMembranes(d);
@@ -2270,6 +2523,13 @@ void DeviceWing::build(const Definition& d) {
         check(hipMemcpy(v.qy, y.data(), nodes_ * 8, hipMemcpyHostToDevice), "upload qy");
         check(hipMemcpy(v.qz, z.data(), nodes_ * 8, hipMemcpyHostToDevice), "upload qz");
         v.w = deviceCopy(allocations_, w);
+        // An isolated free membrane (no suspension, no contact, every node
+        // dynamic) has its predicted momentum restored after each substep
+        // (soft_body.cpp:1196-1205).
+        preserveMomentum_ = !d.suspension && (membranes_ > 0 || hinges_ > 0) && pairCount == 0 &&
+                            std::all_of(w.begin(), w.end(), [](double x) { return x > 0.0; });
+        v.preserveMomentum = preserveMomentum_ ? 1 : 0;
+        if (preserveMomentum_) v.momentum = deviceCopy(allocations_, std::vector<double>(16, 0.0));
         std::vector<double> zeros(nodes_, 0.0);
         v.fx = deviceCopy(allocations_, zeros);
         v.fy = deviceCopy(allocations_, zeros);
@@ -2443,6 +2703,22 @@ void DeviceWing::build(const Definition& d) {
         v.srest = deviceCopy(allocations_, rest);
         v.sstiff = deviceCopy(allocations_, stiff);
         v.slastPayload = deviceCopy(allocations_, last);
+        // Axial line damping (suspension.cpp:2413-2479) visits the damped
+        // segments in runtime order.
+        std::vector<int> dampSlot;
+        std::vector<double> dampValue;
+        for (std::size_t k = 0; k < segments_; ++k) {
+            const double damping = d.suspension->segmentDamping[layout_.segmentOrder[k]];
+            if (!(damping > 0.0)) continue;
+            dampSlot.push_back(static_cast<int>(segmentDevicePosition_[k]));
+            dampValue.push_back(damping);
+        }
+        v.dampCount = static_cast<int>(dampSlot.size());
+        dampedSegments_ = dampSlot.size();
+        if (!dampSlot.empty()) {
+            v.dampSlot = deviceCopy(allocations_, dampSlot);
+            v.dampValue = deviceCopy(allocations_, dampValue);
+        }
         // Levels -> phases; consecutive single-item levels merge into one
         // serial run, staged in LDS in chunks of at most kChainItems
         // segments and kChainSlots distinct nodes.

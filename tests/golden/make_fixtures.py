@@ -24,8 +24,10 @@ wings, and writes into tests/golden/:
   first step and after the last, the checkpoint's public fields and
   serializeStructureCheckpoint's bytes (tiny: the bytes; small: their
   SHA-256 digest and size);
+* ``<preset>.structure.damped.swfx`` -- the same with workerThreads 1 and
+  every suspension segment's axial damping set to LINE_DAMPING;
 * ``<preset>.digest.swfx``    -- for the ~70k-triangle wing (export) and the
-  tethered small wing (small-tethered): SHA-256 digests
+  tethered and free small wings (small-tethered, small-free): SHA-256 digests
   of every assembly and colouring array, and strided samples plus digests of
   the state after 1 and 240 steps (the full arrays are too large to commit;
   dihedral arrays are in the reference's hinge insertion order, which is the
@@ -59,10 +61,14 @@ PLAN = {
     # lines are load-path Cable constraints swept in pairs.
     "tiny-tethered": {"steps": 20, "snapshots": [1, 5, 20], "workers": 12},
     "small-tethered": {"steps": 20, "snapshots": [1, 20], "workers": 12},
+    # ... at two dynamic riser junctions: every node is dynamic and there is
+    # no suspension, so the reference restores the free membrane's momentum.
+    "tiny-free": {"steps": 20, "snapshots": [1, 5, 20], "workers": 12},
+    "small-free": {"steps": 20, "snapshots": [1, 20], "workers": 12},
 }
 STRIDE = 101
 # Written as digests and strided samples (the full arrays are too large).
-DIGESTED = {"export", "small-tethered"}
+DIGESTED = {"export", "small-tethered", "small-free"}
 FRAME_STEPS = {"tiny": 20, "small": 40}
 
 
@@ -147,11 +153,16 @@ def run_scene(preset: str) -> None:
 
 
 STRUCTURE_STEPS = 20
+# Axial damping (N s/m) given to every suspension segment for the damped run.
+LINE_DAMPING = 40.0
 
 
-def run_structure(preset: str, workers: int) -> None:
-    path = WORK / f"{preset}.structure.w{workers}.swfx"
+def run_structure(preset: str, workers: int, damped: bool = False) -> None:
+    tag = "damped" if damped else f"w{workers}"
+    path = WORK / f"{preset}.structure.{tag}.swfx"
     cmd = [str(DRIVER), "--structure", preset, str(path), str(STRUCTURE_STEPS), str(workers)]
+    if damped:
+        cmd.append(repr(LINE_DAMPING))
     print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     fixture = swfx.read(str(path))
@@ -159,7 +170,7 @@ def run_structure(preset: str, workers: int) -> None:
         data = fixture.pop("checkpoint_bytes")
         fixture["checkpoint_bytes_digest"] = digest(data)
         fixture["checkpoint_bytes_size"] = np.array([data.size], np.uint64)
-    swfx.write(str(HERE / f"{preset}.structure.w{workers}.swfx"), fixture)
+    swfx.write(str(HERE / f"{preset}.structure.{tag}.swfx"), fixture)
 
 
 def main(argv: list[str]) -> int:
@@ -173,6 +184,7 @@ def main(argv: list[str]) -> int:
             run_scene(preset)
             for workers in (0, 1):
                 run_structure(preset, workers)
+            run_structure(preset, 1, damped=True)
     return 0
 
 

@@ -245,20 +245,6 @@ def test_invalid_settings_are_rejected():
     with pytest.raises(sw.NativeError) as error:
         structure.add_external_force(10 ** 6, [0.0, 0.0, 1.0])
     assert error.value.status == native.SW_OUT_OF_RANGE
-    # An isolated free membrane (no suspension, every node dynamic): the
-    # reference's momentum restoration is not computed, so stepping it is
-    # refused rather than run differently.
-    d, pressure, loads = flight_case("tiny")
-    d = d.copy()
-    d.suspension = None
-    d.arrays["node_fixed"] = np.zeros_like(d.arrays["node_fixed"])
-    free = sw.Structure(d)
-    free.set_external_forces(loads)
-    with pytest.raises(sw.NativeError) as error:
-        free.step(sw.flight_step_settings())
-    assert error.value.status == native.SW_INVALID_ARGUMENT
-    assert free.accepted_step_count() == 0
-    free.close()
 
 
 def test_device_schedules_equal_host_schedules(golden):
