@@ -36,13 +36,15 @@
 //        ref_driver --compare-workers <preset> <steps>
 //        ref_driver --frame <preset> <out-file> <steps>
 //        ref_driver --scene <preset> <out-file>
-//        ref_driver --structure <preset> <out-file> <steps> <workers> [line-damping]
+//        ref_driver --structure <preset> <out-file> <steps> <workers> [line-damping [compression-ratio]]
 //
 // The --structure mode drives simwing::fsi::Structure like --frame (colour-
 // major hinges, flight loads before every step and again after the last, no
 // face pressure) with `workers` worker threads (0: the serial index-order
 // sweep), every suspension segment's axial damping set to `line-damping`
-// (N s/m, default 0), and writes what the adapter's public surface reports:
+// (N s/m, default 0) and every membrane's compressionStiffnessRatio to
+// `compression-ratio` (default 1), and writes what the adapter's public
+// surface reports:
 // definitionFingerprint(), suspensionState() before the first step and
 // after the last, the committed node states, the checkpoint's public fields
 // and serializeStructureCheckpoint's bytes of that checkpoint.
@@ -717,11 +719,12 @@ void writeSuspensionState(swfx::Writer& w, const std::string& tag, const fsi::St
 }
 
 int structureMode(const std::string& preset, const std::string& out, int steps, unsigned workers,
-                  double lineDamping) {
+                  double lineDamping, double compressionRatio) {
     Prepared p = prepare(preset);
     fsi::StructureDefinition def = p.assembly.definition;
     if (def.suspension)
         for (auto& g : def.suspension->segments) g.axialDampingNewtonSecondsPerMeter = lineDamping;
+    for (auto& m : def.membranes) m.material.compressionStiffnessRatio = compressionRatio;
     std::vector<fsi::StructureDihedralDefinition> hinges;
     for (const std::size_t index : p.hinges.order) hinges.push_back(def.dihedrals[index]);
     def.dihedrals = std::move(hinges);
@@ -796,7 +799,7 @@ int main(int argc, char** argv) {
         }
         if (argc >= 6 && std::strcmp(argv[1], "--structure") == 0)
             return structureMode(argv[2], argv[3], std::atoi(argv[4]), static_cast<unsigned>(std::atoi(argv[5])),
-                                 argc >= 7 ? std::atof(argv[6]) : 0.0);
+                                 argc >= 7 ? std::atof(argv[6]) : 0.0, argc >= 8 ? std::atof(argv[7]) : 1.0);
         if (argc >= 5 && std::strcmp(argv[1], "--frame") == 0)
             return frameMode(argv[2], argv[3], std::atoi(argv[4]));
         if (argc >= 4 && std::strcmp(argv[1], "--scene") == 0) {

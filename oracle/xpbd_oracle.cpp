@@ -405,6 +405,8 @@ struct Membrane {
     Real preW, preT, dampingTime;
     V lambda, resultant;
     Real residual = 0.0;
+    Sym stiffness;      // material stiffness matrix (membrane.cpp:259-266)
+    Real ratio = 1.0;   // compressionStiffnessRatio
 };
 struct Hinge {
     std::size_t a, b, c, d;
@@ -508,6 +510,21 @@ MemEval evaluate(const oracle_body& b, const Membrane& m) {
     return r;
 }
 
+// effectiveMembraneStiffness (membrane.cpp:273-296) inverted by
+// checkedInverse (membrane.cpp:208-235).
+Sym stateCompliance(const Membrane& m, const V& greenStrain) {
+    const Sym& k = m.stiffness;
+    const Real retained = sqrt(m.ratio);
+    const Real warp = greenStrain.x < 0.0 ? retained : Real(1.0);
+    const Real weft = greenStrain.y < 0.0 ? retained : Real(1.0);
+    const Real shear = sqrt(warp * weft);
+    const Sym effective{k.xx * warp * warp, k.yy * weft * weft, k.zz * shear * shear,
+                        k.xy * warp * weft, k.xz * warp * shear, k.yz * weft * shear};
+    Sym inverse;
+    if (!symInverse(effective, inverse)) throw std::invalid_argument("SymmetricMatrix3 is singular or ill-conditioned");
+    return inverse;
+}
+
 // soft_body.cpp:1822-1929
 void projectMembrane(oracle_body& b, Membrane& m, Real dt) {
     const MemEval e = evaluate(b, m);
@@ -534,7 +551,9 @@ void projectMembrane(oracle_body& b, Membrane& m, Real dt) {
         dm.z += dot(J[2][c], disp);
     }
     const Real invDt2 = 1.0 / (dt * dt);
-    const Sym at = invDt2 * m.compliance;
+    // A compression ratio below 1 softens the compressed material directions
+    // for this projection (soft_body.cpp:1869-1873).
+    const Sym at = invDt2 * (m.ratio == 1.0 ? m.compliance : stateCompliance(m, e.strain));
     const Real gamma = m.dampingTime / dt;
     const Sym S = (1.0 + gamma) * M + at;
     const V rhs = -(s * e.strain) - at * m.lambda - gamma * dm;
@@ -1084,7 +1103,6 @@ int oracle_create(const sw_definition* d, oracle_body** out) {
                 throw std::invalid_argument("Matrix2 is singular or ill-conditioned");
             const Real inv = 1.0 / det;
             const double* mat = &d->membrane_material[8 * m];
-            if (mat[7] != 1.0) throw std::invalid_argument("oracle restates compressionStiffnessRatio == 1 only");
             Sym C;
             if (!symInverse(Sym{mat[0], mat[1], mat[3], mat[2], 0.0, 0.0}, C))
                 throw std::invalid_argument("SymmetricMatrix3 is singular or ill-conditioned");
@@ -1103,6 +1121,8 @@ int oracle_create(const sw_definition* d, oracle_body** out) {
                          {},
                          {},
                          0.0};
+            mem.stiffness = Sym{mat[0], mat[1], mat[3], mat[2], 0.0, 0.0};
+            mem.ratio = mat[7];
             b->membranes.push_back(mem);
             corners.insert(corners.end(), {mem.a, mem.b, mem.c});
         }

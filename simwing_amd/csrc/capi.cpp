@@ -114,8 +114,8 @@ struct sw_structure {
 namespace {
 
 // What the adjoints do not differentiate: load-path cable pairs interleaved
-// with the iterations, a free membrane's momentum restoration and axial
-// line damping.
+// with the iterations, a free membrane's momentum restoration, axial line
+// damping and compression-softened membranes.
 void requireDifferentiable(const sw_structure& s) {
     if (s.wing->interleavedCablePairs() > 0)
         throw std::invalid_argument(
@@ -124,6 +124,8 @@ void requireDifferentiable(const sw_structure& s) {
         throw std::invalid_argument("the adjoint does not differentiate free-membrane momentum restoration");
     if (s.wing->dampedSegments() > 0)
         throw std::invalid_argument("the adjoint does not differentiate axial line damping");
+    if (s.wing->compressedMembranes())
+        throw std::invalid_argument("the adjoint does not differentiate compressionStiffnessRatio other than 1");
 }
 
 // SoftBody::stepCoupled / step setting checks (soft_body.cpp:981-1050).

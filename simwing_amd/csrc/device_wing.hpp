@@ -199,10 +199,19 @@ public:
     void setCablePairs(int pairs, int iterations);
     int interleavedCablePairs() const { return cableInterleaved_; }
     bool serialSweep() const { return serialSweep_; }
-    // A free membrane (momentum restored after each substep) and damped
-    // suspension lines: stepped, not differentiated.
+    // A free membrane (momentum restored after each substep), damped
+    // suspension lines and membranes with a compressionStiffnessRatio other
+    // than 1: stepped, not differentiated.
     bool preservesMomentum() const { return preserveMomentum_; }
     std::size_t dampedSegments() const { return dampedSegments_; }
+    bool compressedMembranes() const { return compressedMembranes_; }
+    // Whether a substep needs the kernel variant compiled with the list
+    // phases (serial sweep, interleaved cable pairs), compression softening,
+    // momentum restoration and line damping.
+    bool needsGeneralKernel() const {
+        return serialSweep_ || cableInterleaved_ > 0 || preserveMomentum_ || dampedSegments_ > 0 ||
+               compressedMembranes_;
+    }
     bool xcdLocal() const { return xcdLocal_; }
     int workgroups() const { return workgroups_; }
     int threads() const { return threads_; }
@@ -236,6 +245,7 @@ private:
     bool serialSweep_ = false;
     bool preserveMomentum_ = false;
     std::size_t dampedSegments_ = 0;
+    bool compressedMembranes_ = false;
     std::vector<Phase> colouredBase_, serialBase_;   // the programs without cable phases
     std::vector<int> stitchList_;                    // seam-stitch closing levels (device slots)
     std::vector<Phase> stitchLevels_;

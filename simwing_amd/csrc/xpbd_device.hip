@@ -49,6 +49,8 @@ struct WingView {
     double *preW, *preT, *mdamp;
     double *mlx, *mly, *mlz, *mrx, *mry, *mrz, *mres;
     int anyPreTension, anyDamping;
+    int anyCompression;                  // some compressionStiffnessRatio != 1:
+    double *mratio, *kxx, *kyy, *kzz, *kxy;  //   the ratio and the material stiffness
     int hingeCount;
     int *ha, *hb, *hc, *hd;
     double *hrest, *hcomp, *hlam;
@@ -178,8 +180,11 @@ __device__ __forceinline__ bool syncTimedOut(const WingView& v, unsigned spins) 
 // One-XCD release half: every wave waits until its own earlier global
 // stores are complete in the XCD's L2. Called by all threads before the
 // workgroup barrier that precedes a publication.
+// kLocal: the launch is a one-XCD launch (compiled per kernel variant).
+template <bool kLocal = false>
 __device__ __forceinline__ void drainStores(const WingView& v) {
-    if (v.xcdLocal) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    (void)v;
+    if constexpr (kLocal) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 // Acquire after a wait: the agent-scope acquire (buffer_inv sc1: this
@@ -193,21 +198,22 @@ __device__ __forceinline__ void acquireAfterWait(const WingView& v) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+template <bool kLocal = false>
 __device__ __forceinline__ void gridSync(const WingView& v, int workers) {
     if (workers == 1) {
         __syncthreads();
         return;
     }
-    drainStores(v);
+    drainStores<kLocal>(v);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const bool local = v.xcdLocal != 0;
+        constexpr bool local = kLocal;
         unsigned* count = v.barrier;
         unsigned* generation = v.barrier + 32;
         const unsigned gen = __hip_atomic_load(generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // The generation must be read before this workgroup arrives: once it
         // has arrived, the last arrival may already advance it.
-        if (!local) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if constexpr (!local) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == static_cast<unsigned>(workers) - 1u) {
@@ -216,7 +222,7 @@ __device__ __forceinline__ void gridSync(const WingView& v, int workers) {
             // one XCD: the reset is an earlier atomic at the same L2).
             acquireAfterWait(v);
             __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (local) {
+            if constexpr (local) {
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 __hip_atomic_store(generation, gen + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
@@ -443,6 +449,22 @@ __device__ __forceinline__ MembraneItem membraneItem(const WingView& v, const No
     return it;
 }
 
+// The compliance of a membrane whose compressionStiffnessRatio is not 1 for
+// its current Green strain: effectiveMembraneStiffness (membrane.cpp:273-296;
+// the material stiffness has no xz and yz entries) inverted by
+// checkedInverse (membrane.cpp:208-235). False where the reference throws.
+__device__ __forceinline__ bool compressedCompliance(const WingView& v, int k, double ratio, d3 strain, s3& out) {
+    const double retained = sqrt(ratio);
+    const double warp = strain.x < 0.0 ? retained : 1.0;
+    const double weft = strain.y < 0.0 ? retained : 1.0;
+    const double shear = sqrt(warp * weft);
+    const s3 effective{v.kxx[k] * warp * warp, v.kyy[k] * weft * weft, v.kzz[k] * shear * shear,
+                       v.kxy[k] * warp * weft, 0.0, 0.0};
+    return checkedInverse(effective, out);
+}
+
+// kGeneral: compiled with the compressionStiffnessRatio branch.
+template <bool kGeneral = true>
 __device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRefs& N, int k, const MembraneItem& it,
                                                 const StepArgs& a, double* rec) {
     const int i0 = it.i0, i1 = it.i1, i2 = it.i2;
@@ -474,8 +496,21 @@ __device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRef
         M.xz += wc * dot(J[0][corner], J[2][corner]);
         M.yz += wc * dot(J[1][corner], J[2][corner]);
     }
-    const s3 at{a.invDt2 * it.cxx, a.invDt2 * it.cyy, a.invDt2 * it.czz,
-                a.invDt2 * it.cxy, a.invDt2 * it.cxz, a.invDt2 * it.cyz};
+    s3 at{a.invDt2 * it.cxx, a.invDt2 * it.cyy, a.invDt2 * it.czz,
+          a.invDt2 * it.cxy, a.invDt2 * it.cxz, a.invDt2 * it.cyz};
+    if (kGeneral && v.anyCompression) {
+        // soft_body.cpp:1869-1873
+        const double ratio = v.mratio[k];
+        if (ratio != 1.0) {
+            s3 state;
+            if (!compressedCompliance(v, k, ratio, e.strain, state)) {
+                raise(v, SW_INVALID_ARGUMENT, -1, 0, k);
+                return;
+            }
+            at = s3{a.invDt2 * state.xx, a.invDt2 * state.yy, a.invDt2 * state.zz,
+                    a.invDt2 * state.xy, a.invDt2 * state.xz, a.invDt2 * state.yz};
+        }
+    }
     const d3 lambda = it.lambda;
     const d3 cv = scale(e.strain, areaScale);
     const d3 atl = mul(at, lambda);
@@ -520,9 +555,10 @@ __device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRef
     N.put(i2, add(p[2], corr[2]));
 }
 
+template <bool kGeneral = true>
 __device__ __forceinline__ void projectMembrane(const WingView& v, const NodeRefs& N, int k, const StepArgs& a,
                                                 double* rec = nullptr) {
-    projectMembrane(v, N, k, membraneItem(v, N, k), a, rec);
+    projectMembrane<kGeneral>(v, N, k, membraneItem(v, N, k), a, rec);
 }
 
 struct HingeItem {
@@ -1165,9 +1201,11 @@ __device__ __forceinline__ void prefetchedPhase(const WingView& v, const StagedP
 // earlier writes (call after a workgroup barrier).
 // One XCD: every wave drained its stores before that barrier
 // (drainStores), so only the progress word itself is written.
+template <bool kLocal = false>
 __device__ __forceinline__ void publishCounter(const WingView& v, unsigned* counter, unsigned value) {
+    (void)v;
     if (threadIdx.x == 0) {
-        if (!v.xcdLocal) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if constexpr (!kLocal) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_store(counter, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -1177,7 +1215,7 @@ __device__ __forceinline__ void publishCounter(const WingView& v, unsigned* coun
 // is loaded (fixed values and own multiplier, as prefetchedPhase does) while
 // this phase's writes drain and its progress is published. A participant
 // with no item in a phase neither waits nor publishes for it.
-template <class Item, class Load, class Project>
+template <bool kLocal, class Item, class Load, class Project>
 __device__ __forceinline__ void pipelinedRun(const WingView& v, const StagedProgram& S, int it, int ph0, int ph1,
                                              int p, Load load, Project project) {
     const int m = v.iterPhaseCount;
@@ -1204,9 +1242,9 @@ __device__ __forceinline__ void pipelinedRun(const WingView& v, const StagedProg
         Item next{};
         if (nfirst < ne) next = load(nfirst);
         if (b < e) {
-            drainStores(v);
+            drainStores<kLocal>(v);
             __syncthreads();
-            publishCounter(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
+            publishCounter<kLocal>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
         }
         b = nb, e = ne, first = nfirst, cur = next;
     }
@@ -1570,6 +1608,10 @@ __device__ __forceinline__ int cablePairsOf(const StepArgs& a, int it) {
     return a.cableBase + (it < a.cableRem ? 1 : 0);
 }
 
+// kGeneral: compiled with the list phases (serial sweep, interleaved cable
+// pairs) and the compressionStiffnessRatio branch; the host launches the
+// general kernel variant whenever a wing's programs or definition use them.
+template <bool kGeneral>
 __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, int first, int end, int stride,
                          PayloadDev& payload, const StepArgs& a, double* rec, const TraceLayout& L) {
     switch (P.type) {
@@ -1579,7 +1621,7 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
         break;
     case kPhaseMembrane:
         for (int k = first; k < end; k += stride)
-            projectMembrane(v, N, k, a, rec ? rec + L.membBase + kRecMemb * static_cast<long>(k) : nullptr);
+            projectMembrane<kGeneral>(v, N, k, a, rec ? rec + L.membBase + kRecMemb * static_cast<long>(k) : nullptr);
         break;
     case kPhaseHinge:
         for (int k = first; k < end; k += stride)
@@ -1596,19 +1638,24 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
             projectSphere(v, N, k, a.invDt2, rec ? rec + L.sphereBase + kRecSphere * static_cast<long>(k) : nullptr);
         break;
     case kPhaseDistanceList:
-        for (int j = first; j < end; j += stride) {
-            const int k = v.distList[j];
-            projectDistance(v, N, k, a.invDt2, rec ? rec + L.distBase + kRecDist * static_cast<long>(k) : nullptr);
+        if constexpr (kGeneral) {
+            for (int j = first; j < end; j += stride) {
+                const int k = v.distList[j];
+                projectDistance(v, N, k, a.invDt2, rec ? rec + L.distBase + kRecDist * static_cast<long>(k) : nullptr);
+            }
         }
         break;
     case kPhaseMembraneList:
-        for (int j = first; j < end; j += stride) {
-            const int k = v.membList[j];
-            projectMembrane(v, N, k, a, rec ? rec + L.membBase + kRecMemb * static_cast<long>(k) : nullptr);
+        if constexpr (kGeneral) {
+            for (int j = first; j < end; j += stride) {
+                const int k = v.membList[j];
+                projectMembrane<true>(v, N, k, a, rec ? rec + L.membBase + kRecMemb * static_cast<long>(k) : nullptr);
+            }
         }
         break;
     case kPhaseCableList:  // the caller skips pairs beyond the iteration's share; never recorded
-        for (int j = first; j < end; j += stride) projectDistance(v, N, v.cableList[j], a.invDt2, nullptr);
+        if constexpr (kGeneral)
+            for (int j = first; j < end; j += stride) projectDistance(v, N, v.cableList[j], a.invDt2, nullptr);
         break;
     default:
         break;
@@ -1618,7 +1665,9 @@ __device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, i
 // One substep of one wing, run by `blocks` workgroups of which this is
 // `block`. kMode: 0 nodes in HBM, 1 nodes resident in LDS, 2 either, as the
 // wing's view says (batched launches mix both kinds of wing).
-template <int kMode>
+// kLocal: a one-XCD launch; kGeneral: compiled with the list phases, the
+// compressionStiffnessRatio branch, momentum restoration and line damping.
+template <int kMode, bool kLocal, bool kGeneral>
 // `trace`: the adjoint's recording of this substep, or null.
 __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a, int block, int blocks, double* trace) {
     extern __shared__ double lds[];
@@ -1696,7 +1745,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
             !refreshInverseInertia(payload, quat{payload.q[0], payload.q[1], payload.q[2], payload.q[3]}))
             raise(v, SW_SUSPENSION_ERROR, 5, 2, 0);
     }
-    gridSync(v, blocks);
+    gridSync<kLocal>(v, blocks);
 
     // B: pressure gather from the start-of-substep positions, then predict
     // (soft_body.cpp:1492-1566).
@@ -1727,8 +1776,8 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         v.vz[i] = vel.z;
         N.put(i, add(N.get(i), scale(vel, a.dt)));
     }
-    gridSync(v, blocks);
-    if (v.preserveMomentum) {
+    gridSync<kLocal>(v, blocks);
+    if (kGeneral && v.preserveMomentum) {
         // The free membrane's momentum after prediction (soft_body.cpp:1196-1205),
         // restored after finalisation.
         if (block == 0 && threadIdx.x < 64) {
@@ -1746,7 +1795,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                 v.momentum[9] = m.angular.z;
             }
         }
-        gridSync(v, blocks);
+        gridSync<kLocal>(v, blocks);
     }
 
     // Constraint iterations.
@@ -1792,9 +1841,9 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     const int step = it * m + ph;
                     waitStagedDeps(v, S, ph, step);
                     runChain(v, N, P, payload, a, rec ? rec + layout.segBase : nullptr);
-                    drainStores(v);
+                    drainStores<kLocal>(v);
                     __syncthreads();
-                    publishCounter(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
+                    publishCounter<kLocal>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
                 }
             }
         }
@@ -1806,7 +1855,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     int end = ph + 1;
                     while (end < m && S.phase[end].type == P.type) ++end;
                     if (P.type == kPhaseDistance) {
-                        pipelinedRun<DistanceItem>(
+                        pipelinedRun<kLocal, DistanceItem>(
                             v, S, it, ph, end, p, [&](int k) { return distanceItem(v, N, k); },
                             [&](int k, const DistanceItem& item) {
                                 projectDistance(v, N, k, item, a.invDt2,
@@ -1814,15 +1863,15 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                                                      : nullptr);
                             });
                     } else if (P.type == kPhaseMembrane) {
-                        pipelinedRun<MembraneItem>(
+                        pipelinedRun<kLocal, MembraneItem>(
                             v, S, it, ph, end, p, [&](int k) { return membraneItem(v, N, k); },
                             [&](int k, const MembraneItem& item) {
-                                projectMembrane(v, N, k, item, a,
+                                projectMembrane<kGeneral>(v, N, k, item, a,
                                                 irec ? irec + layout.membBase + kRecMemb * static_cast<long>(k)
                                                      : nullptr);
                             });
                     } else {
-                        pipelinedRun<HingeItem>(
+                        pipelinedRun<kLocal, HingeItem>(
                             v, S, it, ph, end, p, [&](int k) { return hingeItem(v, N, k); },
                             [&](int k, const HingeItem& item) {
                                 projectHinge(v, N, k, item, a.invDt2,
@@ -1851,7 +1900,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     prefetchedPhase<MembraneItem>(
                         v, S, ph, step, first, e, stride, [&](int k) { return membraneItem(v, N, k); },
                         [&](int k, const MembraneItem& item) {
-                            projectMembrane(v, N, k, item, a,
+                            projectMembrane<kGeneral>(v, N, k, item, a,
                                             rec ? rec + layout.membBase + kRecMemb * static_cast<long>(k) : nullptr);
                         });
                     break;
@@ -1880,22 +1929,23 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                 case kPhaseDistanceList:
                 case kPhaseMembraneList:
                     waitStagedDeps(v, S, ph, step);
-                    runPhase(v, N, P, first, e, stride, payload, a, rec, layout);
+                    if constexpr (kGeneral) runPhase<true>(v, N, P, first, e, stride, payload, a, rec, layout);
                     break;
                 case kPhaseCableList:
                     waitStagedDeps(v, S, ph, step);
-                    if (P.flags < cablePairsOf(a, it)) runPhase(v, N, P, first, e, stride, payload, a, rec, layout);
+                    if constexpr (kGeneral)
+                        if (P.flags < cablePairsOf(a, it)) runPhase<true>(v, N, P, first, e, stride, payload, a, rec, layout);
                     break;
                 default:
                     waitStagedDeps(v, S, ph, step);
                     break;
                 }
-                drainStores(v);
+                drainStores<kLocal>(v);
                 __syncthreads();
-                publishCounter(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
+                publishCounter<kLocal>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
             }
         }
-        gridSync(v, blocks);
+        gridSync<kLocal>(v, blocks);
         // Every participant has left the loop: nobody polls the progress
         // words any more, so each resets its own for the next launch.
         if (threadIdx.x == 0) __hip_atomic_store(v.flags + 32 * p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1906,11 +1956,11 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
             for (int ph = 0; ph < v.iterPhaseCount; ++ph) {
                 const Phase P = v.iterPhases[ph];
                 if (P.type == kPhaseSegmentRun) {
-                    if (block == 0) runPhase(v, N, P, P.begin, P.end, 1, payload, a, rec, layout);
-                } else if (P.type != kPhaseCableList || P.flags < cablePairsOf(a, it)) {
-                    runPhase(v, N, P, P.begin + gtid, P.end, gstride, payload, a, rec, layout);
+                    if (block == 0) runPhase<kGeneral>(v, N, P, P.begin, P.end, 1, payload, a, rec, layout);
+                } else if (!kGeneral || P.type != kPhaseCableList || P.flags < cablePairsOf(a, it)) {
+                    runPhase<kGeneral>(v, N, P, P.begin + gtid, P.end, gstride, payload, a, rec, layout);
                 }
-                gridSync(v, workers);
+                gridSync<kLocal>(v, workers);
             }
         }
     }
@@ -1925,7 +1975,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         for (int k = P.begin + gtid; k < P.end; k += gstride)
             projectDistance(v, N, v.stitchItems[k], a.invDt2,
                             trace ? trace + layout.closeBase + kRecDist * static_cast<long>(k) : nullptr);
-        gridSync(v, workers);
+        gridSync<kLocal>(v, workers);
     }
 
     // Membrane solver diagnostics (soft_body.cpp:2366-2404), velocity
@@ -1997,8 +2047,8 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         }
         if (threadIdx.x == 0) atomicMaxNonNegative(&payload.maxResidual, scratch[0]);
     }
-    gridSync(v, workers);
-    if (v.preserveMomentum && block == 0 && threadIdx.x < 64) {
+    gridSync<kLocal>(v, workers);
+    if (kGeneral && v.preserveMomentum && block == 0 && threadIdx.x < 64) {
         // soft_body.cpp:1446-1448
         const MomentumDev target{v.momentum[0], d3{v.momentum[1], v.momentum[2], v.momentum[3]},
                                  d3{v.momentum[4], v.momentum[5], v.momentum[6]},
@@ -2009,7 +2059,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         // Payload velocity, line damping, then the payload and line-residual
         // certification (suspension.cpp:2324-2411, 2413-2479).
         if (!reconstructPayloadVelocity(payload, a)) raise(v, SW_SUSPENSION_ERROR, 8, 2, 0);
-        else if (v.dampCount > 0 && !dampLines(v, N, payload, a)) {
+        else if (kGeneral && v.dampCount > 0 && !dampLines(v, N, payload, a)) {
         } else if (!payloadCertified(payload)) raise(v, SW_SUSPENSION_ERROR, 8, 2, 0);
         else if (payload.maxResidual > a.maxLineResidual) raise(v, SW_SUSPENSION_ERROR, 8, 3, 0);
     }
@@ -2022,10 +2072,10 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
     }
 }
 
-template <bool kShared>
+template <bool kShared, bool kLocal, bool kGeneral>
 __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArgs a) {
     int block = static_cast<int>(blockIdx.x), blocks = static_cast<int>(gridDim.x);
-    if (v.xcdLocal) {
+    if constexpr (kLocal) {
         // One-XCD launch: the grid holds several workgroups per XCD; those
         // holding the first launchBlocks tickets of the XCD that fills them
         // first are the participants, the rest leave.
@@ -2036,18 +2086,37 @@ __global__ void __launch_bounds__(kMaxThreads) substepKernel(WingView v, StepArg
         block = role;
         blocks = v.launchBlocks;
     }
-    substepBody<kShared ? 1 : 0>(v, a, block, blocks, v.trace);
+    substepBody<kShared ? 1 : 0, kLocal, kGeneral>(v, a, block, blocks, v.trace);
 }
 
 // Batched substep: workgroup b runs workgroup blockLocal[b] of wing
 // blockWing[b]; each wing's view holds its own workgroup count and modes.
 // `traces` (recording only): each wing's trace for this substep.
+template <bool kGeneral>
 __global__ void __launch_bounds__(kMaxThreads) batchSubstepKernel(const WingView* views, const StepArgs* args,
                                                                    const int* blockWing, const int* blockLocal,
                                                                    double* const* traces) {
     const int wing = blockWing[blockIdx.x];
     const WingView& v = views[wing];
-    substepBody<2>(v, args[wing], blockLocal[blockIdx.x], v.launchBlocks, traces ? traces[wing] : nullptr);
+    substepBody<2, false, kGeneral>(v, args[wing], blockLocal[blockIdx.x], v.launchBlocks, traces ? traces[wing] : nullptr);
+}
+
+// The substep kernel variants: nodes in LDS or HBM, a one-XCD launch, and
+// the general features (list phases, compression softening, momentum
+// restoration, line damping) compiled in or out.
+const void* substepKernelFor(bool shared, bool local, bool general) {
+    if (shared)
+        return general ? reinterpret_cast<const void*>(&substepKernel<true, false, true>)
+                       : reinterpret_cast<const void*>(&substepKernel<true, false, false>);
+    if (local)
+        return general ? reinterpret_cast<const void*>(&substepKernel<false, true, true>)
+                       : reinterpret_cast<const void*>(&substepKernel<false, true, false>);
+    return general ? reinterpret_cast<const void*>(&substepKernel<false, false, true>)
+                   : reinterpret_cast<const void*>(&substepKernel<false, false, false>);
+}
+const void* batchKernelFor(bool general) {
+    return general ? reinterpret_cast<const void*>(&batchSubstepKernel<true>)
+                   : reinterpret_cast<const void*>(&batchSubstepKernel<false>);
 }
 
 // Copies the mutable state block into the snapshot while no error is
@@ -2437,10 +2506,6 @@ void DeviceWing::build(const Definition& d) {
     hinges_ = d.dihedralCount();
     if (nodes_ > 0x3fffffff || 3 * triangles_ > 0x3fffffff || constraints_ > 0x3fffffff)
         throw std::invalid_argument("structure exceeds the device index range");
-    for (std::size_t m = 0; m < membranes_; ++m)
-        if (d.membraneMaterial[8 * m + 7] != 1.0)
-            throw std::invalid_argument(
-                "compressionStiffnessRatio other than 1 is not supported by the device membrane sweep");
 
     distance_ = colourDistance(d);
     membrane_ = colourMembranes(d);
@@ -2584,7 +2649,8 @@ void DeviceWing::build(const Definition& d) {
         std::vector<double> m00(o.size()), m01(o.size()), m10(o.size()), m11(o.size()), sc(o.size());
         std::vector<double> cxx(o.size()), cyy(o.size()), czz(o.size()), cxy(o.size()), cxz(o.size()), cyz(o.size());
         std::vector<double> pw(o.size()), pt(o.size()), damp(o.size());
-        int anyPre = 0, anyDamp = 0;
+        std::vector<double> ratio(o.size()), kxx(o.size()), kyy(o.size()), kzz(o.size()), kxy(o.size());
+        int anyPre = 0, anyDamp = 0, anyCompression = 0;
         for (std::size_t k = 0; k < o.size(); ++k) {
             const std::size_t m = o[k];
             const std::uint64_t t = d.membraneTriangle[m];
@@ -2617,6 +2683,13 @@ void DeviceWing::build(const Definition& d) {
             damp[k] = mat[6];
             anyPre |= (mat[4] != 0.0 || mat[5] != 0.0) ? 1 : 0;
             anyDamp |= mat[6] != 0.0 ? 1 : 0;
+            // stiffnessMatrix() (membrane.cpp:259-266): warp, weft, shear, coupling.
+            ratio[k] = mat[7];
+            kxx[k] = mat[0];
+            kyy[k] = mat[1];
+            kzz[k] = mat[3];
+            kxy[k] = mat[2];
+            anyCompression |= mat[7] != 1.0 ? 1 : 0;
         }
         v.ma = deviceCopy(allocations_, a);
         v.mb = deviceCopy(allocations_, b);
@@ -2637,6 +2710,15 @@ void DeviceWing::build(const Definition& d) {
         v.mdamp = deviceCopy(allocations_, damp);
         v.anyPreTension = anyPre;
         v.anyDamping = anyDamp;
+        v.anyCompression = anyCompression;
+        compressedMembranes_ = anyCompression != 0;
+        if (anyCompression) {
+            v.mratio = deviceCopy(allocations_, ratio);
+            v.kxx = deviceCopy(allocations_, kxx);
+            v.kyy = deviceCopy(allocations_, kyy);
+            v.kzz = deviceCopy(allocations_, kzz);
+            v.kxy = deviceCopy(allocations_, kxy);
+        }
     }
     // Hinges, colour-major.
     {
@@ -2984,9 +3066,10 @@ void DeviceWing::build(const Definition& d) {
         workgroups_ = std::max(1, std::min(cus, itemCount() < 50000 ? 16 : 64));
     }
     if (dynamicLds <= sharedNodeBudget())  // also for a later single-workgroup setLaunch
-        check(hipFuncSetAttribute(reinterpret_cast<const void*>(&substepKernel<true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamicLds)),
-              "shared memory attribute");
+        for (bool general : {false, true})
+            check(hipFuncSetAttribute(substepKernelFor(true, false, general), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(dynamicLds)),
+                  "shared memory attribute");
     {
         int cus = 0;
         check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
@@ -3020,11 +3103,14 @@ void DeviceWing::build(const Definition& d) {
 std::size_t DeviceWing::sharedNodeBudget() const {
     int maxShared = 0;
     check(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device_), "attribute");
-    hipFuncAttributes solo{}, batch{};
-    check(hipFuncGetAttributes(&solo, reinterpret_cast<const void*>(&substepKernel<true>)), "kernel attributes");
-    check(hipFuncGetAttributes(&batch, reinterpret_cast<const void*>(&batchSubstepKernel)), "kernel attributes");
+    std::size_t fixed = 0;
+    for (bool general : {false, true}) {
+        hipFuncAttributes solo{}, batch{};
+        check(hipFuncGetAttributes(&solo, substepKernelFor(true, false, general)), "kernel attributes");
+        check(hipFuncGetAttributes(&batch, batchKernelFor(general)), "kernel attributes");
+        fixed = std::max({fixed, solo.sharedSizeBytes, batch.sharedSizeBytes});
+    }
     const std::size_t total = static_cast<std::size_t>(maxShared);
-    const std::size_t fixed = std::max(solo.sharedSizeBytes, batch.sharedSizeBytes);
     return total > fixed ? total - fixed : 0;
 }
 
@@ -3040,9 +3126,10 @@ void DeviceWing::setLaunch(int workgroups, int threads) {
     const std::size_t dynamicLds = (4 * nodes_ + kMaxThreads) * sizeof(double);
     sharedNodes_ = workgroups == 1 && dynamicLds <= sharedNodeBudget();
     if (sharedNodes_) {
-        check(hipFuncSetAttribute(reinterpret_cast<const void*>(&substepKernel<true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamicLds)),
-              "shared memory attribute");
+        for (bool general : {false, true})
+            check(hipFuncSetAttribute(substepKernelFor(true, false, general), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(dynamicLds)),
+                  "shared memory attribute");
     }
     buildSync();
     setXcdLocal(xcdRequested_);
@@ -3136,9 +3223,14 @@ bool DeviceWing::setXcdLocal(bool enabled) {
     int cus = 0, perCu = 0;
     check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "attribute");
     const std::size_t scratch = kMaxThreads * sizeof(double) + (neighbourSync_ ? stageBytes_ : 0);
-    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(&substepKernel<false>),
-                                                       threads_, scratch),
-          "occupancy");
+    perCu = 1 << 30;
+    for (bool general : {false, true}) {
+        int blocks = 0;
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, substepKernelFor(false, true, general), threads_,
+                                                           scratch),
+              "occupancy");
+        perCu = std::min(perCu, blocks);
+    }
     // Every workgroup of the grid resident at once, and one XCD's share of
     // the compute units able to hold all participants.
     const long grid = static_cast<long>(kXcdLaunchFactor) * workgroups_;
@@ -3183,11 +3275,16 @@ void DeviceWing::launchSubstep(const StepArgs& stepArgs, double* trace) {
         check(hipEventRecord(events_[eventsUsed_], stream_), "hipEventRecord");
     }
     const std::size_t scratch = kMaxThreads * sizeof(double) + (neighbourSync_ ? stageBytes_ : 0);
+    const bool general = needsGeneralKernel();
+    void* params[] = {&v, &args};
     if (sharedNodes_) {
         const std::size_t bytes = 4 * nodes_ * sizeof(double) + scratch;
-        hipLaunchKernelGGL(substepKernel<true>, dim3(1), dim3(threads_), bytes, stream_, v, args);
+        check(hipLaunchKernel(substepKernelFor(true, false, general), dim3(1), dim3(threads_), params, bytes, stream_),
+              "substep launch");
     } else if (workgroups_ == 1) {
-        hipLaunchKernelGGL(substepKernel<false>, dim3(1), dim3(threads_), scratch, stream_, v, args);
+        check(hipLaunchKernel(substepKernelFor(false, false, general), dim3(1), dim3(threads_), params, scratch,
+                              stream_),
+              "substep launch");
     } else {
         int grid = workgroups_;
         if (xcdLocal_) {
@@ -3199,12 +3296,11 @@ void DeviceWing::launchSubstep(const StepArgs& stepArgs, double* trace) {
             v.epoch = epoch_;
             grid = kXcdLaunchFactor * workgroups_;
         }
+        const void* kernel = substepKernelFor(false, xcdLocal_, general);
         if (!cooperative_) {
-            hipLaunchKernelGGL(substepKernel<false>, dim3(grid), dim3(threads_), scratch, stream_, v, args);
+            check(hipLaunchKernel(kernel, dim3(grid), dim3(threads_), params, scratch, stream_), "substep launch");
         } else {
-            void* params[] = {&v, &args};
-            check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&substepKernel<false>), dim3(grid),
-                                             dim3(threads_), params, scratch, stream_),
+            check(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(threads_), params, scratch, stream_),
                   "cooperative launch");
         }
     }
@@ -3718,9 +3814,10 @@ void DeviceBatch::plan(int cap, int threads) {
         }
     }
     if (lds_ > kMaxThreads * sizeof(double))
-        check(hipFuncSetAttribute(reinterpret_cast<const void*>(&batchSubstepKernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_)),
-              "shared memory attribute");
+        for (bool general : {false, true})
+            check(hipFuncSetAttribute(batchKernelFor(general), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(lds_)),
+                  "shared memory attribute");
     total_ = static_cast<int>(blockWing.size());
     threads_ = threads;
     cap_ = cap;
@@ -3769,13 +3866,14 @@ void DeviceBatch::launchSubstep(const std::vector<StepArgs>& args, double* const
     const StepArgs* stepArgs = static_cast<const StepArgs*>(args_);
     const int* blockWing = static_cast<const int*>(blockWing_);
     const int* blockLocal = static_cast<const int*>(blockLocal_);
+    bool general = false;
+    for (const DeviceWing* w : wings_) general = general || w->needsGeneralKernel();
+    void* params[] = {&views, &stepArgs, &blockWing, &blockLocal, &traces};
     if (!cooperative_) {
-        hipLaunchKernelGGL(batchSubstepKernel, dim3(total_), dim3(threads_), lds_, stream_, views, stepArgs, blockWing,
-                           blockLocal, traces);
+        check(hipLaunchKernel(batchKernelFor(general), dim3(total_), dim3(threads_), params, lds_, stream_),
+              "batch substep launch");
     } else {
-        void* params[] = {&views, &stepArgs, &blockWing, &blockLocal, &traces};
-        check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&batchSubstepKernel), dim3(total_),
-                                         dim3(threads_), params, lds_, stream_),
+        check(hipLaunchCooperativeKernel(batchKernelFor(general), dim3(total_), dim3(threads_), params, lds_, stream_),
               "cooperative batch launch");
     }
     check(hipGetLastError(), "batch substep launch");

@@ -26,6 +26,8 @@ wings, and writes into tests/golden/:
   SHA-256 digest and size);
 * ``<preset>.structure.damped.swfx`` -- the same with workerThreads 1 and
   every suspension segment's axial damping set to LINE_DAMPING;
+* ``<preset>.structure.compressed.swfx`` -- the same with workerThreads 1 and
+  every membrane's compressionStiffnessRatio set to COMPRESSION_RATIO;
 * ``<preset>.digest.swfx``    -- for the ~70k-triangle wing (export) and the
   tethered and free small wings (small-tethered, small-free): SHA-256 digests
   of every assembly and colouring array, and strided samples plus digests of
@@ -153,16 +155,21 @@ def run_scene(preset: str) -> None:
 
 
 STRUCTURE_STEPS = 20
-# Axial damping (N s/m) given to every suspension segment for the damped run.
+# Axial damping (N s/m) given to every suspension segment for the damped run,
+# and the compressionStiffnessRatio given to every membrane for the
+# compressed run.
 LINE_DAMPING = 40.0
+COMPRESSION_RATIO = 0.25
 
 
-def run_structure(preset: str, workers: int, damped: bool = False) -> None:
-    tag = "damped" if damped else f"w{workers}"
+def run_structure(preset: str, workers: int, variant: str = "") -> None:
+    tag = variant or f"w{workers}"
     path = WORK / f"{preset}.structure.{tag}.swfx"
     cmd = [str(DRIVER), "--structure", preset, str(path), str(STRUCTURE_STEPS), str(workers)]
-    if damped:
+    if variant == "damped":
         cmd.append(repr(LINE_DAMPING))
+    elif variant == "compressed":
+        cmd += ["0", repr(COMPRESSION_RATIO)]
     print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     fixture = swfx.read(str(path))
@@ -184,7 +191,8 @@ def main(argv: list[str]) -> int:
             run_scene(preset)
             for workers in (0, 1):
                 run_structure(preset, workers)
-            run_structure(preset, 1, damped=True)
+            run_structure(preset, 1, "damped")
+            run_structure(preset, 1, "compressed")
     return 0
 
 
