@@ -331,6 +331,23 @@ sw_status sw_structure_checkpoint(const sw_structure* s, sw_checkpoint** out);
 sw_status sw_structure_restore(sw_structure* s, const sw_checkpoint* checkpoint);
 void sw_checkpoint_release(sw_checkpoint* checkpoint);
 
+/* serializeStructureCheckpoint (structure_checkpoint_persistence.cpp:229-333)
+ * of a checkpoint of `s`: the SWST bytes the reference writes for the same
+ * state, its SoftBody section the SWBC bytes of serializeSoftBodyCheckpoint
+ * (soft_body_checkpoint_persistence.cpp:858-919). *size receives the
+ * encoded length; with out == NULL nothing else is written. Structures with
+ * a suspension (whose SWSC section this codec does not write) and with
+ * contact spheres (no reference counterpart) are rejected with
+ * SW_INVALID_ARGUMENT. */
+sw_status sw_checkpoint_serialize(const sw_structure* s, const sw_checkpoint* checkpoint, uint8_t* out,
+                                  size_t capacity, size_t* size);
+/* deserializeStructureCheckpoint (structure_checkpoint_persistence.cpp:
+ * 335-520): reads SWST bytes written for `s`'s definition into a new
+ * checkpoint for sw_structure_restore. Malformed, truncated, mismatched
+ * (fingerprints, counts) or inconsistent input is rejected with
+ * SW_INVALID_ARGUMENT and the reference's reason. */
+sw_status sw_checkpoint_deserialize(sw_structure* s, const uint8_t* bytes, size_t size, sw_checkpoint** out);
+
 /* StructureCheckpoint's public fields (structure.h:254-271): version
  * (structureCheckpointVersion = 2), definitionFingerprint,
  * acceptedStepCount, simulationTimeSeconds and the node count; then the

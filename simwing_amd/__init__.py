@@ -618,6 +618,24 @@ class Structure:
     def restore(self, checkpoint: "Checkpoint") -> None:
         check(native.library().sw_structure_restore(self._handle, checkpoint._handle))
 
+    def serialize_checkpoint(self, checkpoint: "Checkpoint") -> bytes:
+        """The reference's SWST bytes of a checkpoint of this structure
+        (sw_checkpoint_serialize, serializeStructureCheckpoint)."""
+        lib = native.library()
+        size = C.c_size_t()
+        check(lib.sw_checkpoint_serialize(self._handle, checkpoint._handle, None, 0, C.byref(size)))
+        out = (C.c_uint8 * size.value)()
+        check(lib.sw_checkpoint_serialize(self._handle, checkpoint._handle, out, size.value, C.byref(size)))
+        return bytes(out)
+
+    def deserialize_checkpoint(self, data: bytes) -> "Checkpoint":
+        """A checkpoint of this structure read from SWST bytes
+        (sw_checkpoint_deserialize, deserializeStructureCheckpoint)."""
+        buffer = (C.c_uint8 * len(data)).from_buffer_copy(data)
+        handle = C.c_void_p()
+        check(native.library().sw_checkpoint_deserialize(self._handle, buffer, len(data), C.byref(handle)))
+        return Checkpoint(handle)
+
     def colouring(self, family: int):
         lib = native.library()
         nc, no = C.c_size_t(), C.c_size_t()

@@ -52,6 +52,7 @@ SOURCES = [
     ("frame.cpp", False),
     ("scene_io.cpp", False),
     ("wing_generator.cpp", False),
+    ("checkpoint_codec.cpp", False),
 ]
 
 

@@ -1,6 +1,7 @@
 // extern "C" boundary of the library (include/simwing_xpbd.h): the
 // Structure adapter semantics (src/fsi/structure.cpp:682-1025) over the
 // device wing, scene generation and assembly, and error translation.
+#include "checkpoint_codec.hpp"
 #include "../../include/simwing_xpbd.h"
 
 #include "assembly.hpp"
@@ -606,6 +607,51 @@ sw_status sw_structure_restore(sw_structure* s, const sw_checkpoint* c) {
 }
 
 void sw_checkpoint_release(sw_checkpoint* checkpoint) { delete checkpoint; }
+
+sw_status sw_checkpoint_serialize(const sw_structure* s, const sw_checkpoint* c, uint8_t* out, size_t capacity,
+                                  size_t* size) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(c, "checkpoint");
+        requirePointer(size, "size");
+        if (c->fingerprint != s->fingerprint)
+            throw std::invalid_argument("invalid Structure checkpoint or persistence limits");
+        swx::CheckpointImage image;
+        image.fingerprint = c->fingerprint;
+        image.acceptedSteps = c->acceptedSteps;
+        image.time = c->time;
+        image.state = c->state;
+        image.pending = c->pending;
+        for (int k = 0; k < 3; ++k) image.lastApplied[k] = c->lastApplied[k];
+        const std::vector<std::uint8_t> bytes = swx::encodeStructureCheckpoint(s->definition, image);
+        *size = bytes.size();
+        if (!out) return;
+        if (capacity < bytes.size()) throw std::invalid_argument("checkpoint buffer is smaller than the encoding");
+        std::copy(bytes.begin(), bytes.end(), out);
+    });
+}
+
+sw_status sw_checkpoint_deserialize(sw_structure* s, const uint8_t* bytes, size_t size, sw_checkpoint** out) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(bytes, "bytes");
+        requirePointer(out, "out");
+        *out = nullptr;
+        swx::CheckpointImage image;
+        image.state = s->wing->download();  // the shape of every state array
+        swx::decodeStructureCheckpoint(s->definition, s->fingerprint, bytes, size, image);
+        auto c = std::make_unique<sw_checkpoint>();
+        c->fingerprint = image.fingerprint;
+        c->acceptedSteps = image.acceptedSteps;
+        c->time = image.time;
+        // Contact-sphere multipliers restart from zero every substep.
+        std::fill(image.state.sphereLambda.begin(), image.state.sphereLambda.end(), 0.0);
+        c->state = std::move(image.state);
+        c->pending = std::move(image.pending);
+        for (int k = 0; k < 3; ++k) c->lastApplied[k] = image.lastApplied[k];
+        *out = c.release();
+    });
+}
 
 sw_status sw_checkpoint_info(const sw_checkpoint* c, uint32_t* version, uint64_t* fingerprint,
                              uint64_t* accepted_steps, double* simulation_time, size_t* node_count) {

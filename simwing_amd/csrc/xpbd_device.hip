@@ -2402,6 +2402,8 @@ T* deviceCopy(std::vector<void*>& owned, const std::vector<T>& host, std::size_t
     return static_cast<T*>(p);
 }
 
+}  // namespace
+
 // Membrane compliance from the orthotropic material: the checkedInverse
 // cofactors of the stiffness {xx warp, yy weft, zz shear, xy coupling,
 // xz 0, yz 0} (membrane.cpp:268-271), with the four stiffness entries scaled
@@ -2422,6 +2424,8 @@ void membraneCompliance(const double* mat, double stiffnessScale, double out[6])
     out[4] = (xy * yz - xz * yy) * r;
     out[5] = (xy * xz - xx * yz) * r;
 }
+
+namespace {
 
 template <typename T>
 std::vector<T> permute(const std::vector<T>& values, const std::vector<std::uint32_t>& order, std::size_t width = 1) {

@@ -258,6 +258,8 @@ _SIGNATURES = {
     "sw_checkpoint_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_double), C.POINTER(C.c_size_t)]),
     "sw_checkpoint_nodes": (C.c_int, [C.c_void_p, _F64P, _F64P, _F64P, _F64P, _F64P]),
+    "sw_checkpoint_serialize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "sw_checkpoint_deserialize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "sw_structure_suspension_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), _F64P, _F64P, _F64P,
                                                 C.POINTER(C.c_uint64), _F64P, _F64P, _F64P]),
     "sw_scene_read": (C.c_int, [_U8P, C.c_size_t, C.POINTER(C.c_void_p)]),

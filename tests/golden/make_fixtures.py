@@ -24,6 +24,8 @@ wings, and writes into tests/golden/:
   first step and after the last, the checkpoint's public fields and
   serializeStructureCheckpoint's bytes (tiny: the bytes; small: their
   SHA-256 digest and size);
+  (also, with workerThreads 1 only, for the tethered and free wings, whose
+  checkpoints have no suspension section);
 * ``<preset>.structure.damped.swfx`` -- the same with workerThreads 1 and
   every suspension segment's axial damping set to LINE_DAMPING;
 * ``<preset>.structure.compressed.swfx`` -- the same with workerThreads 1 and
@@ -72,6 +74,9 @@ STRIDE = 101
 # Written as digests and strided samples (the full arrays are too large).
 DIGESTED = {"export", "small-tethered", "small-free"}
 FRAME_STEPS = {"tiny": 20, "small": 40}
+# Wings without a suspension whose Structure checkpoints (SWST around SWBC
+# alone) are also kept.
+STRUCTURE_ONLY = {"tiny-tethered", "tiny-free", "small-tethered"}
 
 
 def digest(array: np.ndarray) -> np.ndarray:
@@ -173,7 +178,7 @@ def run_structure(preset: str, workers: int, variant: str = "") -> None:
     print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     fixture = swfx.read(str(path))
-    if preset != "tiny":
+    if not preset.startswith("tiny"):
         data = fixture.pop("checkpoint_bytes")
         fixture["checkpoint_bytes_digest"] = digest(data)
         fixture["checkpoint_bytes_size"] = np.array([data.size], np.uint64)
@@ -193,6 +198,8 @@ def main(argv: list[str]) -> int:
                 run_structure(preset, workers)
             run_structure(preset, 1, "damped")
             run_structure(preset, 1, "compressed")
+        if preset in STRUCTURE_ONLY:
+            run_structure(preset, 1)
     return 0
 
 
