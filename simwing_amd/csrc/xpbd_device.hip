@@ -254,7 +254,7 @@ __device__ __forceinline__ unsigned xccId() { return __builtin_amdgcn_s_getreg(2
 // launch count as empty. Participants therefore all run on one XCD,
 // whatever the dispatcher's placement, provided one XCD receives at least
 // `participants` workgroups (the host launches 8 x participants).
-__device__ int xcdParticipant(const WingView& v, int participants) {
+__device__ __forceinline__ int xcdParticipant(const WingView& v, int participants) {
     const unsigned xcc = xccId();
     unsigned* counter = v.barrier + 96 + xcc;
     unsigned* claim = v.barrier + 112;
@@ -1021,7 +1021,7 @@ __device__ __forceinline__ void recordSegment(double* r, const d3& pa, const d3&
 }
 
 // `rec`: the iteration's segment records (recording only).
-__device__ __noinline__ void runChain(const WingView& v, const NodeRefs& N, const Phase P, PayloadDev& payload,
+__device__ __forceinline__ void runChain(const WingView& v, const NodeRefs& N, const Phase P, PayloadDev& payload,
                                       const StepArgs& a, double* rec) {
     const int slots = P.slotEnd - P.slotBegin;
     const int items = P.end - P.begin;
@@ -1282,7 +1282,7 @@ __device__ __forceinline__ void projectSphere(const WingView& v, const NodeRefs&
 
 // Payload prediction of beginSubstep (suspension.cpp:2072-2182 with
 // integratePayloadTorqueFree, suspension.cpp:1075-1102).
-__device__ bool beginPayload(PayloadDev& p, const StepArgs& a) {
+__device__ __forceinline__ bool beginPayload(PayloadDev& p, const StepArgs& a) {
     for (int k = 0; k < 3; ++k) p.prevCom[k] = p.com[k];
     for (int k = 0; k < 4; ++k) p.prevQ[k] = p.q[k];
     const d3 force{0.0 + p.mass * a.gravity[0], 0.0 + p.mass * a.gravity[1], 0.0 + p.mass * a.gravity[2]};
@@ -1322,7 +1322,7 @@ __device__ bool beginPayload(PayloadDev& p, const StepArgs& a) {
 
 // reconstructPayloadVelocity (suspension.cpp:2386-2411). False when the
 // pose delta does not normalise.
-__device__ bool reconstructPayloadVelocity(PayloadDev& p, const StepArgs& a) {
+__device__ __forceinline__ bool reconstructPayloadVelocity(PayloadDev& p, const StepArgs& a) {
     const d3 vel{(p.com[0] - p.prevCom[0]) / a.dt, (p.com[1] - p.prevCom[1]) / a.dt,
                  (p.com[2] - p.prevCom[2]) / a.dt};
     const quat q{p.q[0], p.q[1], p.q[2], p.q[3]};
@@ -1372,7 +1372,7 @@ struct MomentumDev {
 };
 
 template <class Store>
-__device__ MomentumDev measureMomentum(const WingView& v, const Store& N) {
+__device__ __forceinline__ MomentumDev measureMomentum(const WingView& v, const Store& N) {
     const int lane = static_cast<int>(threadIdx.x) & 63;
     MomentumDev r{0.0, d3{0.0, 0.0, 0.0}, d3{0.0, 0.0, 0.0}, d3{0.0, 0.0, 0.0}};
     const int n = v.nodeCount;
@@ -1418,7 +1418,7 @@ __device__ MomentumDev measureMomentum(const WingView& v, const Store& N) {
 // mass that restores the angular momentum. One whole wave; false when the
 // inertia solve fails (the reference throws from checkedSolve).
 template <class Store>
-__device__ bool restoreMomentum(const WingView& v, const Store& N, const MomentumDev& target) {
+__device__ __forceinline__ bool restoreMomentum(const WingView& v, const Store& N, const MomentumDev& target) {
     if (!(target.mass > 0.0)) return true;
     const int lane = static_cast<int>(threadIdx.x) & 63;
     const int n = v.nodeCount;
@@ -1485,7 +1485,7 @@ __device__ __forceinline__ bool boundedAxialDampingImpulse(double relative, doub
 // each seeing the velocities the earlier ones left. One lane; false after
 // raising a failure.
 template <class Store>
-__device__ bool dampLines(const WingView& v, const Store& N, PayloadDev& payload, const StepArgs& a) {
+__device__ __forceinline__ bool dampLines(const WingView& v, const Store& N, PayloadDev& payload, const StepArgs& a) {
     bool any = false;
     for (int j = 0; j < v.dampCount; ++j) any = any || v.slam[v.dampSlot[j]] > 0.0;
     if (!any) return true;
@@ -1612,7 +1612,7 @@ __device__ __forceinline__ int cablePairsOf(const StepArgs& a, int it) {
 // pairs) and the compressionStiffnessRatio branch; the host launches the
 // general kernel variant whenever a wing's programs or definition use them.
 template <bool kGeneral>
-__device__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, int first, int end, int stride,
+__device__ __forceinline__ void runPhase(const WingView& v, const NodeRefs& N, const Phase& P, int first, int end, int stride,
                          PayloadDev& payload, const StepArgs& a, double* rec, const TraceLayout& L) {
     switch (P.type) {
     case kPhaseDistance:
