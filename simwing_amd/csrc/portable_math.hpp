@@ -238,8 +238,9 @@ SWX_PM dd sqrtDD(dd a) {  // a >= 0
     return fastTwoSum(s, e);
 }
 
-// atan of t = (th, tl) in [0, 1] as a double-double.
-SWX_PM dd atanUnit(dd t) {
+// atan of t = (th, tl) in [0, 1] as a double-double. `table` holds
+// kAtanTable's values (a device caller may pass a copy in LDS).
+SWX_PM dd atanUnit(dd t, const double (*table)[2] = kAtanTable) {
     const int k = static_cast<int>(t.hi * 64.0 + 0.5);
     const double c = k * 0.015625;
     const dd num = twoSum(t.hi - c, t.lo);  // t.hi - c is exact (Sterbenz)
@@ -258,14 +259,14 @@ SWX_PM dd atanUnit(dd t) {
     dd uw = twoProd(u.hi, w.hi);
     uw.lo += u.hi * w.lo + u.lo * w.hi;
     const dd series = add(u, fastTwoSum(uw.hi, uw.lo));
-    return add(dd{kAtanTable[k][0], kAtanTable[k][1]}, series);
+    return add(dd{table[k][0], table[k][1]}, series);
 }
 
 // atan2 of non-negative double-double operands, both finite and not both zero.
-SWX_PM dd atan2Positive(dd y, dd x) {
+SWX_PM dd atan2Positive(dd y, dd x, const double (*table)[2] = kAtanTable) {
     const bool swap = y.hi > x.hi;
     const dd t = swap ? quotient(x, y) : quotient(y, x);
-    dd r = atanUnit(t);
+    dd r = atanUnit(t, table);
     if (swap) r = sub(dd{kPio2Hi, kPio2Lo}, r);
     return r;
 }
@@ -273,7 +274,7 @@ SWX_PM dd atan2Positive(dd y, dd x) {
 SWX_PM double roundDD(dd a) { return a.hi + a.lo; }
 
 // atan2(y, x) with the special values of C99 Annex F.
-SWX_PM double atan2(double y, double x) {
+SWX_PM double atan2(double y, double x, const double (*table)[2] = kAtanTable) {
     if (x != x || y != y) return x + y;
     const double ax = fabs(x), ay = fabs(y);
     const bool negX = copysign(1.0, x) < 0.0;
@@ -288,7 +289,7 @@ SWX_PM double atan2(double y, double x) {
     } else if (ax == 0.0) {
         r = kPio2Hi;
     } else {
-        dd a = atan2Positive(dd{ay, 0.0}, dd{ax, 0.0});
+        dd a = atan2Positive(dd{ay, 0.0}, dd{ax, 0.0}, table);
         if (negX) a = sub(dd{kPiHi, kPiLo}, a);
         r = roundDD(a);
     }

@@ -581,6 +581,12 @@ __device__ __forceinline__ HingeItem hingeItem(const WingView& v, const NodeRefs
     return it;
 }
 
+// The atan table of the portable atan2 (pm::kAtanTable), copied into LDS at
+// the start of every substep (substepBody): a lookup in global memory would
+// miss in the caches each hinge phase, after the acquire that precedes it has
+// invalidated them.
+__shared__ double atanLds[65][2];
+
 // Dihedral hinge (soft_body.cpp:1648-1711 and 1754-1792).
 // remainder(x, 2π) (the double nearest 2π): for |x| < 3π the quotient is
 // -1, 0 or 1, and |x| - 2π is exact (Sterbenz), so the result equals the
@@ -633,7 +639,7 @@ __device__ __forceinline__ void projectHinge(const WingView& v, const NodeRefs& 
     const d3 cAdj = cross(a1Adj, edge);
     edgeAdj = add(edgeAdj, cross(a2Adj, toD));
     const d3 dAdj = cross(edge, a2Adj);
-    const double angle = pm::atan2(sine, cosine);
+    const double angle = pm::atan2(sine, cosine, atanLds);
     const d3 g[4] = {scale(add(add(edgeAdj, cAdj), dAdj), -1.0), edgeAdj, cAdj, dAdj};
     if (!isfinite(angle) || !fin3(g[0]) || !fin3(g[1]) || !fin3(g[2]) || !fin3(g[3])) return;
     double wgrad = 0.0;
@@ -1678,6 +1684,11 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
     if (blocks == 1 && __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int n = v.nodeCount;
     const bool shared = kMode == 1 || (kMode == 2 && v.sharedNodes != 0);
+    for (int i = threadIdx.x; i < 65; i += blockDim.x) {
+        atanLds[i][0] = pm::kAtanTable[i][0];
+        atanLds[i][1] = pm::kAtanTable[i][1];
+    }
+    __syncthreads();
     const bool dedicated = !shared && v.neighbourSync != 0 && v.dedicatedChain != 0 && blocks > 1 && v.hasRun;
     const int workers = dedicated ? blocks - 1 : blocks;
     // The chain workgroup takes part in the grid barriers around the
