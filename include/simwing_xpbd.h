@@ -333,18 +333,26 @@ void sw_checkpoint_release(sw_checkpoint* checkpoint);
 
 /* serializeStructureCheckpoint (structure_checkpoint_persistence.cpp:229-333)
  * of a checkpoint of `s`: the SWST bytes the reference writes for the same
- * state, its SoftBody section the SWBC bytes of serializeSoftBodyCheckpoint
- * (soft_body_checkpoint_persistence.cpp:858-919). *size receives the
- * encoded length; with out == NULL nothing else is written. Structures with
- * a suspension (whose SWSC section this codec does not write) and with
- * contact spheres (no reference counterpart) are rejected with
- * SW_INVALID_ARGUMENT. */
+ * state, with the SoftBody section of serializeSoftBodyCheckpoint
+ * (soft_body_checkpoint_persistence.cpp:858-919) and, with a pilot, the
+ * suspension section of serializeSuspensionCheckpoint
+ * (suspension_checkpoint_persistence.cpp:752-800), whose diagnostics are
+ * those SuspensionSystem::finishSubstep leaves after the last substep
+ * (suspension.cpp:2481-2610), restated from the committed state (or, for a
+ * checkpoint read from bytes and not stepped since, the bytes read). *size
+ * receives the encoded length; with out == NULL nothing else is written.
+ * Rejected with SW_INVALID_ARGUMENT: structures with contact spheres (no
+ * reference counterpart) and, after a step, damped suspension lines (their
+ * damping impulses are diagnostics the device does not record). */
 sw_status sw_checkpoint_serialize(const sw_structure* s, const sw_checkpoint* checkpoint, uint8_t* out,
                                   size_t capacity, size_t* size);
 /* deserializeStructureCheckpoint (structure_checkpoint_persistence.cpp:
  * 335-520): reads SWST bytes written for `s`'s definition into a new
- * checkpoint for sw_structure_restore. Malformed, truncated, mismatched
- * (fingerprints, counts) or inconsistent input is rejected with
+ * checkpoint for sw_structure_restore; until the restored structure steps,
+ * its suspensionState() and checkpoints report the diagnostics the bytes
+ * hold. Malformed, truncated, mismatched (fingerprints, counts, checksums,
+ * the suspension's state fingerprint) or inconsistent input, and suspension
+ * sections commanding controls or ground contact, are rejected with
  * SW_INVALID_ARGUMENT and the reference's reason. */
 sw_status sw_checkpoint_deserialize(sw_structure* s, const uint8_t* bytes, size_t size, sw_checkpoint** out);
 

@@ -47,7 +47,9 @@
 // surface reports:
 // definitionFingerprint(), suspensionState() before the first step and
 // after the last, the committed node states, the checkpoint's public fields
-// and serializeStructureCheckpoint's bytes of that checkpoint.
+// and serializeStructureCheckpoint's bytes of that checkpoint and of the
+// checkpoint before the first step, and the canonical text of the
+// suspension definition the Structure builds.
 #include "fixture_io.hpp"
 #include "oracle_colouring.hpp"
 
@@ -718,6 +720,82 @@ void writeSuspensionState(swfx::Writer& w, const std::string& tag, const fsi::St
     w.f64(tag + "/segment_tension", tension);
 }
 
+// The softwing::SuspensionDefinition that Structure's buildSuspension
+// (structure.cpp:459-548) makes of a structure definition's suspension, as
+// the canonical text of serializeSuspensionDefinition (suspension.cpp:
+// 2642-2760). Its checkpoint topology fingerprint hashes this text.
+std::string suspensionText(const fsi::StructureDefinition& def) {
+    const fsi::StructureSuspensionDefinition& source = *def.suspension;
+    const char* provenanceId = "simwing-structure-resolved";
+    const auto id = [](fsi::StructureSuspensionEndpointKind kind, std::uint64_t stable) {
+        switch (kind) {
+        case fsi::StructureSuspensionEndpointKind::SurfaceAttachment: return "attachment:" + std::to_string(stable);
+        case fsi::StructureSuspensionEndpointKind::Junction: return "junction:" + std::to_string(stable);
+        default: return "harness:" + std::to_string(stable);
+        }
+    };
+    const auto kind = [](fsi::StructureSuspensionEndpointKind k) {
+        switch (k) {
+        case fsi::StructureSuspensionEndpointKind::SurfaceAttachment: return softwing::SuspensionEndpointKind::Attachment;
+        case fsi::StructureSuspensionEndpointKind::Junction: return softwing::SuspensionEndpointKind::Junction;
+        default: return softwing::SuspensionEndpointKind::HangPoint;
+        }
+    };
+    const auto vec = [](const fsi::StructureVector3& v) { return softwing::Vec3{v.x, v.y, v.z}; };
+    softwing::SuspensionDefinition d;
+    d.identifier = "simwing-structure";
+    d.description = "Resolved scene-v2 suspension and rigid pilot";
+    d.unitsFrameTag = std::string(softwing::suspensionStage5FrameTag);
+    d.provenance = {{provenanceId, "authoritative scene-v2 stable topology"}};
+    for (const auto& a : source.attachments)
+        d.attachments.push_back({id(fsi::StructureSuspensionEndpointKind::SurfaceAttachment, a.stableId),
+                                 "resolved-node:" + std::to_string(a.node), {}, softwing::SuspensionSide::Centre, {},
+                                 provenanceId});
+    for (const auto& j : source.junctions) {
+        const auto& node = def.nodes[j.node];
+        const double inverseMass = 1.0 / node.massKg;
+        d.junctions.push_back({id(fsi::StructureSuspensionEndpointKind::Junction, j.stableId), vec(node.positionMeters),
+                               1.0 / inverseMass, softwing::SuspensionSide::Centre, provenanceId});
+    }
+    d.payload.mass = source.pilotMassKg;
+    d.payload.centreOfMassLocal = vec(source.pilotCenterOfMassLocalMeters);
+    const auto& I = source.pilotPrincipalInertiaKgSquareMeters;
+    d.payload.inertiaBody = {{I.x, 0.0, 0.0, 0.0, I.y, 0.0, 0.0, 0.0, I.z}};
+    const auto& q = source.pilotInitialBodyToWorld;
+    d.payload.initialState = {vec(source.pilotInitialCenterOfMassWorldMeters), {q.w, q.x, q.y, q.z},
+                              vec(source.pilotInitialLinearVelocityMetersPerSecond),
+                              vec(source.pilotInitialAngularVelocityRadiansPerSecond)};
+    d.payload.provenanceId = provenanceId;
+    for (const auto& h : source.harnessPoints)
+        d.payload.hangPoints.push_back({id(fsi::StructureSuspensionEndpointKind::PilotHarness, h.stableId),
+                                        vec(h.localPositionMeters), softwing::SuspensionSide::Centre, provenanceId});
+    for (const auto& g : source.segments)
+        d.segments.push_back({"segment:" + std::to_string(g.stableId),
+                              {kind(g.from.kind), id(g.from.kind, g.from.stableId)},
+                              {kind(g.to.kind), id(g.to.kind, g.to.stableId)},
+                              g.restLengthMeters,
+                              g.axialStiffnessNewtons,
+                              g.axialDampingNewtonSecondsPerMeter,
+                              softwing::SuspensionSide::Centre,
+                              {"role:" + std::to_string(g.role)},
+                              provenanceId});
+    d.solver.lineIterations = source.solverIterations;
+    d.solver.attachmentTolerance = source.attachmentTolerance;
+    d.solver.minimumLineLength = source.minimumLineLengthMeters;
+    d.solver.maximumLineResidual = source.maximumLineResidualMeters;
+    d.solver.maximumControlWork = source.maximumControlWorkJoules;
+    d.ground.mode = softwing::PayloadGroundMode::Free;
+    return softwing::serializeSuspensionDefinition(d);
+}
+
+std::vector<std::uint8_t> checkpointBytes(const fsi::Structure& structure) {
+    std::vector<std::uint8_t> bytes;
+    fsi::StructureCheckpointPersistenceError error;
+    if (!fsi::serializeStructureCheckpoint(structure, structure.checkpoint(), bytes, &error))
+        throw std::runtime_error("serializeStructureCheckpoint: " + error.message);
+    return bytes;
+}
+
 int structureMode(const std::string& preset, const std::string& out, int steps, unsigned workers,
                   double lineDamping, double compressionRatio) {
     Prepared p = prepare(preset);
@@ -735,6 +813,11 @@ int structureMode(const std::string& preset, const std::string& out, int steps, 
     w.u64("workers", {workers});
     w.u64("fingerprint", {structure.definitionFingerprint()});
     writeSuspensionState(w, "initial", structure);
+    w.u8("checkpoint_bytes_initial", checkpointBytes(structure));
+    if (def.suspension) {
+        const std::string text = suspensionText(def);
+        w.u8("suspension_text", std::vector<std::uint8_t>(text.begin(), text.end()));
+    }
     fsi::StructureStepSettings settings;
     settings.timeStepSeconds = 1.0 / 120.0;
     settings.substeps = 1;

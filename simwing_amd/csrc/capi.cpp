@@ -79,6 +79,17 @@ void requirePointer(const void* p, const char* what) {
 
 }  // namespace
 
+// What the suspension's checkpoint section restates beyond the device state:
+// the last substep's gravity and line iterations, and a section read from
+// bytes (kept verbatim until the next step) with its diagnostics'
+// per-segment length, commanded rest length and tension (runtime order).
+struct SuspensionRecord {
+    double gravity[3] = {0.0, 0.0, 0.0};
+    int lineIterations = 0;
+    std::vector<std::uint8_t> section;
+    std::vector<double> length, commanded, tension;
+};
+
 struct sw_checkpoint {
     std::uint64_t fingerprint = 0;
     std::uint64_t acceptedSteps = 0;
@@ -89,6 +100,7 @@ struct sw_checkpoint {
     double maxResidual = 0.0;
     double lastSubstepDt = 0.0;
     bool suspensionStepped = false;
+    SuspensionRecord record;
 };
 
 struct sw_structure {
@@ -108,6 +120,10 @@ struct sw_structure {
     // lengths and zero tension, structure.cpp:765-768).
     double lastSubstepDt = 0.0;
     bool suspensionStepped = false;
+    // The gravity and line iterations of the last substep, and a
+    // suspension checkpoint section restored from bytes and not stepped
+    // since, for the checkpoint codec and suspensionState().
+    SuspensionRecord record;
     std::vector<std::string> segmentNames;  // runtime order
     bool hasLoadPath = false;
 };
@@ -194,6 +210,18 @@ swx::StepArgs prepare(sw_structure& s, const sw_step_settings& st, bool serialAl
 }
 
 void runSteps(sw_structure& s, const sw_step_settings& settings, int count, sw_diagnostics* out);
+
+// Bookkeeping of accepted steps' last substep (solo and batched steps).
+void noteSubsteps(sw_structure& s, double dt, const sw_step_settings& settings) {
+    s.lastSubstepDt = dt;
+    s.suspensionStepped = s.suspensionStepped || s.definition.suspension.has_value();
+    for (int k = 0; k < 3; ++k) s.record.gravity[k] = settings.gravity[k];
+    s.record.lineIterations = settings.constraint_iterations;
+    s.record.section.clear();
+    s.record.length.clear();
+    s.record.commanded.clear();
+    s.record.tension.clear();
+}
 
 void fillDiagnostics(sw_structure& s, sw_diagnostics* out) {
     if (!out) return;
@@ -283,8 +311,7 @@ void runSteps(sw_structure& s, const sw_step_settings& settings, int count, sw_d
         s.pendingDirty = true;
         s.acceptedSteps += static_cast<std::uint64_t>(steps);
         for (int step = 0; step < steps; ++step) s.time += settings.time_step;
-        s.lastSubstepDt = args.dt;
-        s.suspensionStepped = s.suspensionStepped || s.definition.suspension.has_value();
+        noteSubsteps(s, args.dt, settings);
     };
     if (e.code != 0) {
         s.wing->restoreSnapshot();
@@ -579,6 +606,7 @@ sw_status sw_structure_checkpoint(const sw_structure* s, sw_checkpoint** out) {
         c->maxResidual = s->maxResidual;
         c->lastSubstepDt = s->lastSubstepDt;
         c->suspensionStepped = s->suspensionStepped;
+        c->record = s->record;
         *out = c.release();
     });
 }
@@ -603,6 +631,7 @@ sw_status sw_structure_restore(sw_structure* s, const sw_checkpoint* c) {
         s->maxResidual = c->maxResidual;
         s->lastSubstepDt = c->lastSubstepDt;
         s->suspensionStepped = c->suspensionStepped;
+        s->record = c->record;
     });
 }
 
@@ -623,6 +652,12 @@ sw_status sw_checkpoint_serialize(const sw_structure* s, const sw_checkpoint* c,
         image.state = c->state;
         image.pending = c->pending;
         for (int k = 0; k < 3; ++k) image.lastApplied[k] = c->lastApplied[k];
+        image.suspensionStepped = c->suspensionStepped;
+        image.substepDt = c->lastSubstepDt;
+        for (int k = 0; k < 3; ++k) image.gravity[k] = c->record.gravity[k];
+        image.lineIterations = c->record.lineIterations;
+        image.maxResidual = c->maxResidual;
+        image.suspensionSection = c->record.section;
         const std::vector<std::uint8_t> bytes = swx::encodeStructureCheckpoint(s->definition, image);
         *size = bytes.size();
         if (!out) return;
@@ -649,6 +684,14 @@ sw_status sw_checkpoint_deserialize(sw_structure* s, const uint8_t* bytes, size_
         c->state = std::move(image.state);
         c->pending = std::move(image.pending);
         for (int k = 0; k < 3; ++k) c->lastApplied[k] = image.lastApplied[k];
+        c->suspensionStepped = image.suspensionStepped;
+        c->maxResidual = image.maxResidual;
+        for (int k = 0; k < 3; ++k) c->record.gravity[k] = image.gravity[k];
+        c->record.lineIterations = image.lineIterations;
+        c->record.section = std::move(image.suspensionSection);
+        c->record.length = std::move(image.sectionLength);
+        c->record.commanded = std::move(image.sectionCommanded);
+        c->record.tension = std::move(image.sectionTension);
         *out = c.release();
     });
 }
@@ -702,12 +745,23 @@ sw_status sw_structure_suspension_state(const sw_structure* s, int* present, dou
         for (std::size_t r = 0; r < order.size(); ++r) lambda[order[r]] = st.segmentLambda[r];
         const std::size_t constraints = d.constraintCount();
         const double dt = s->lastSubstepDt;
+        // A restored checkpoint section reports its own diagnostics (those of
+        // the substep it was written after) until the next step.
+        const bool restored = !s->record.section.empty();
+        std::vector<double> length(segments), commanded(segments), tension(segments);
+        for (std::size_t r = 0; r < order.size() && restored && !s->record.length.empty(); ++r) {
+            length[order[r]] = s->record.length[r];
+            commanded[order[r]] = s->record.commanded[r];
+            tension[order[r]] = s->record.tension[r];
+        }
+        const bool stepped = restored ? !s->record.length.empty() : s->suspensionStepped;
         for (std::size_t j = 0; j < segments; ++j) {
             const double rest = sd.segmentRest[j];
             if (segment_ids) segment_ids[j] = sd.segmentIds[j];
-            if (segment_commanded_rest) segment_commanded_rest[j] = rest;
-            if (segment_length) segment_length[j] = s->suspensionStepped ? f.lineLength[constraints + j] : rest;
-            if (segment_tension) segment_tension[j] = s->suspensionStepped ? lambda[j] / (dt * dt) : 0.0;
+            if (segment_commanded_rest) segment_commanded_rest[j] = restored && stepped ? commanded[j] : rest;
+            if (segment_length)
+                segment_length[j] = !stepped ? rest : restored ? length[j] : f.lineLength[constraints + j];
+            if (segment_tension) segment_tension[j] = !stepped ? 0.0 : restored ? tension[j] : lambda[j] / (dt * dt);
         }
     });
 }
@@ -1088,6 +1142,7 @@ sw_status sw_batch_step(sw_batch* b, const sw_step_settings* settings, int count
             s.acceptedSteps += static_cast<std::uint64_t>(count);
             for (int step = 0; step < count; ++step) s.time += settings->time_step;
             s.maxResidual = residual[w];
+            noteSubsteps(s, settings->time_step / settings->substeps, *settings);
         }
         for (size_t w = 0; w < n; ++w)
             if (failed[w]) throwDeviceError(*b->wings[w], errors[w]);

@@ -63,6 +63,8 @@ struct PayloadDev {
     double iinv[9];
     double maxResidual;
     double minLineLength;
+    double prevV[3];  // velocities at the start of the substep
+    double prevW[3];  // (previousPayloadState, suspension.cpp:2085)
 };
 
 // First failure of a step: code (SW_* status), suspension phase, the
@@ -123,7 +125,7 @@ public:
         std::vector<double> position, previous, velocity;
         std::vector<double> constraintLambda, membraneLambda, membraneResultant, membraneResidual;
         std::vector<double> dihedralLambda, segmentLambda, sphereLambda, pressure;
-        std::vector<double> payload;  // com, q, v, w, prevCom, prevQ (20)
+        std::vector<double> payload;  // com, q, v, w, prevCom, prevQ, prevV, prevW (26)
     };
     State download() const;
     void upload(const State& state);

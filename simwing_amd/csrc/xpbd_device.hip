@@ -1291,6 +1291,8 @@ __device__ __forceinline__ void projectSphere(const WingView& v, const NodeRefs&
 __device__ __forceinline__ bool beginPayload(PayloadDev& p, const StepArgs& a) {
     for (int k = 0; k < 3; ++k) p.prevCom[k] = p.com[k];
     for (int k = 0; k < 4; ++k) p.prevQ[k] = p.q[k];
+    for (int k = 0; k < 3; ++k) p.prevV[k] = p.v[k];
+    for (int k = 0; k < 3; ++k) p.prevW[k] = p.w[k];
     const d3 force{0.0 + p.mass * a.gravity[0], 0.0 + p.mass * a.gravity[1], 0.0 + p.mass * a.gravity[2]};
     const d3 moment{0.0, 0.0, 0.0};
     if (!(a.dt > 0.0) || !isfinite(a.dt) || !fin3(force)) return false;
@@ -3053,6 +3055,8 @@ void DeviceWing::build(const Definition& d) {
             for (int k = 0; k < 4; ++k) p.q[k] = flip ? -q[k] : q[k];
             for (int k = 0; k < 3; ++k) p.prevCom[k] = p.com[k];
             for (int k = 0; k < 4; ++k) p.prevQ[k] = p.q[k];
+            for (int k = 0; k < 3; ++k) p.prevV[k] = p.v[k];
+            for (int k = 0; k < 3; ++k) p.prevW[k] = p.w[k];
             p.mass = s.pilotMass;
             p.minLineLength = s.minimumLineLength;
         }
@@ -3576,12 +3580,14 @@ DeviceWing::State DeviceWing::download() const {
     s.pressure.assign(at(v.pressure), at(v.pressure) + triangles_);
     const double* p = at(reinterpret_cast<double*>(v.payload));
     const PayloadDev* pd = reinterpret_cast<const PayloadDev*>(p);
-    s.payload.assign(20, 0.0);
+    s.payload.assign(26, 0.0);
     for (int k = 0; k < 3; ++k) {
         s.payload[k] = pd->com[k];
         s.payload[7 + k] = pd->v[k];
         s.payload[10 + k] = pd->w[k];
         s.payload[13 + k] = pd->prevCom[k];
+        s.payload[20 + k] = pd->prevV[k];
+        s.payload[23 + k] = pd->prevW[k];
     }
     for (int k = 0; k < 4; ++k) {
         s.payload[3 + k] = pd->q[k];
@@ -3629,6 +3635,8 @@ void DeviceWing::upload(const State& s) {
         pd->v[k] = s.payload[7 + k];
         pd->w[k] = s.payload[10 + k];
         pd->prevCom[k] = s.payload[13 + k];
+        pd->prevV[k] = s.payload[20 + k];
+        pd->prevW[k] = s.payload[23 + k];
     }
     for (int k = 0; k < 4; ++k) {
         pd->q[k] = s.payload[3 + k];

@@ -21,9 +21,10 @@ wings, and writes into tests/golden/:
   reference's simwing::fsi::Structure stepped 20 times with the flight loads
   (no face pressure) with workerThreads N (0: the serial sweep, 1: the
   coloured sweep): definitionFingerprint(), suspensionState() before the
-  first step and after the last, the checkpoint's public fields and
-  serializeStructureCheckpoint's bytes (tiny: the bytes; small: their
-  SHA-256 digest and size);
+  first step and after the last, the checkpoint's public fields,
+  serializeStructureCheckpoint's bytes after the last step and before the
+  first, and the canonical text of the suspension definition the Structure
+  builds (tiny: the bytes; small: their SHA-256 digests and sizes);
   (also, with workerThreads 1 only, for the tethered and free wings, whose
   checkpoints have no suspension section);
 * ``<preset>.structure.damped.swfx`` -- the same with workerThreads 1 and
@@ -179,9 +180,12 @@ def run_structure(preset: str, workers: int, variant: str = "") -> None:
     subprocess.run(cmd, check=True)
     fixture = swfx.read(str(path))
     if not preset.startswith("tiny"):
-        data = fixture.pop("checkpoint_bytes")
-        fixture["checkpoint_bytes_digest"] = digest(data)
-        fixture["checkpoint_bytes_size"] = np.array([data.size], np.uint64)
+        for key in ("checkpoint_bytes", "checkpoint_bytes_initial", "suspension_text"):
+            if key not in fixture:
+                continue
+            data = fixture.pop(key)
+            fixture[f"{key}_digest"] = digest(data)
+            fixture[f"{key}_size"] = np.array([data.size], np.uint64)
     swfx.write(str(HERE / f"{preset}.structure.{tag}.swfx"), fixture)
 
 
