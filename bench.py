@@ -134,24 +134,59 @@ def cpu_share() -> int:
     return max(1, min(cores, 16))
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(workload: str, counts_no_spheres: dict, iterations: int, budget_s: float) -> dict | None:
+    """The reference's own CPU solver (oracle/_ref/ref_driver --bench, built
+    from the reference sources) on the same wing, in the softwing-bench
+    manner: warm-up steps, steady_clock around each step, best of several
+    runs, and the reference's per-family StepPerformanceProfile split. Timed
+    with workerThreads = 1, 12 (the surface-flight setting) and P, the
+    reference's hardwarePhysicalCoreCount() (parallel.cpp:359-398, on Linux
+    std::thread::hardware_concurrency()), run on at most this process's CPU
+    share; the P run is the reported value."""
     driver = ROOT / "oracle" / "_ref" / "ref_driver"
-    workers = cpu_share()
+    share = cpu_share()
     if driver.exists():
-        cmd = [str(driver), "--bench", workload, str(workers), str(budget_s)]
-        out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
-        if out.returncode != 0:
-            raise RuntimeError(f"reference CPU baseline failed: {out.stderr.strip()}")
-        line = json.loads(out.stdout.strip().splitlines()[-1])
-        ms = line["ms_per_step_mean"]
+        def run(workers: int, seconds: float, warmup: int, runs: int) -> dict:
+            cmd = [str(driver), "--bench", workload, str(workers), str(seconds), str(warmup), str(runs)]
+            out = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+            if out.returncode != 0:
+                raise RuntimeError(f"reference CPU baseline failed: {out.stderr.strip()}")
+            return json.loads(out.stdout.strip().splitlines()[-1])
+        p_workers = max(1, min(os.cpu_count() or 1, share))  # hardware_concurrency, capped at the share
+        legs = {}
+        for workers, warmup, runs, share_of_budget in ((p_workers, 5, 5, 0.5), (12, 2, 3, 0.25), (1, 1, 3, 0.25)):
+            if workers in legs:
+                continue
+            legs[workers] = run(workers, budget_s * share_of_budget, warmup, runs)
+        best = legs[p_workers]
+        physical = int(best["physical_cores"])
+        ms = best["ms_per_step_best_run"]
+        per_leg = {str(w): {"ms_per_step_best_run": round(l["ms_per_step_best_run"], 3),
+                            "ms_per_step_mean": round(l["ms_per_step_mean"], 3), "warmup": l["warmup"],
+                            "runs": l["runs"], "steps": l["steps"]} for w, l in legs.items()}
         return {
             "value": projections_per_substep(counts_no_spheres, iterations) / (ms * 1e-3),
             "unit": UNIT,
-            "cores": workers,
+            "cores": p_workers,
             "kind": "reference",
-            "sample": (f"{workload} wing without contact spheres (the reference has no sphere family), "
-                       f"{line['warmup']} warm-up + {line['steps']} timed steps, workerThreads={workers}, "
-                       f"mean {ms:.1f} ms/step, best {line['ms_per_step_best']:.1f} ms/step"),
+            "sample": (f"{workload} wing without contact spheres (the reference has no sphere family); "
+                       f"workerThreads={p_workers} (P={physical} from hardwarePhysicalCoreCount, capped at this "
+                       f"process's {share}-core share), {best['warmup']} warm-up steps, best of {best['runs']} "
+                       f"runs ({best['steps']} timed steps): {ms:.1f} ms/step"),
+            "cpu_model": cpu_model(),
+            "physical_cores": physical,
+            "workers": per_leg,
+            "split_ms_per_step": best["split_ms"],
         }
     # Serial oracle (bit-identical to the reference path), one core.
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -217,6 +252,10 @@ def batched_leg(count: int, steps: int, seed: int = 0x5EED0002) -> dict:
     batched_rate = count * steps / batched_s
     solo_rate = count / solo_s
     batch.close()
+    # Roofline of the batched step: the wings' algorithmic bytes per substep
+    # (the per-family model above, summed over the wings) over its duration.
+    step_bytes = sum(bytes_per_substep(family_counts(c[0]), 48, False) for c in cases)
+    achieved = step_bytes / (batched_s / steps) / 1e9
     return {
         "metric": "batched wings/s",
         "value": batched_rate,
@@ -224,6 +263,8 @@ def batched_leg(count: int, steps: int, seed: int = 0x5EED0002) -> dict:
         "per_wing_launches": solo_rate,
         "speedup": batched_rate / solo_rate,
         "ms_per_batched_step": 1e3 * batched_s / steps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_step": step_bytes},
         "steps": steps,
         "workload": (f"{count} wings, {min(nodes)}-{max(nodes)} nodes (total {sum(nodes)}), seed {seed:#x}, "
                      f"dt 1/120 s, 48 iterations; {int(blocks.sum())} workgroups "

@@ -60,6 +60,7 @@
 #include "structure_checkpoint_persistence.h"
 #include "structure_frame.h"
 #include "viewer_protocol.h"
+#include <softwing/parallel.h>
 #include <softwing/soft_body.h>
 #include <softwing/suspension.h>
 
@@ -907,34 +908,57 @@ int main(int argc, char** argv) {
             return file ? 0 : 1;
         }
         if (argc >= 5 && std::strcmp(argv[1], "--bench") == 0) {
-            // Timing in the softwing-bench manner (tools/softwing_bench.cpp:1783-1801):
-            // warm-up steps, then steady_clock around each step until the budget is spent.
+            // Timing in the softwing-bench manner (tools/softwing_bench.cpp:
+            // 1783-1801): warm-up steps, then `runs` runs of steady_clock
+            // around each step (a run ends when its share of the budget is
+            // spent), the best run's mean reported beside the mean of all,
+            // with the per-family split of the reference's own
+            // StepPerformanceProfile (soft_body.h:136-170) over the timed steps.
             const std::string preset = argv[2];
             const unsigned workers = static_cast<unsigned>(std::atoi(argv[3]));
             const double budget = std::atof(argv[4]);
+            const int warmup = argc >= 6 ? std::atoi(argv[5]) : 2;
+            const int runs = argc >= 7 ? std::atoi(argv[6]) : 1;
             Prepared p = prepare(preset);
             Rig rig = buildRig(p.assembly.definition, p.hinges.order, p.wing.facePressure, p.assembly.mappings.nodeVertexIds.size());
-            const sw::StepSettings settings = stepSettings(workers);
-            const int warmup = 2;
+            sw::StepSettings settings = stepSettings(workers);
             for (int s = 0; s < warmup; ++s) stepRig(rig, settings);
-            double total = 0.0, best = 1e300;
+            sw::StepPerformanceProfile profile;
+            settings.performanceProfile = &profile;
+            double total = 0.0, bestRun = 1e300, bestStep = 1e300;
             int steps = 0;
-            while (steps < 3 || (total < budget && steps < 1000)) {
-                const auto t0 = std::chrono::steady_clock::now();
-                stepRig(rig, settings);
-                const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-                total += dt;
-                best = std::min(best, dt);
-                ++steps;
+            for (int run = 0; run < runs; ++run) {
+                double runTotal = 0.0;
+                int runSteps = 0;
+                while (runSteps < 1 || (runTotal < budget / runs && runSteps < 1000)) {
+                    const auto t0 = std::chrono::steady_clock::now();
+                    stepRig(rig, settings);
+                    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                    runTotal += dt;
+                    bestStep = std::min(bestStep, dt);
+                    ++runSteps;
+                }
+                total += runTotal;
+                steps += runSteps;
+                bestRun = std::min(bestRun, runTotal / runSteps);
             }
-            std::printf("{\"preset\": \"%s\", \"workers\": %u, \"warmup\": %d, \"steps\": %d, "
-                        "\"ms_per_step_mean\": %.6f, \"ms_per_step_best\": %.6f}\n",
-                        preset.c_str(), workers, warmup, steps, 1e3 * total / steps, 1e3 * best);
+            const auto ms = [&](std::uint64_t ns) { return 1e-6 * static_cast<double>(ns) / steps; };
+            std::printf("{\"preset\": \"%s\", \"workers\": %u, \"physical_cores\": %u, \"warmup\": %d, "
+                        "\"runs\": %d, \"steps\": %d, \"ms_per_step_mean\": %.6f, \"ms_per_step_best_run\": %.6f, "
+                        "\"ms_per_step_best\": %.6f, \"split_ms\": {\"prediction\": %.4f, \"distance\": %.4f, "
+                        "\"cable\": %.4f, \"membrane\": %.4f, \"bending\": %.4f, \"suspension\": %.4f, "
+                        "\"membrane_diagnostics\": %.4f, \"finalization\": %.4f, \"soft_body_total\": %.4f}}\n",
+                        preset.c_str(), workers, sw::hardwarePhysicalCoreCount(), warmup, runs, steps,
+                        1e3 * total / steps, 1e3 * bestRun, 1e3 * bestStep, ms(profile.predictionNanoseconds),
+                        ms(profile.distanceConstraintNanoseconds), ms(profile.cableConstraintNanoseconds),
+                        ms(profile.membraneConstraintNanoseconds), ms(profile.bendingConstraintNanoseconds),
+                        ms(profile.suspensionConstraintNanoseconds), ms(profile.membraneDiagnosticsNanoseconds),
+                        ms(profile.finalizationNanoseconds), ms(profile.softBodyTotalNanoseconds));
             return 0;
         }
         if (argc < 5) {
             std::fprintf(stderr, "usage: ref_driver <preset> <out-prefix> <steps> <workers> [snapshot ...]\n"
-                                 "       ref_driver --bench <preset> <workers> <seconds>\n"
+                                 "       ref_driver --bench <preset> <workers> <seconds> [warmup [runs]]\n"
                                  "       ref_driver --compare-workers <preset> <steps>\n");
             return 2;
         }
