@@ -36,6 +36,7 @@
 //        ref_driver --compare-workers <preset> <steps>
 //        ref_driver --frame <preset> <out-file> <steps>
 //        ref_driver --scene <preset> <out-file>
+//        ref_driver --assemble <scene-file>
 //        ref_driver --structure <preset> <out-file> <steps> <workers> [line-damping [compression-ratio]]
 //
 // The --structure mode drives simwing::fsi::Structure like --frame (colour-
@@ -886,6 +887,20 @@ int main(int argc, char** argv) {
                                  argc >= 7 ? std::atof(argv[6]) : 0.0, argc >= 8 ? std::atof(argv[7]) : 1.0);
         if (argc >= 5 && std::strcmp(argv[1], "--frame") == 0)
             return frameMode(argv[2], argv[3], std::atoi(argv[4]));
+        if (argc >= 3 && std::strcmp(argv[1], "--assemble") == 0) {
+            // readScene of a scene-v2 stream, then assembleSceneStructure with
+            // the flight settings: prints "ok" or the first diagnostic's
+            // message (the one callers such as requireStructureAssembly,
+            // surface_flight_case.cpp:15-27, report).
+            std::ifstream file(argv[2], std::ios::binary);
+            fsi::Scene scene;
+            std::string error;
+            if (!fsi::readScene(file, scene, &error)) throw std::runtime_error("readScene: " + error);
+            const fsi::SceneStructureAssembly assembly = fsi::assembleSceneStructure(scene, {}, flightSettings());
+            if (assembly.ok()) std::printf("ok\n");
+            else std::printf("%s\n", assembly.diagnostics.front().message.c_str());
+            return 0;
+        }
         if (argc >= 4 && std::strcmp(argv[1], "--scene") == 0) {
             // simwing::fsi::writeScene (scene.cpp:1051-1223) of the seeded
             // wing, and readScene of those bytes back (the stream must

@@ -252,12 +252,19 @@ void assembleHinges(const std::vector<const Triangle*>& triangles,
     }
     Definition& d = out.definition;
     for (const auto& [key, incidences] : edges) {
-        if (incidences.size() == 1) continue;
-        if (incidences.size() != 2) reject("non-manifold edge cannot define one fabric bending hinge");
+        if (incidences.size() == 1) continue;  // a boundary: no bending constraint
+        // UnsupportedBendingTopology diagnostics (scene_structure.cpp:311-335),
+        // with the reference's message; the reference goes on collecting
+        // diagnostics and then rejects the scene with the first of them.
+        const std::string edgeName = std::to_string(key[1]) + "-" + std::to_string(key[2]);
+        if (incidences.size() != 2)
+            reject("non-manifold edge " + edgeName + " in sheet " + std::to_string(key[0]) +
+                   " cannot define one fabric bending hinge");
         const HalfEdge& e0 = incidences[0];
         const HalfEdge& e1 = incidences[1];
         if (e0.from != e1.to || e0.to != e1.from)
-            reject("inconsistently oriented edge cannot define a signed fabric bending hinge");
+            reject("inconsistently oriented edge " + edgeName + " in sheet " + std::to_string(key[0]) +
+                   " cannot define a signed fabric bending hinge");
         const double rigidity0 = materials.at(e0.triangle->materialId)->bendingStiffness;
         const double rigidity1 = materials.at(e1.triangle->materialId)->bendingStiffness;
         if (!(rigidity0 > 0.0) || !(rigidity1 > 0.0)) continue;

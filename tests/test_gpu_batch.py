@@ -94,3 +94,26 @@ def test_failing_wing_is_rolled_back_alone():
     expected = solo_states([good], 1)[0]
     assert bits_equal(x, expected[0]) and bits_equal(v, expected[1])
     batch.close()
+
+
+def test_config_batch_of_64_wings_equals_solo():
+    """The benchmark's batched configuration: the 64 generated wings of seed
+    0x5EED0002 (2k-40k nodes each) in one persistent launch, 3 steps, each
+    wing's bits against the same wing stepped alone."""
+    count, steps = 64, 3
+    names = [f"batch:{0x5EED0002}:{i}:{count}" for i in range(count)]
+    cases = [flight_case(name) for name in names]
+    print(f"64 wings: {sum(d.node_count for d, _, _ in cases)} nodes", flush=True)
+    batch = sw.Batch([d for d, _, _ in cases])
+    for w, (_, pressure, _) in enumerate(cases):
+        batch.set_face_pressure(w, pressure)
+    for _ in range(steps):
+        for w, (_, _, loads) in enumerate(cases):
+            batch.set_external_forces(w, loads)
+        batch.step(1)
+    got = [batch.node_states(w) for w in range(count)]
+    batch.close()
+    print("batch stepped; stepping each wing alone", flush=True)
+    expected = solo_states(cases, steps)
+    for w, name in enumerate(names):
+        assert bits_equal(got[w][0], expected[w][0]) and bits_equal(got[w][1], expected[w][1]), name

@@ -69,7 +69,9 @@ def walk(data: bytes) -> dict:
     for _ in range(n):
         u64(), string(), take(16)
     n = counts["triangles"] = u32()
+    out["triangle_offsets"] = []
     for _ in range(n):
+        out["triangle_offsets"].append(at)
         take(8 + 24 + 48 + 32)
         u8()
     n = counts["openings"] = u32()
@@ -193,3 +195,27 @@ def test_rejections():
     duplicated = data[:second] + struct.pack("<Q", layout["vertex_ids"][0]) + data[second + 8:]
     status, message = read_status(duplicated)
     assert status == native.SW_INVALID_ARGUMENT and "vertex stable ID is duplicated" in message, message
+
+
+def test_bending_topology_rejections():
+    """UnsupportedBendingTopology (scene_structure.cpp:311-335): a triangle
+    whose corners are listed in the opposite order shares its edges with its
+    neighbours in the same direction, so no signed hinge exists; the scene is
+    rejected with the reference's message for the first such edge."""
+    data = reference_tiny()
+    layout = walk(data)
+    scene = sw.read_scene(data)
+    assert scene is not None
+    at = layout["triangle_offsets"][0]
+    a, b, c = struct.unpack("<QQQ", data[at + 8:at + 32])
+    charts = data[at + 32:at + 80]
+    # Swap the second and third corners' vertices (the material chart keeps
+    # its positive orientation).
+    flipped = data[:at + 8] + struct.pack("<QQQ", a, c, b) + charts + data[at + 80:]
+    flipped_scene = sw.read_scene(flipped)
+    with pytest.raises(sw.NativeError) as error:
+        flipped_scene.assemble()
+    # The reference's assembleSceneStructure of the same stream reports this
+    # first diagnostic (oracle/_ref/ref_driver --assemble on these bytes).
+    assert str(error.value).endswith(
+        "inconsistently oriented edge 1-18 in sheet 1 cannot define a signed fabric bending hinge")
