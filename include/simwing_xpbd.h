@@ -521,8 +521,9 @@ typedef struct sw_adjoint sw_adjoint;
 
 /* Reverse-mode gradients of the shape loss
  *     L = sum over nodes of |x(end) - x_target|^2
- * after `steps` steps of `settings` from the structure's current state, with
- * the loads pending now applied at every step (no counterpart in the
+ * after `steps` steps of `settings` from the structure's committed state at
+ * the start of each run (face pressure included), with the loads that were
+ * pending at configure applied at every step (no counterpart in the
  * reference, which has no differentiation). Parameters:
  *   - log-compliance per membrane element (definition order): the element's
  *     compliance matrix becomes C e^theta (its stiffness is scaled by
@@ -534,8 +535,9 @@ typedef struct sw_adjoint sw_adjoint;
  * steps; the reverse pass replays each interval from its checkpoint with
  * the trajectory recorded and runs the reverse kernels substep by substep
  * in FP64, carrying the Lagrange-multiplier adjoints through every colour
- * sweep and iteration. The structure must outlive the adjoint; its state,
- * step count and parameters are unchanged by every run. Membrane damping is
+ * sweep and iteration. The structure must outlive the adjoint; its state
+ * (face pressure included), step count and parameters are what they were
+ * before each run once the run returns, whether it succeeds or fails. Membrane damping is
  * rejected with SW_INVALID_ARGUMENT. */
 sw_status sw_adjoint_configure(sw_structure* structure, const sw_step_settings* settings, uint64_t steps,
                                uint32_t checkpoint_interval, sw_adjoint** out);

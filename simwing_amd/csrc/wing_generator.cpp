@@ -551,6 +551,15 @@ WingParams wingPreset(const std::string& name) {
         t.tether = 2;
         return t;
     }
+    if (name == "small-stiff") {
+        // The cfg0 wing with reinforced (stiff) hinges, 1 N m of bending
+        // rigidity: under the flight loads its hinges stay within 0.75 rad
+        // of rest and the motion settles, so last-bit differences stay at
+        // roundoff over hundreds of steps (the long-horizon parity wing).
+        WingParams t = wingPreset("small");
+        t.bendingRigidity = 1.0;
+        return t;
+    }
     if (name == "variant10k") {
         p.seed = 0x5EED0003ULL;
         p.spanNodes = 64;

@@ -118,6 +118,42 @@ def test_adjoint_leaves_structure_and_repeats():
     fresh.close()
 
 
+def test_adjoint_starts_from_the_current_state_and_restores_face_pressure():
+    """A run starts from the state the structure holds when it starts (here
+    after a step taken since configure) and puts that state back, face
+    pressure included, after running at other pressures."""
+    d, s, adj, pressure, rest = setup("tiny", 3, 2)
+    settings = sw.flight_step_settings()
+    _, p2, l2 = flight_case("tiny")
+    target = np.asarray(d.arrays["node_position"]).reshape(-1, 3) + 0.01
+    s.set_external_forces(l2)
+    s.step(settings)
+    out = adj.run(target, pressure=1.5 * pressure, gradients=False)
+
+    expected = sw.Structure(flight_case("tiny")[0])
+    expected.set_face_pressure(p2)
+    expected.set_external_forces(l2)
+    expected.step(settings)
+    expected.set_face_pressure(1.5 * p2)
+    for _ in range(3):
+        expected.set_external_forces(l2)
+        expected.step(settings)
+    assert np.array_equal(out["final_position"].reshape(-1), expected.node_states()[0].reshape(-1))
+
+    # Afterwards the structure steps on at its own face pressure.
+    after = sw.Structure(flight_case("tiny")[0])
+    after.set_face_pressure(p2)
+    for t in (s, after):
+        t.set_external_forces(l2)
+        t.step(settings)
+    after.set_external_forces(l2)
+    after.step(settings)
+    assert np.array_equal(s.node_states()[0], after.node_states()[0])
+    assert s.accepted_step_count() == 2
+    for t in (adj, s, expected, after):
+        t.close()
+
+
 def test_adjoint_sequential_and_pipelined_reverse_passes_agree():
     """Three checkpoint intervals, so the pipelined reverse pass reuses its
     first trace buffer. 200 sweep workgroups fit beside the forward grid
@@ -187,5 +223,24 @@ def test_autograd_harness_tiny():
     assert np.array_equal(lc.grad.numpy(), 2.0 * r["grad_log_compliance"])
     assert np.array_equal(rl.grad.numpy(), 2.0 * r["grad_rest_length"])
     assert np.array_equal(pr.grad.numpy(), 2.0 * r["grad_pressure"])
+    adj.close()
+    s.close()
+
+
+def test_autograd_harness_with_device_tensors():
+    """Parameter tensors on the GPU get their gradients on the GPU."""
+    d, s, adj, pressure, rest = setup("tiny", 3, 2)
+    target = np.asarray(d.arrays["node_position"]).reshape(-1, 3) + 0.02
+    dev = torch.device("cuda:0")
+    lc = torch.zeros(adj.membrane_count, dtype=torch.float64, device=dev, requires_grad=True)
+    rl = torch.tensor(rest, dtype=torch.float64, device=dev, requires_grad=True)
+    pr = torch.tensor(pressure, dtype=torch.float64, requires_grad=True)  # a CPU leaf beside them
+    loss = shape_loss(lc, rl, pr, adj, target)
+    loss.backward()
+    r = adj.run(target, np.zeros(adj.membrane_count), rest, pressure)
+    assert loss.device == dev and lc.grad.device == dev and rl.grad.device == dev and pr.grad.device.type == "cpu"
+    assert np.array_equal(lc.grad.cpu().numpy(), r["grad_log_compliance"])
+    assert np.array_equal(rl.grad.cpu().numpy(), r["grad_rest_length"])
+    assert np.array_equal(pr.grad.numpy(), r["grad_pressure"])
     adj.close()
     s.close()

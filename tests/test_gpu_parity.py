@@ -261,6 +261,40 @@ def export_state(structure: sw.Structure, order: np.ndarray) -> dict:
     return state
 
 
+def test_long_horizon_matches_reference(golden):
+    """The cfg0 wing with stiff hinges (preset small-stiff, which stays near
+    rest under the flight loads): 240 flight steps on the device, every state
+    array within 1e-10 of the reference's own CPU run at steps 1, 120 and 240
+    (strided samples, relative to the array's max norm), and the line
+    multipliers and payload in full."""
+    reference = golden("small-stiff.digest.swfx")
+    d, pressure, loads = flight_case("small-stiff")
+    order, _ = sw.definition_colouring(d, 2)
+    structure = sw.Structure(d)
+    structure.set_face_pressure(pressure)
+    st = sw.flight_step_settings()
+    for step in range(1, 241):
+        structure.set_external_forces(loads)
+        structure.step(st)
+        if step not in (1, 120, 240):
+            continue
+        got = export_state(structure, order.astype(np.int64))
+        report = {}
+        for key in ("position", "previous", "velocity", "membrane_residual", "membrane_lambda", "dihedral_lambda",
+                    "constraint_lambda"):
+            source = "dihedral_lambda_colour" if key == "dihedral_lambda" else key
+            width = 3 if key in ("position", "previous", "velocity", "membrane_lambda") else 1
+            rows = got[source].reshape(-1, width)[reference[f"step{step}/{key}/rows"].astype(np.int64)].reshape(-1)
+            scale = float(reference[f"step{step}/{key}/maxabs"][0])
+            report[key] = float(np.abs(rows - reference[f"step{step}/{key}/samples"]).max()) / max(scale, 1e-300)
+        for key in ("segment_lambda", "payload"):
+            report[key] = relative_error(got[key], reference[f"step{step}/{key}"])
+        print(f"small-stiff step {step} vs reference:", {k: f"{e:.1e}" for k, e in report.items()}, flush=True)
+        for key, err in report.items():
+            assert err <= TOL, (step, key, err)
+    structure.close()
+
+
 def test_export_scale(golden):
     """The ~70k-triangle wing (43k nodes, 300 line segments, rigid pilot, 8
     contact spheres), 240 flight steps: bit-equal to the portable-math oracle

@@ -61,3 +61,31 @@ def test_oracle_reproduces_reference_export_digests(golden, steps):
         body.step(settings)
         if step in steps:
             check_step(reference, body.state(), hinge_order.astype(np.int64), step)
+
+
+def test_oracle_reproduces_reference_stiff_wing_over_240_steps(golden):
+    """The cfg0 wing with stiff hinges (preset small-stiff) over 240 flight
+    steps: the oracle in C-library mode reproduces the reference's digests at
+    steps 1, 120 and 240, and in portable-math mode (the device's
+    transcendentals) it stays within 1e-12 of the reference, because this
+    wing stays near rest and last-bit differences do not grow."""
+    reference = golden("small-stiff.digest.swfx")
+    definition, pressure, loads = flight_case("small-stiff")
+    hinge_order, _ = sw.definition_colouring(definition, 2)
+    assert np.array_equal(digest(hinge_order.astype(np.uint64)), reference["dihedral_colour_order_digest"])
+    bodies = [orc.OracleBody(definition) for _ in range(2)]
+    bodies[1].use_portable_math(True)
+    settings = sw.flight_step_settings()
+    for step in range(1, 241):
+        for body in bodies:
+            body.set_face_pressure(pressure)
+            body.set_external_forces(loads)
+            body.step(settings)
+        if step in (1, 120, 240):
+            check_step(reference, bodies[0].state(), hinge_order.astype(np.int64), step)
+            state = bodies[1].state()
+            for key in ("position", "velocity"):
+                rows = state[key].reshape(-1, 3)[reference[f"step{step}/{key}/rows"].astype(np.int64)].reshape(-1)
+                err = float(np.abs(rows - reference[f"step{step}/{key}/samples"]).max())
+                err /= float(reference[f"step{step}/{key}/maxabs"][0])
+                assert err <= 1e-12, (step, key, err)
