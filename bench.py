@@ -92,25 +92,35 @@ def launch_candidates(cus: int) -> list[tuple[int, int]]:
     return shapes
 
 
-def tune(structure, loads, settings, cus: int) -> tuple[tuple[int, int], dict]:
-    """Launch-shape autotuner: one timed step per candidate shape, keep the
-    fastest. The state trajectory does not depend on the shape."""
+def tune(structure, loads, settings, cus: int) -> tuple[tuple[int, int, bool], dict]:
+    """Launch autotuner: one timed step per candidate, keep the fastest. A
+    candidate is a workgroup count (which sets each colour's block of items
+    per workgroup: the colour's size over the workgroups), a tile of threads
+    per workgroup (items per lane within the block) and, where every
+    participant fits on one XCD, the one-XCD synchronisation mode (no L2
+    write-back per phase) or the device-wide one. The integer tables (colour
+    orders, levels) do not depend on the candidate, nor does the trajectory."""
     results = {}
     for shape in launch_candidates(cus):
         try:
             structure.set_launch(*shape)
         except Exception:  # shape outside the device's limits
             continue
-        structure.set_kernel_timing(True)
-        structure.set_external_forces(loads)
-        structure.step_many(1, settings)
-        ms, launches = structure.kernel_time()
-        structure.set_kernel_timing(False)
-        results[f"{shape[0]}x{shape[1]}"] = ms / max(launches, 1)
+        for xcd in (True, False):
+            if structure.set_xcd_local(xcd) != xcd:
+                continue  # the one-XCD mode is not available for this shape
+            structure.set_kernel_timing(True)
+            structure.set_external_forces(loads)
+            structure.step_many(1, settings)
+            ms, launches = structure.kernel_time()
+            structure.set_kernel_timing(False)
+            results[f"{shape[0]}x{shape[1]}" + ("/xcd" if xcd else "")] = ms / max(launches, 1)
     best = min(results, key=results.get)
-    wg, threads = (int(v) for v in best.split("x"))
+    xcd = best.endswith("/xcd")
+    wg, threads = (int(v) for v in best.split("/")[0].split("x"))
     structure.set_launch(wg, threads)
-    return (wg, threads), results
+    structure.set_xcd_local(xcd)
+    return (wg, threads, xcd), results
 
 
 def measured_traffic(workload: str, launch: str) -> int | None:
@@ -247,7 +257,7 @@ def batched_leg(count: int, steps: int, seed: int = 0x5EED0002) -> dict:
         start = time.perf_counter()
         s.step_many(solo_steps, settings)
         solo_s += (time.perf_counter() - start) / solo_steps
-    batch.set_launch(8, 256)
+    batch.set_launch(64, 256)  # the default plan again after the solo runs
     nodes = [c[0].node_count for c in cases]
     batched_rate = count * steps / batched_s
     solo_rate = count / solo_s
@@ -419,7 +429,7 @@ def main() -> int:
     parser.add_argument("--no-adjoint", action="store_true")
     parser.add_argument("--adjoint-steps", type=int, default=120)
     parser.add_argument("--adjoint-wings", type=int, default=32)
-    parser.add_argument("--launch", default=None, help="fixed launch shape WGxTHREADS")
+    parser.add_argument("--launch", default=None, help="fixed launch shape WGxTHREADS, /xcd for the one-XCD mode")
     parser.add_argument("--plain-launch", action="store_true",
                         help="plain instead of cooperative launches (for profilers that do not follow the latter)")
     args = parser.parse_args()
@@ -446,11 +456,11 @@ def main() -> int:
     structure.step_many(1, settings)  # first launch: code object load
     tuning = {}
     if args.launch:
-        wg, threads = (int(v) for v in args.launch.split("x"))
+        wg, threads = (int(v) for v in args.launch.split("/")[0].split("x"))
         structure.set_launch(wg, threads)
-        shape = (wg, threads)
+        shape = (wg, threads, structure.set_xcd_local(args.launch.endswith("/xcd")))
     elif args.no_tune:
-        shape = structure.launch()
+        shape = (*structure.launch(), structure.set_xcd_local(True))  # the default request
     else:
         shape, tuning = tune(structure, loads, settings, cus)
     print(f"# launch shape {shape} tuning(ms/launch)={tuning}", file=sys.stderr, flush=True)
@@ -506,7 +516,8 @@ def main() -> int:
                         f"{counts['segments']} line segments + rigid pilot, {counts['sphere_pairs']} sphere pairs; "
                         f"dt 1/120 s, {args.iterations} iterations, {substeps} substep",
             "projections_per_step": proj_step,
-            "launch": f"{shape[0]}x{shape[1]}",
+            "launch": f"{shape[0]}x{shape[1]}" + ("/xcd" if shape[2] else ""),
+            "tuning_ms_per_launch": tuning,
         },
         "roofline": {
             "bound": "hbm",
@@ -514,7 +525,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.workload, f"{shape[0]}x{shape[1]}"),
+            "traffic": measured_traffic(args.workload, f"{shape[0]}x{shape[1]}" + ("/xcd" if shape[2] else "")),
             "kernel": "substepKernel",
             "kernel_ms": avg_kernel_ms,
             "algorithmic_bytes": algorithmic,

@@ -1079,7 +1079,7 @@ sw_status sw_batch_create(const sw_definition* const* definitions, size_t count,
             devices.push_back(s->wing.get());
         }
         b->device = std::make_unique<swx::DeviceBatch>(devices, device);
-        b->device->plan(8, 256);
+        b->device->plan(64, 256);
         *out = b.release();
     });
 }

@@ -1576,6 +1576,23 @@ __device__ __forceinline__ bool dampLines(const WingView& v, const Store& N, Pay
     return true;
 }
 
+// Wavefront reductions over the 64 lanes of a wave: a butterfly of
+// cross-lane exchanges (__shfl_xor, lowered to DPP / ds_swizzle), no LDS.
+// Every lane ends with the same value (a + b == b + a exactly), and the
+// fixed exchange pattern makes sums reproducible run to run.
+__device__ __forceinline__ double waveSum(double x) {
+    for (int offset = 32; offset > 0; offset >>= 1) x += __shfl_xor(x, offset, 64);
+    return x;
+}
+__device__ __forceinline__ double waveMax(double x) {
+    for (int offset = 32; offset > 0; offset >>= 1) x = stdmax(x, __shfl_xor(x, offset, 64));
+    return x;
+}
+__device__ __forceinline__ int waveSumInt(int x) {
+    for (int offset = 32; offset > 0; offset >>= 1) x += __shfl_xor(x, offset, 64);
+    return x;
+}
+
 __device__ __forceinline__ void atomicMaxNonNegative(double* address, double value) {
     // Non-negative doubles order like their bit patterns.
     atomicMax(reinterpret_cast<unsigned long long*>(address), static_cast<unsigned long long>(__double_as_longlong(value)));
@@ -2051,14 +2068,10 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
             const double compliant = v.slam[k] / (v.sstiff[k] * a.dt * a.dt);
             worst = stdmax(worst, fabs(stretch - compliant));
         }
-        double* scratch = shared ? lds + 4 * n : lds;  // kMaxThreads doubles
-        scratch[threadIdx.x] = worst;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) scratch[threadIdx.x] = stdmax(scratch[threadIdx.x], scratch[threadIdx.x + s]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) atomicMaxNonNegative(&payload.maxResidual, scratch[0]);
+        // Wave maxima by lane exchanges, then one atomic per wave (the max
+        // is order-free, so the result does not depend on arrival order).
+        worst = waveMax(worst);
+        if ((threadIdx.x & 63) == 0) atomicMaxNonNegative(&payload.maxResidual, worst);
     }
     gridSync<kLocal>(v, workers);
     if (kGeneral && v.preserveMomentum && block == 0 && threadIdx.x < 64) {
@@ -2171,7 +2184,7 @@ struct Partial {
 };
 
 __global__ void __launch_bounds__(256) diagnosticsKernel(WingView v, Partial* out) {
-    __shared__ Partial block[256];
+    __shared__ Partial block[4];  // one per wave
     Partial p{};
     p.finite = 1;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2212,12 +2225,28 @@ __global__ void __launch_bounds__(256) diagnosticsKernel(WingView v, Partial* ou
         p.maxResidual = stdmax(p.maxResidual, fabs(v.mres[k]));
         p.finite &= (fin3(e.strain) && isfinite(v.mres[k])) ? 1 : 0;
     }
-    block[threadIdx.x] = p;
+    // Each wave reduces its lanes by exchanges; the waves' results are then
+    // combined in wave order through LDS.
+    p.mass = waveSum(p.mass);
+    p.wx = waveSum(p.wx);
+    p.wy = waveSum(p.wy);
+    p.wz = waveSum(p.wz);
+    p.mx = waveSum(p.mx);
+    p.my = waveSum(p.my);
+    p.mz = waveSum(p.mz);
+    p.kinetic = waveSum(p.kinetic);
+    p.maxDistance = waveMax(p.maxDistance);
+    p.maxCable = waveMax(p.maxCable);
+    p.maxStrain = waveMax(p.maxStrain);
+    p.maxResidual = waveMax(p.maxResidual);
+    p.finite = __all(p.finite) ? 1 : 0;
+    p.dynamicNodes = waveSumInt(p.dynamicNodes);
+    if ((threadIdx.x & 63) == 0) block[threadIdx.x >> 6] = p;
     __syncthreads();
     if (threadIdx.x == 0) {
         Partial r{};
         r.finite = 1;
-        for (int t = 0; t < static_cast<int>(blockDim.x); ++t) {
+        for (int t = 0; t < static_cast<int>(blockDim.x + 63) / 64; ++t) {
             const Partial& s = block[t];
             r.mass += s.mass;
             r.wx += s.wx;
@@ -3798,31 +3827,33 @@ void DeviceBatch::plan(int cap, int threads) {
     const int count = static_cast<int>(wings_.size());
     if (count > cus) throw std::invalid_argument("a batch holds at most one wing per compute unit");
     cap = std::max(cap, 1);
-    // One LDS-resident workgroup per small wing; the remaining compute units
-    // are shared by the large wings in proportion to their items.
+    // One LDS-resident workgroup per small wing. The large wings first get
+    // a floor of three workgroups (workers plus the chain workgroup); the
+    // remaining compute units are then dealt out one at a time, longest-
+    // processing-time first: each goes to the large wing with the most items
+    // per workgroup so far (ties to the lower wing index), until none are
+    // left or every large wing is at the cap. The batched step lasts as long
+    // as its slowest wing, and a wing's phases take longer the more items
+    // each of its threads holds, so this levels the wings' phase loads.
     blocks_.assign(count, 1);
     std::vector<int> large;
-    double largeItems = 0.0;
     for (int w = 0; w < count; ++w)
-        if (!wings_[w]->fitsInLds() || cap == 1) {
-            large.push_back(w);
-            largeItems += static_cast<double>(wings_[w]->itemCount());
-        }
+        if (!wings_[w]->fitsInLds() || cap == 1) large.push_back(w);
     const int floor = std::min(3, cap);
     int spare = cus - count;
     for (int w : large) {
-        const int extra = std::min(floor, cap) - 1;
-        blocks_[w] += std::max(0, std::min(extra, spare));
-        spare -= std::max(0, std::min(extra, spare));
+        const int add = std::max(0, std::min(floor - 1, spare));
+        blocks_[w] += add;
+        spare -= add;
     }
-    if (spare > 0 && largeItems > 0.0) {
-        const int pool = spare;
-        for (int w : large) {
-            const int want = static_cast<int>(pool * (wings_[w]->itemCount() / largeItems));
-            const int add = std::max(0, std::min({want, cap - blocks_[w], spare}));
-            blocks_[w] += add;
-            spare -= add;
-        }
+    const auto load = [&](int w) { return static_cast<double>(wings_[w]->itemCount()) / blocks_[w]; };
+    while (spare > 0) {
+        int best = -1;
+        for (int w : large)
+            if (blocks_[w] < cap && (best < 0 || load(w) > load(best))) best = w;
+        if (best < 0) break;
+        ++blocks_[best];
+        --spare;
     }
     lds_ = 0;
     std::vector<WingView> views(count);
