@@ -452,6 +452,14 @@ private:
     double forwardMs_ = 0.0, reverseMs_ = 0.0, replayMs_ = 0.0, jacobianMs_ = 0.0, sweepMs_ = 0.0;
     hipEvent_t events_[3] = {nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> marks_;
+    // The wings' Jacobian passes of one chunk are independent launches: they
+    // run on kJacobianStreams streams of their own (default priority), wing w
+    // on stream w mod kJacobianStreams, between a fork event on the batch
+    // stream and one join event per stream that the batch stream waits on.
+    static constexpr int kJacobianStreams = 4;
+    hipStream_t jacobianStreams_[kJacobianStreams] = {};
+    hipEvent_t jacobianJoin_[kJacobianStreams] = {};
+    hipEvent_t jacobianFork_ = nullptr;
 
 public:
     // The reverse pass split: recorded replays, Jacobian passes, sweeps.

@@ -322,7 +322,10 @@ GeneratedWing generateWing(const WingParams& P) {
     pilot.id = 1;
     pilot.name = "pilot";
     pilot.mass = P.pilotMass;
-    pilot.centerOfMass = V3{0.35 * P.rootChord, 0.0, -P.lineLength - 0.35};
+    // Lines `lineRestScale` times their nominal length: the pilot hangs that
+    // much lower (or higher), so every line starts at its rest length and no
+    // wing is released with its lines pre-stretched.
+    pilot.centerOfMass = V3{0.35 * P.rootChord, 0.0, -P.lineLength - 0.35 - (P.lineRestScale - 1.0) * P.lineLength};
     pilot.principalInertia = P.pilotInertia;
     // Tethered wing: no rigid pilot; the lines end at two fixed riser
     // junctions where the harness points would be (the scene then carries
@@ -470,8 +473,7 @@ GeneratedWing generateWing(const WingParams& P) {
             line.startAttachment = lineNodes[l][hop];
             line.endAttachment = lineNodes[l][hop + 1];
             line.materialId = 1;
-            line.restLength =
-                P.lineRestScale * lengthv(sub(linePoints[l][hop + 1], linePoints[l][hop]));
+            line.restLength = lengthv(sub(linePoints[l][hop + 1], linePoints[l][hop]));
             line.role = LineRole::Suspension;
             S.lines.push_back(line);
         }

@@ -194,6 +194,17 @@ def test_device_closed_forms():
     s.step(settings(dt=0.01, iterations=1))
     x, _, _ = s.node_states()
     assert math.isclose(x[1, 0], 0.05, rel_tol=1e-15) and np.array_equal(x[2], [0, 0.3, 0])
+    # A compliant sphere (c > 0): the node moves out by (R - L) / (1 + c / dt^2),
+    # and a second iteration leaves it at that fixed point.
+    dt, c = 0.01, 2.0e-4
+    expected = 0.02 + 0.03 / (1.0 + c / dt ** 2)
+    for iterations in (1, 2):
+        d = definition([[0, 0, 0], [0.02, 0, 0], [0, 0.3, 0]], [1.0, 1.0, 1.0], spheres=[(0, 0.05, c, [1, 2])])
+        s = sw.Structure(d)
+        s.step(settings(dt=dt, iterations=iterations))
+        x, _, _ = s.node_states()
+        assert math.isclose(x[1, 0], expected, rel_tol=1e-14) and x[1, 1] == 0.0, (iterations, x[1])
+        s.close()
 
 
 def test_checkpoint_restore_replays_bit_identically():
@@ -340,3 +351,27 @@ def test_export_scale(golden):
         assert err <= TOL
     for key in ("segment_lambda", "payload"):
         assert relative_error(got[key], reference[f"step1/{key}"]) <= TOL, key
+
+
+def test_decoded_scene_stream_steps_like_the_reference(golden):
+    """The reference's own scene-v2 stream of the tiny wing
+    (tests/golden/tiny.scene.bin, written by writeScene), decoded by
+    sw_scene_read, assembled and stepped on the device for 20 flight steps
+    with the generated wing's face pressure: within 1e-10 of the reference's
+    run of the same scene at steps 1, 5 and 20."""
+    stream = (GOLDEN / "tiny.scene.bin").read_bytes()
+    assembly = sw.read_scene(stream).assemble()
+    d = assembly.definition
+    generated = sw.generate_scene("tiny").assemble().definition
+    for key, value in generated.arrays.items():
+        assert np.array_equal(d.arrays[key], value), f"decoded scene assembles differently: {key}"
+    structure = sw.Structure(d)
+    structure.set_face_pressure(sw.generate_scene("tiny").face_pressure())
+    loads = assembly.flight_loads()
+    st = sw.flight_step_settings()
+    for step in range(1, 21):
+        structure.set_external_forces(loads)
+        structure.step(st)
+        if step in (1, 5, 20):
+            compare(device_state(structure), reference_state(golden, "tiny", step), f"decoded tiny step {step}")
+    structure.close()

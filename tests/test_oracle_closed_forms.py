@@ -125,3 +125,23 @@ def test_contact_sphere_pushes_node_to_radius():
     assert np.array_equal(x[0], [0, 0, 0])
     assert math.isclose(x[1, 0], 0.05, rel_tol=1e-15) and x[1, 1] == 0.0
     assert np.array_equal(x[2], [0, 0.3, 0])
+
+
+def test_compliant_contact_sphere_closed_form():
+    # Compliance c > 0: one projection moves an intruding node (w = 1) out
+    # by dl = (R - L) / (1 + c / dt^2) along the centre line, dl becoming
+    # the multiplier; a second iteration finds the compliant constraint at
+    # its fixed point (-gap - alpha * lambda = 0 up to roundoff) and leaves
+    # the node where it is.
+    dt, c, radius, start = 0.01, 2.0e-4, 0.05, 0.02
+    alpha = c / dt ** 2
+    expected = start + (radius - start) / (1.0 + alpha)
+    for iterations in (1, 2):
+        d = definition([[0, 0, 0], [start, 0, 0], [0, 0.3, 0]], [1.0, 1.0, 1.0],
+                       spheres=[(0, radius, c, [1, 2])])
+        body = orc.OracleBody(d)
+        body.step(settings(dt=dt, iterations=iterations))
+        x = body.state()["position"].reshape(-1, 3)
+        assert math.isclose(x[1, 0], expected, rel_tol=1e-14), (iterations, x[1, 0], expected)
+        assert x[1, 1] == 0.0 and x[1, 2] == 0.0
+        assert np.array_equal(x[0], [0, 0, 0]) and np.array_equal(x[2], [0, 0.3, 0])
