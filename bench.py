@@ -85,7 +85,7 @@ def bytes_per_substep(c: dict, iterations: int, membrane_damped: bool) -> int:
 
 def launch_candidates(cus: int) -> list[tuple[int, int]]:
     shapes = []
-    for wg in (cus // 16, cus // 8, cus // 4, cus // 2):
+    for wg in (cus // 16, cus // 8, cus // 4, cus // 2, cus):
         for threads in (128, 256):
             if wg >= 1:
                 shapes.append((wg, threads))

@@ -177,14 +177,18 @@ __device__ __forceinline__ bool syncTimedOut(const WingView& v, unsigned spins) 
 // leaving make every workgroup's earlier global writes visible to every
 // workgroup's later reads; the counter and generation words are touched only
 // with relaxed atomics in between.
-// One-XCD release half: every wave waits until its own earlier global
-// stores are complete in the XCD's L2. Called by all threads before the
-// workgroup barrier that precedes a publication.
-// kLocal: the launch is a one-XCD launch (compiled per kernel variant).
-template <bool kLocal = false>
+// Release half of a publication without an L2 write-back: every wave waits
+// until its own earlier global stores are complete (in the XCD's L2 for a
+// one-XCD launch; at the device's coherence point for the node stores of a
+// device-wide substep, which are device-scope stores, see NodeRefs). Called
+// by all threads before the workgroup barrier that precedes a publication.
+// kLocal: the launch is a one-XCD launch (compiled per kernel variant);
+// kCoherent: a device-wide substep whose cross-workgroup stores are
+// device-scope.
+template <bool kLocal = false, bool kCoherent = false>
 __device__ __forceinline__ void drainStores(const WingView& v) {
     (void)v;
-    if constexpr (kLocal) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if constexpr (kLocal || kCoherent) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 // Acquire after a wait: the agent-scope acquire (buffer_inv sc1: this
@@ -298,7 +302,18 @@ struct NodeRefs {
         return d3{hangLocal[3 * h], hangLocal[3 * h + 1], hangLocal[3 * h + 2]};
     }
     __device__ __forceinline__ double invMass(int i) const { return w[i]; }
+    // coherent: nodes in HBM under device-wide point-to-point publication
+    // (publishCounter with kCoherent): stores are device-scope relaxed atomic
+    // stores (global_store sc1), complete at the device's coherence point
+    // once drained, so no L2 write-back is needed before a publication.
+    bool coherent = false;
     __device__ __forceinline__ void put(int i, d3 p) const {
+        if (coherent) {
+            __hip_atomic_store(x + i, p.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(y + i, p.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(z + i, p.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
         x[i] = p.x;
         y[i] = p.y;
         z[i] = p.z;
@@ -844,21 +859,23 @@ __device__ __forceinline__ bool rotationIncrementDeferred(d3 v, quat& out) {
 __device__ __forceinline__ bool refreshInverseInertia(PayloadDev& def, const quat& q) {
     m3 world, inv;
     if (!worldInertia(q, def.inertia, world) || !inverseM(world, inv)) return false;
-    for (int k = 0; k < 9; ++k) def.iinv[k] = inv.a[k];
+    for (int k = 0; k < 9; ++k) __hip_atomic_store(&def.iinv[k], inv.a[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // see storePayload
     return true;
 }
 
 __device__ __forceinline__ Payload loadPayload(const PayloadDev& p) {
     return Payload{d3{p.com[0], p.com[1], p.com[2]}, quat{p.q[0], p.q[1], p.q[2], p.q[3]}};
 }
+// Device-scope stores: the payload pose is read by other workgroups after
+// a publication without an L2 write-back (publishCounter, kCoherent).
 __device__ __forceinline__ void storePayload(PayloadDev& p, const Payload& s) {
-    p.com[0] = s.com.x;
-    p.com[1] = s.com.y;
-    p.com[2] = s.com.z;
-    p.q[0] = s.q.w;
-    p.q[1] = s.q.x;
-    p.q[2] = s.q.y;
-    p.q[3] = s.q.z;
+    __hip_atomic_store(&p.com[0], s.com.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p.com[1], s.com.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p.com[2], s.com.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p.q[0], s.q.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p.q[1], s.q.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p.q[2], s.q.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p.q[3], s.q.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ m3 loadInverseInertia(const PayloadDev& p) {
     m3 r;
@@ -1099,7 +1116,7 @@ __device__ __forceinline__ void runChain(const WingView& v, const NodeRefs& N, c
                 } else {
                     iinv = inv;
                     iinvFinite = finiteM(iinv);
-                    for (int c = 0; c < 9; ++c) payload.iinv[c] = inv.a[c];
+                    for (int c = 0; c < 9; ++c) __hip_atomic_store(&payload.iinv[c], inv.a[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
         }
@@ -1108,9 +1125,7 @@ __device__ __forceinline__ void runChain(const WingView& v, const NodeRefs& N, c
     __syncthreads();
     for (int s = threadIdx.x; s < slots; s += blockDim.x) {
         const int node = v.runNodes[P.slotBegin + s];
-        N.x[node] = chainX[s];
-        N.y[node] = chainY[s];
-        N.z[node] = chainZ[s];
+        N.put(node, d3{chainX[s], chainY[s], chainZ[s]});
     }
     for (int j = threadIdx.x; j < items; j += blockDim.x) v.slam[P.begin + j] = chainLambda[j];
 }
@@ -1205,13 +1220,23 @@ __device__ __forceinline__ void prefetchedPhase(const WingView& v, const StagedP
 
 // Publishes `value` on a progress word after releasing this workgroup's
 // earlier writes (call after a workgroup barrier).
-// One XCD: every wave drained its stores before that barrier
-// (drainStores), so only the progress word itself is written.
-template <bool kLocal = false>
+// Device-wide, general case: an agent-scope release (it writes the XCD's
+// dirty L2 lines back, buffer_wbl2, so that other XCDs read them).
+// One XCD (kLocal), or a device-wide substep (kCoherent): every wave drained
+// its stores before that barrier (drainStores), so only the progress word
+// itself is written. For kCoherent the only values other workgroups read
+// during the iterations are node positions and the payload pose, and those
+// are written with device-scope stores (NodeRefs::put, storePayload), which
+// are complete at the device's coherence point once drained; the readers'
+// acquire (acquireAfterWait) drops their stale cached lines. Each item's
+// multipliers are read and written by one thread only. This saves the
+// per-phase L2 write-back (2.1 ms of the export wing's substep, measured).
+template <bool kLocal = false, bool kCoherent = false>
 __device__ __forceinline__ void publishCounter(const WingView& v, unsigned* counter, unsigned value) {
     (void)v;
     if (threadIdx.x == 0) {
-        if constexpr (!kLocal) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if constexpr (kLocal || kCoherent) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_store(counter, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -1248,9 +1273,9 @@ __device__ __forceinline__ void pipelinedRun(const WingView& v, const StagedProg
         Item next{};
         if (nfirst < ne) next = load(nfirst);
         if (b < e) {
-            drainStores<kLocal>(v);
+            drainStores<kLocal, true>(v);
             __syncthreads();
-            publishCounter<kLocal>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
+            publishCounter<kLocal, true>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
         }
         b = nb, e = ne, first = nfirst, cur = next;
     }
@@ -1727,6 +1752,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         __syncthreads();
     } else {
         N = NodeRefs{v.px, v.py, v.pz, v.w, v.hangLocal};
+        N.coherent = !kLocal;  // device-wide publication without L2 write-back
     }
     PayloadDev& payload = *v.payload;
     const TraceLayout layout = traceLayout(v, a.iterations);
@@ -1871,9 +1897,9 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     const int step = it * m + ph;
                     waitStagedDeps(v, S, ph, step);
                     runChain(v, N, P, payload, a, rec ? rec + layout.segBase : nullptr);
-                    drainStores<kLocal>(v);
+                    drainStores<kLocal, true>(v);
                     __syncthreads();
-                    publishCounter<kLocal>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
+                    publishCounter<kLocal, true>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
                 }
             }
         }
@@ -1970,9 +1996,9 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     waitStagedDeps(v, S, ph, step);
                     break;
                 }
-                drainStores<kLocal>(v);
+                drainStores<kLocal, true>(v);
                 __syncthreads();
-                publishCounter<kLocal>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
+                publishCounter<kLocal, true>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
             }
         }
         gridSync<kLocal>(v, blocks);
