@@ -2149,8 +2149,18 @@ __global__ void __launch_bounds__(kMaxThreads) batchSubstepKernel(const WingView
                                                                    const int* blockWing, const int* blockLocal,
                                                                    double* const* traces) {
     const int wing = blockWing[blockIdx.x];
-    const WingView& v = views[wing];
-    substepBody<2, false, kGeneral>(v, args[wing], blockLocal[blockIdx.x], v.launchBlocks, traces ? traces[wing] : nullptr);
+    // The wing's view in LDS: the substep reads its fields after every
+    // acquire, which would otherwise send each read back to global memory.
+    __shared__ WingView sView;
+    static_assert(sizeof(WingView) % 8 == 0, "the view copies as 8-byte words");
+    {
+        const auto* from = reinterpret_cast<const unsigned long long*>(&views[wing]);
+        auto* to = reinterpret_cast<unsigned long long*>(&sView);
+        for (int k = threadIdx.x; k < static_cast<int>(sizeof(WingView) / 8); k += blockDim.x) to[k] = from[k];
+    }
+    __syncthreads();
+    substepBody<2, false, kGeneral>(sView, args[wing], blockLocal[blockIdx.x], sView.launchBlocks,
+                                    traces ? traces[wing] : nullptr);
 }
 
 // The substep kernel variants: nodes in LDS or HBM, a one-XCD launch, and
