@@ -297,15 +297,22 @@ __device__ __forceinline__ int xcdParticipant(const WingView& v, int participant
 struct NodeRefs {
     double *x, *y, *z, *w;
     const double* hangLocal;  // harness points in the payload body frame (3 per point)
-    __device__ __forceinline__ d3 get(int i) const { return d3{x[i], y[i], z[i]}; }
+    __device__ __forceinline__ d3 get(int i) const {
+        if (coherent)
+            return d3{__hip_atomic_load(x + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), __hip_atomic_load(y + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                      __hip_atomic_load(z + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+        return d3{x[i], y[i], z[i]};
+    }
     __device__ __forceinline__ d3 hang(int h) const {
         return d3{hangLocal[3 * h], hangLocal[3 * h + 1], hangLocal[3 * h + 2]};
     }
     __device__ __forceinline__ double invMass(int i) const { return w[i]; }
     // coherent: nodes in HBM under device-wide point-to-point publication
-    // (publishCounter with kCoherent): stores are device-scope relaxed atomic
-    // stores (global_store sc1), complete at the device's coherence point
-    // once drained, so no L2 write-back is needed before a publication.
+    // (publishCounter with kCoherent): loads and stores are device-scope
+    // relaxed atomics (global_load / global_store sc1). The stores are
+    // complete at the device's coherence point once drained, so no L2
+    // write-back is needed before a publication, and the loads read that
+    // point, so no cache invalidation is needed after a wait.
     bool coherent = false;
     __device__ __forceinline__ void put(int i, d3 p) const {
         if (coherent) {
@@ -863,8 +870,12 @@ __device__ __forceinline__ bool refreshInverseInertia(PayloadDev& def, const qua
     return true;
 }
 
+// Device-scope loads (see storePayload).
 __device__ __forceinline__ Payload loadPayload(const PayloadDev& p) {
-    return Payload{d3{p.com[0], p.com[1], p.com[2]}, quat{p.q[0], p.q[1], p.q[2], p.q[3]}};
+    return Payload{d3{__hip_atomic_load(&p.com[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), __hip_atomic_load(&p.com[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                      __hip_atomic_load(&p.com[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)},
+                   quat{__hip_atomic_load(&p.q[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), __hip_atomic_load(&p.q[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                        __hip_atomic_load(&p.q[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), __hip_atomic_load(&p.q[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)}};
 }
 // Device-scope stores: the payload pose is read by other workgroups after
 // a publication without an L2 write-back (publishCounter, kCoherent).
@@ -879,7 +890,7 @@ __device__ __forceinline__ void storePayload(PayloadDev& p, const Payload& s) {
 }
 __device__ __forceinline__ m3 loadInverseInertia(const PayloadDev& p) {
     m3 r;
-    for (int k = 0; k < 9; ++k) r.a[k] = p.iinv[k];
+    for (int k = 0; k < 9; ++k) r.a[k] = __hip_atomic_load(&p.iinv[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return r;
 }
 
@@ -1050,9 +1061,10 @@ __device__ __forceinline__ void runChain(const WingView& v, const NodeRefs& N, c
     const int items = P.end - P.begin;
     for (int s = threadIdx.x; s < slots; s += blockDim.x) {
         const int node = v.runNodes[P.slotBegin + s];
-        chainX[s] = N.x[node];
-        chainY[s] = N.y[node];
-        chainZ[s] = N.z[node];
+        const d3 x = N.get(node);
+        chainX[s] = x.x;
+        chainY[s] = x.y;
+        chainZ[s] = x.z;
         chainW[s] = N.w[node];
     }
     for (int j = threadIdx.x; j < items; j += blockDim.x) {
@@ -1164,6 +1176,9 @@ struct StagedProgram {
     int* offset;
     int2* dep;
     int2* share;  // this participant's items [x, y) of every phase (forward loop only)
+    // Acquire after each wait (buffer_inv: drops stale cached lines). Off when
+    // every value read across workgroups is read with device-scope loads.
+    bool acquire = true;
 };
 
 __host__ __device__ constexpr std::size_t alignTo8(std::size_t bytes) { return (bytes + 7) & ~std::size_t{7}; }
@@ -1199,7 +1214,7 @@ __device__ __forceinline__ void waitStagedDeps(const WingView& v, const StagedPr
                 if (syncTimedOut(v, ++spins)) break;
             }
         }
-        acquireAfterWait(v);
+        if (S.acquire) acquireAfterWait(v);
     }
     __syncthreads();
 }
@@ -1866,7 +1881,8 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         const bool chainGroup = chainOnly;
         // The host sizes the dynamic LDS for this participant's program
         // (DeviceWing::buildSync); it follows the reduction scratch.
-        const StagedProgram S = stagedProgram(reinterpret_cast<char*>(lds + kMaxThreads), m);
+        StagedProgram S = stagedProgram(reinterpret_cast<char*>(lds + kMaxThreads), m);
+        S.acquire = !N.coherent;
         const int depBase = v.depOffsets[p * m];
         const int depCount = v.depOffsets[p * m + m] - depBase;
         for (int k = threadIdx.x; k < m; k += blockDim.x) {
