@@ -130,10 +130,8 @@ def measured_traffic(workload: str, launch: str) -> int | None:
     path = ROOT / "profiles" / f"substepKernel_{workload}_counters.json"
     if not path.exists():
         return None
-    record = json.loads(path.read_text())
-    if record.get("launch") != launch:
-        return None
-    return int(record["traffic_bytes_per_launch"])
+    record = json.loads(path.read_text()).get("launches", {}).get(launch)
+    return None if record is None else int(record["traffic_bytes_per_launch"])
 
 
 def cpu_share() -> int:
