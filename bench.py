@@ -476,6 +476,16 @@ def main() -> int:
     diagnostics = structure.diagnostics()
     if diagnostics["finite"] != 1:
         raise RuntimeError("non-finite state after the timed steps")
+    # The device's per-part split of a substep (StepPerformanceProfile's
+    # fields), from a few profiled steps after the timed region.
+    structure.set_profiling(True)
+    profiled = 3
+    for _ in range(profiled):
+        structure.set_external_forces(loads)
+        structure.step_many(1, settings)
+    prof = structure.profile()
+    structure.set_profiling(False)
+    device_split = {k[:-3]: prof[k] * 1e-6 / profiled for k in prof if k.endswith("_ns")}
 
     substeps = settings.substeps
     proj_step = projections_per_substep(counts, args.iterations) * substeps
@@ -527,6 +537,7 @@ def main() -> int:
             "kernel": "substepKernel",
             "kernel_ms": avg_kernel_ms,
             "algorithmic_bytes": algorithmic,
+            "device_split_ms_per_step": device_split,
         },
         "cpu_baseline": cpu,
         "batched": batched,

@@ -431,6 +431,45 @@ sw_status sw_structure_set_xcd_local(sw_structure* s, int enabled, int* active);
 sw_status sw_structure_set_kernel_timing(sw_structure* s, int enabled);
 sw_status sw_structure_kernel_time(sw_structure* s, double* milliseconds, long long* launches);
 
+/* Step profile: the soft-body fields of softwing::StepPerformanceProfile
+ * (soft_body.h:136-170; filled by SoftBody::stepCoupled when
+ * StepSettings::performanceProfile is set, soft_body.cpp:1075-1444),
+ * accumulated over the steps taken while profiling is on. The visit counts
+ * follow the reference's rules exactly (soft_body.cpp:1222-1245, 1389-1404:
+ * per substep, iterations x constraints / membranes / dihedrals / segments,
+ * 2 x cable pairs x load-path constraints, one more visit per seam stitch in
+ * the closing pass, iterations once for a wing with contact pairs). The times
+ * are the device's: soft_body_total_ns is the host's wall clock around each
+ * step call (launch to completion); each part's time is the mean workgroup's
+ * wall-clock time in that part of the substep kernel (waits included),
+ * measured with the device's 100 MHz clock, so the parts add up to about the
+ * kernel time; transaction_snapshot_ns is the remainder of the total. */
+typedef struct sw_step_profile {
+    uint64_t soft_body_total_ns;
+    uint64_t transaction_snapshot_ns;
+    uint64_t prediction_ns;
+    uint64_t distance_constraint_ns;
+    uint64_t cable_constraint_ns;
+    uint64_t membrane_constraint_ns;
+    uint64_t bending_constraint_ns;
+    uint64_t suspension_constraint_ns;
+    uint64_t contact_constraint_ns;
+    uint64_t membrane_diagnostics_ns;
+    uint64_t finalization_ns;
+    uint64_t structural_substeps;
+    uint64_t constraint_iterations;
+    uint64_t distance_constraint_visits;
+    uint64_t cable_constraint_visits;
+    uint64_t membrane_constraint_visits;
+    uint64_t bending_constraint_visits;
+    uint64_t suspension_constraint_visits;
+    uint64_t contact_constraint_visits;
+} sw_step_profile;
+/* Turns profiling on (the totals start from zero) or off (they are kept). */
+sw_status sw_structure_set_profiling(sw_structure* s, int enabled);
+sw_status sw_structure_profile(sw_structure* s, sw_step_profile* out);
+sw_status sw_structure_reset_profile(sw_structure* s);
+
 /* ---- Scene input and assembly (scene_structure.h:162-165) ------------- */
 
 typedef struct sw_scene sw_scene;

@@ -661,6 +661,20 @@ class Structure:
     def set_kernel_timing(self, enabled: bool) -> None:
         check(native.library().sw_structure_set_kernel_timing(self._handle, int(enabled)))
 
+    def set_profiling(self, enabled: bool) -> None:
+        """Step profile on (totals from zero) or off (sw_structure_set_profiling)."""
+        check(native.library().sw_structure_set_profiling(self._handle, int(enabled)))
+
+    def profile(self) -> dict:
+        """StepPerformanceProfile's soft-body fields over the profiled steps
+        (sw_structure_profile): nanoseconds and visit counts."""
+        out = native.StepProfile()
+        check(native.library().sw_structure_profile(self._handle, C.byref(out)))
+        return {name: int(getattr(out, name)) for name, _ in native.StepProfile._fields_}
+
+    def reset_profile(self) -> None:
+        check(native.library().sw_structure_reset_profile(self._handle))
+
     def kernel_time(self) -> tuple[float, int]:
         """Summed substep-kernel time (ms, HIP events on the structure's
         stream) and launch count since timing was enabled."""

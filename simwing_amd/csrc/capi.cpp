@@ -12,6 +12,7 @@
 #include "scene_io.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -103,7 +104,16 @@ struct sw_checkpoint {
     SuspensionRecord record;
 };
 
+// Host side of the step profile: visit counts and wall-clock totals.
+struct ProfileCounts {
+    bool on = false;
+    double totalNs = 0.0;
+    std::uint64_t substeps = 0, iterations = 0, distance = 0, cable = 0, membrane = 0, bending = 0, suspension = 0,
+                  contact = 0;
+};
+
 struct sw_structure {
+    ProfileCounts profile;
     swx::Definition definition;
     std::uint64_t fingerprint = 0;
     std::unique_ptr<swx::DeviceWing> wing;
@@ -283,6 +293,30 @@ void fillDiagnostics(sw_structure& s, sw_diagnostics* out) {
                       : 0;
 }
 
+// Visits of `count` steps, by StepPerformanceProfile's rules
+// (soft_body.cpp:1078, 1222-1245, 1389-1404), and their wall-clock time.
+void countProfile(sw_structure& s, const sw_step_settings& settings, int count,
+                  std::chrono::steady_clock::time_point started) {
+    const swx::Definition& d = s.definition;
+    std::uint64_t loadPath = 0, stitches = 0;
+    for (std::uint8_t kind : d.constraintKind) {
+        if (kind == SW_CONSTRAINT_CABLE || kind == SW_CONSTRAINT_SUSPENSION_TIE) ++loadPath;
+        if (kind == SW_CONSTRAINT_SEAM_STITCH) ++stitches;
+    }
+    const std::uint64_t substeps = static_cast<std::uint64_t>(count) * static_cast<std::uint64_t>(settings.substeps);
+    const std::uint64_t it = static_cast<std::uint64_t>(settings.constraint_iterations);
+    ProfileCounts& p = s.profile;
+    p.substeps += substeps;
+    p.iterations += substeps * it;
+    p.distance += substeps * (it * d.constraintCount() + (it > 0 ? stitches : 0));
+    p.cable += substeps * 2ULL * static_cast<std::uint64_t>(settings.cable_sweep_pairs) * loadPath;
+    p.membrane += substeps * it * d.membraneCount();
+    p.bending += substeps * it * d.dihedralCount();
+    if (d.suspension) p.suspension += substeps * it * d.suspension->segmentRest.size();
+    if (!d.sphereNodes.empty()) p.contact += substeps * it;
+    p.totalNs += std::chrono::duration<double, std::nano>(std::chrono::steady_clock::now() - started).count();
+}
+
 void runSteps(sw_structure& s, const sw_step_settings& settings, int count, sw_diagnostics* out) {
     if (count < 1) throw std::invalid_argument("step count must be positive");
     const swx::StepArgs args = prepare(s, settings, true);
@@ -291,12 +325,14 @@ void runSteps(sw_structure& s, const sw_step_settings& settings, int count, sw_d
         s.pendingDirty = false;
     }
     s.wing->clearError();
+    const auto started = std::chrono::steady_clock::now();
     for (int step = 0; step < count; ++step) {
         s.wing->snapshot();
         for (int sub = 0; sub < settings.substeps; ++sub) s.wing->launchSubstep(args);
     }
     double residual = 0.0;
     const swx::DeviceError e = s.wing->readError(&residual);
+    if (s.profile.on) countProfile(s, settings, count, started);
     // Steps accepted: all of them, or (after a failure) those before the
     // failing one, whose pre-step snapshot is the last one taken.
     int accepted = count;
@@ -878,6 +914,64 @@ sw_status sw_structure_kernel_time(sw_structure* s, double* milliseconds, long l
     return guarded([&] {
         requirePointer(s, "structure");
         s->wing->kernelTime(milliseconds, launches);
+    });
+}
+
+sw_status sw_structure_set_profiling(sw_structure* s, int enabled) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        if (enabled && !s->profile.on) {
+            s->profile = ProfileCounts{};
+            s->wing->setProfiling(true);
+            s->wing->clearProfile();
+        } else if (!enabled) {
+            s->wing->setProfiling(false);
+        }
+        s->profile.on = enabled != 0;
+    });
+}
+
+sw_status sw_structure_reset_profile(sw_structure* s) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        const bool on = s->profile.on;
+        s->profile = ProfileCounts{};
+        s->profile.on = on;
+        s->wing->clearProfile();
+    });
+}
+
+sw_status sw_structure_profile(sw_structure* s, sw_step_profile* out) {
+    return guarded([&] {
+        requirePointer(s, "structure");
+        requirePointer(out, "profile");
+        double parts[9];
+        long long launches = 0;
+        s->wing->profileTotals(parts, &launches);
+        const ProfileCounts& p = s->profile;
+        const auto ns = [](double v) { return static_cast<std::uint64_t>(v < 0.0 ? 0.0 : v + 0.5); };
+        *out = sw_step_profile{};
+        out->soft_body_total_ns = ns(p.totalNs);
+        out->prediction_ns = ns(parts[0]);
+        out->distance_constraint_ns = ns(parts[1]);
+        out->cable_constraint_ns = ns(parts[2]);
+        out->membrane_constraint_ns = ns(parts[3]);
+        out->bending_constraint_ns = ns(parts[4]);
+        out->suspension_constraint_ns = ns(parts[5]);
+        out->contact_constraint_ns = ns(parts[6]);
+        out->membrane_diagnostics_ns = ns(parts[7]);
+        out->finalization_ns = ns(parts[8]);
+        double kernel = 0.0;
+        for (double v : parts) kernel += v;
+        out->transaction_snapshot_ns = ns(p.totalNs - kernel);
+        out->structural_substeps = p.substeps;
+        out->constraint_iterations = p.iterations;
+        out->distance_constraint_visits = p.distance;
+        out->cable_constraint_visits = p.cable;
+        out->membrane_constraint_visits = p.membrane;
+        out->bending_constraint_visits = p.bending;
+        out->suspension_constraint_visits = p.suspension;
+        out->contact_constraint_visits = p.contact;
     });
 }
 

@@ -169,6 +169,18 @@ public:
     void setTiming(bool enabled);
     void kernelTime(double* milliseconds, long long* launches);
 
+    // Step profile: while enabled, every substep launch adds, per part of
+    // the substep (prediction, distance, cable, membrane, bending,
+    // suspension, contact, membrane diagnostics, finalisation), the
+    // wall-clock ticks its workgroups spent there to device totals.
+    // profileTotals() waits for the stream and returns the totals (10 MHz
+    // ticks converted to nanoseconds, each divided by the workgroups per
+    // launch, so each is the mean workgroup's time) and the launches since
+    // the totals were last cleared.
+    void setProfiling(bool enabled);
+    void profileTotals(double nanoseconds[9], long long* launches);
+    void clearProfile();
+
     void setLaunch(int workgroups, int threads);
     // Incremented by every setLaunch (batches re-plan when a wing changed).
     unsigned long long launchGeneration() const { return launchGeneration_; }
@@ -277,6 +289,8 @@ private:
     unsigned long long launchGeneration_ = 0;
     void* depAlloc_[3] = {nullptr, nullptr, nullptr};
     bool timing_ = false;
+    unsigned long long* profile_ = nullptr;  // device totals (10 slots), allocated on first use
+    long long profiledLaunches_ = 0;
     std::vector<hipEvent_t> events_;  // start/stop pairs, grown on demand
     std::size_t eventsUsed_ = 0;
     double timedMs_ = 0.0;

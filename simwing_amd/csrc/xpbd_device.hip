@@ -106,6 +106,10 @@ struct WingView {
     // writes the values its projection reads (its input vector), laid out as
     // traceLayout() describes; any launch mode records the same values.
     double* trace;
+    // Step profile totals (null unless profiling): per part of the substep,
+    // the wall-clock ticks (100 MHz) its workgroups spent there, then the
+    // number of workgroup launches (kProfileParts entries).
+    unsigned long long* profile;
     double* state;      // contiguous mutable state block
     double* snapshot;   // copy of `state`
     long stateDoubles;
@@ -608,6 +612,47 @@ __device__ __forceinline__ HingeItem hingeItem(const WingView& v, const NodeRefs
 // miss in the caches each hinge phase, after the acquire that precedes it has
 // invalidated them.
 __shared__ double atanLds[65][2];
+
+// Step profile (the device side of StepPerformanceProfile, soft_body.h:136-170):
+// thread 0 of each workgroup adds the wall-clock ticks (100 MHz) it spends in
+// each part of the substep to an LDS tally, which it adds to the wing's
+// totals at the end of the launch. Parts: kProfilePrediction ..
+// kProfileFinalization; the last slot counts workgroup launches.
+constexpr int kProfilePrediction = 0, kProfileDistance = 1, kProfileCable = 2, kProfileMembrane = 3,
+              kProfileBending = 4, kProfileSuspension = 5, kProfileContact = 6, kProfileDiagnostics = 7,
+              kProfileFinalization = 8, kProfileParts = 10;
+__shared__ unsigned long long profileTicks[kProfileParts];
+__device__ __forceinline__ void profileStart(const WingView& v, unsigned long long& since) {
+    if (v.profile && threadIdx.x == 0) {
+        for (int k = 0; k < kProfileParts; ++k) profileTicks[k] = 0;
+        since = wall_clock64();
+    }
+}
+__device__ __forceinline__ void profileAdd(const WingView& v, int part, unsigned long long& since) {
+    if (v.profile && threadIdx.x == 0) {
+        const unsigned long long now = wall_clock64();
+        profileTicks[part] += now - since;
+        since = now;
+    }
+}
+__device__ __forceinline__ int profilePart(int type) {
+    switch (type) {
+    case kPhaseCableList: return kProfileCable;
+    case kPhaseMembrane:
+    case kPhaseMembraneList: return kProfileMembrane;
+    case kPhaseHinge: return kProfileBending;
+    case kPhaseSegments:
+    case kPhaseSegmentRun: return kProfileSuspension;
+    case kPhaseSpheres: return kProfileContact;
+    default: return kProfileDistance;
+    }
+}
+__device__ __forceinline__ void profileFlush(const WingView& v) {
+    if (v.profile && threadIdx.x == 0) {
+        profileTicks[kProfileParts - 1] = 1;
+        for (int k = 0; k < kProfileParts; ++k) atomicAdd(v.profile + k, profileTicks[k]);
+    }
+}
 
 // Dihedral hinge (soft_body.cpp:1648-1711 and 1754-1792).
 // remainder(x, 2π) (the double nearest 2π): for |x| < 3π the quotient is
@@ -1743,6 +1788,8 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
     if (blocks == 1 && __hip_atomic_load(&v.error->code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int n = v.nodeCount;
     const bool shared = kMode == 1 || (kMode == 2 && v.sharedNodes != 0);
+    unsigned long long since = 0;
+    profileStart(v, since);
     for (int i = threadIdx.x; i < 65; i += blockDim.x) {
         atanLds[i][0] = pm::kAtanTable[i][0];
         atanLds[i][1] = pm::kAtanTable[i][1];
@@ -1869,6 +1916,8 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         gridSync<kLocal>(v, blocks);
     }
 
+    profileAdd(v, kProfilePrediction, since);
+
     // Constraint iterations.
     if (v.neighbourSync && !shared && blocks > 1) {
         // Point-to-point synchronisation: workgroup p owns a contiguous share
@@ -1916,6 +1965,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     drainStores<kLocal, true>(v);
                     __syncthreads();
                     publishCounter<kLocal, true>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
+                    profileAdd(v, kProfileSuspension, since);
                 }
             }
         }
@@ -1952,6 +2002,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                             });
                     }
                     ph = end - 1;
+                    profileAdd(v, profilePart(P.type), since);
                     continue;
                 }
                 const int b = S.share[ph].x, e = S.share[ph].y;
@@ -2015,13 +2066,18 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                 drainStores<kLocal, true>(v);
                 __syncthreads();
                 publishCounter<kLocal, true>(v, v.flags + 32 * p, static_cast<unsigned>(step + 1));
+                profileAdd(v, profilePart(P.type), since);
             }
         }
         gridSync<kLocal>(v, blocks);
         // Every participant has left the loop: nobody polls the progress
         // words any more, so each resets its own for the next launch.
         if (threadIdx.x == 0) __hip_atomic_store(v.flags + 32 * p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (chainGroup) return;
+        if (chainGroup) {
+            profileAdd(v, kProfileSuspension, since);
+            profileFlush(v);
+            return;
+        }
     } else {
         for (int it = 0; it < a.iterations; ++it) {
             double* rec = trace ? trace + layout.iterBase + it * layout.iterStride : nullptr;
@@ -2033,6 +2089,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                     runPhase<kGeneral>(v, N, P, P.begin + gtid, P.end, gstride, payload, a, rec, layout);
                 }
                 gridSync<kLocal>(v, workers);
+                profileAdd(v, profilePart(P.type), since);
             }
         }
     }
@@ -2049,6 +2106,9 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
                             trace ? trace + layout.closeBase + kRecDist * static_cast<long>(k) : nullptr);
         gridSync<kLocal>(v, workers);
     }
+    // The wait for the last workgroups of the iterations and the closing
+    // stitches count as distance-constraint time (soft_body.cpp:1389-1408).
+    profileAdd(v, kProfileDistance, since);
 
     // Membrane solver diagnostics (soft_body.cpp:2366-2404), velocity
     // finalisation (soft_body.cpp:1568-1585) and line residuals
@@ -2079,6 +2139,7 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
         v.mry[k] = -lambda.y * f;
         v.mrz[k] = -lambda.z * f;
     }
+    profileAdd(v, kProfileDiagnostics, since);
     for (int i = gtid; i < n; i += gstride) {
         const double wi = N.w[i];
         if (wi == 0.0) {
@@ -2138,6 +2199,8 @@ __device__ __forceinline__ void substepBody(const WingView& v, const StepArgs& a
             v.pz[i] = N.z[i];
         }
     }
+    profileAdd(v, kProfileFinalization, since);
+    profileFlush(v);
 }
 
 template <bool kShared, bool kLocal, bool kGeneral>
@@ -2578,6 +2641,43 @@ void DeviceWing::setTiming(bool enabled) {
         timedLaunches_ = 0;
     }
     timing_ = enabled;
+}
+
+void DeviceWing::setProfiling(bool enabled) {
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipStreamSynchronize(stream_), "profiling");
+    if (enabled && !profile_) {
+        void* p = nullptr;
+        check(hipMalloc(&p, kProfileParts * sizeof(unsigned long long)), "hipMalloc profile");
+        allocations_.push_back(p);
+        profile_ = static_cast<unsigned long long*>(p);
+        clearProfile();
+    }
+    view_->profile = enabled ? profile_ : nullptr;
+    ++launchGeneration_;  // batched views copy the wing's view
+}
+
+void DeviceWing::clearProfile() {
+    if (!profile_) return;
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipMemsetAsync(profile_, 0, kProfileParts * sizeof(unsigned long long), stream_), "clear profile");
+    check(hipStreamSynchronize(stream_), "clear profile");
+    profiledLaunches_ = 0;
+}
+
+void DeviceWing::profileTotals(double nanoseconds[9], long long* launches) {
+    unsigned long long ticks[kProfileParts] = {};
+    if (profile_) {
+        check(hipSetDevice(device_), "hipSetDevice");
+        check(hipStreamSynchronize(stream_), "profile");
+        check(hipMemcpy(ticks, profile_, sizeof(ticks), hipMemcpyDeviceToHost), "profile");
+    }
+    // ticks[9]: workgroup launches; profiledLaunches_: substep launches.
+    const double perLaunch = profiledLaunches_ > 0 && ticks[kProfileParts - 1] > 0
+                                 ? static_cast<double>(ticks[kProfileParts - 1]) / static_cast<double>(profiledLaunches_)
+                                 : 1.0;
+    for (int k = 0; k < 9; ++k) nanoseconds[k] = 10.0 * static_cast<double>(ticks[k]) / perLaunch;  // 100 MHz clock
+    if (launches) *launches = profiledLaunches_;
 }
 
 void DeviceWing::kernelTime(double* milliseconds, long long* launches) {
@@ -3361,6 +3461,7 @@ void DeviceWing::setFacePressure(const std::vector<double>& pressure) {
 void DeviceWing::launchSubstep(const StepArgs& stepArgs, double* trace) {
     WingView v = *view_;
     v.trace = trace;
+    if (v.profile) ++profiledLaunches_;
     v.neighbourSync = neighbourSync_ ? 1 : 0;
     v.dedicatedChain = dedicatedChain_ ? 1 : 0;
     v.sharedNodes = sharedNodes_ ? 1 : 0;
@@ -3913,6 +4014,7 @@ void DeviceBatch::plan(int cap, int threads) {
     for (int w = 0; w < count; ++w) {
         wings_[w]->setLaunch(blocks_[w], threads);
         views[w] = wings_[w]->launchView();
+        views[w].profile = nullptr;  // the step profile covers a wing's own launches
         lds_ = std::max(lds_, wings_[w]->dynamicLdsBytes());
         for (int b = 0; b < views[w].launchBlocks; ++b) {
             blockWing.push_back(w);

@@ -115,6 +115,17 @@ class StepSettings(C.Structure):
     ]
 
 
+class StepProfile(C.Structure):
+    """sw_step_profile: StepPerformanceProfile's soft-body fields."""
+    _fields_ = [(name, C.c_uint64) for name in (
+        "soft_body_total_ns", "transaction_snapshot_ns", "prediction_ns", "distance_constraint_ns",
+        "cable_constraint_ns", "membrane_constraint_ns", "bending_constraint_ns", "suspension_constraint_ns",
+        "contact_constraint_ns", "membrane_diagnostics_ns", "finalization_ns", "structural_substeps",
+        "constraint_iterations", "distance_constraint_visits", "cable_constraint_visits",
+        "membrane_constraint_visits", "bending_constraint_visits", "suspension_constraint_visits",
+        "contact_constraint_visits")]
+
+
 class Diagnostics(C.Structure):
     _fields_ = [
         ("node_count", C.c_size_t),
@@ -242,6 +253,9 @@ _SIGNATURES = {
     "sw_structure_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sw_structure_set_cooperative": (C.c_int, [C.c_void_p, C.c_int]),
     "sw_structure_kernel_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "sw_structure_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "sw_structure_profile": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sw_structure_reset_profile": (C.c_int, [C.c_void_p]),
     "sw_scene_generate": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "sw_scene_destroy": (None, [C.c_void_p]),
     "sw_scene_face_pressure": (C.c_int, [C.c_void_p, C.POINTER(_F64P), C.POINTER(C.c_size_t)]),
