@@ -205,6 +205,18 @@ def test_device_closed_forms():
         x, _, _ = s.node_states()
         assert math.isclose(x[1, 0], expected, rel_tol=1e-14) and x[1, 1] == 0.0, (iterations, x[1])
         s.close()
+    # Release branch: a rigid sphere that already pushed the node out pulls it
+    # back once a second, compliant sphere has moved it past the radius, so
+    # after n iterations the node sits at 0.05 + 0.01 * 2^(1-n).
+    for iterations in (1, 2, 3, 6):
+        d = definition([[0, 0, 0], [0.02, 0, 0], [0, 0.5, 0], [-0.05, 0, 0]], [1.0, 1.0, 1.0, 1.0],
+                       fixed=[1, 0, 1, 1], spheres=[(0, 0.05, 0.0, [1]), (3, 0.12, 1.0e-4, [1])])
+        s = sw.Structure(d)
+        s.step(settings(dt=0.01, iterations=iterations))
+        x, _, _ = s.node_states()
+        expected = 0.05 + 0.01 * 2.0 ** (1 - iterations)
+        assert math.isclose(x[1, 0], expected, rel_tol=1e-14) and x[1, 1] == 0.0, (iterations, x[1])
+        s.close()
 
 
 def test_checkpoint_restore_replays_bit_identically():

@@ -145,3 +145,25 @@ def test_compliant_contact_sphere_closed_form():
         assert math.isclose(x[1, 0], expected, rel_tol=1e-14), (iterations, x[1, 0], expected)
         assert x[1, 1] == 0.0 and x[1, 2] == 0.0
         assert np.array_equal(x[0], [0, 0, 0]) and np.array_equal(x[2], [0, 0.3, 0])
+
+
+def test_contact_sphere_release_branch_closed_form():
+    # Two spheres on one node (w = 1, starting 0.02 along x): a rigid sphere
+    # A (radius 0.05 about node 0 at the origin) and a compliant sphere B
+    # (radius 0.12 about node 3 at x = -0.05, alpha = c / dt^2 = 1).
+    # Iteration 1: A pushes the node out to 0.05 (lambda_A = 0.03), then B
+    # moves it by (0.12 - 0.10) / (1 + 1) = 0.01 to 0.06 (lambda_B = 0.01).
+    # From then on, A finds the node outside with lambda_A > 0 and releases
+    # the excess, pulling it back to 0.05, and B's update (0.02 - lambda_B) / 2
+    # halves each iteration, so after n iterations x = 0.05 + 0.01 * 2^(1-n).
+    # Without the release (a sphere that ignores an outside node) the node
+    # would stay at 0.06 from the second iteration on.
+    for iterations in (1, 2, 3, 4, 6):
+        d = definition([[0, 0, 0], [0.02, 0, 0], [0, 0.5, 0], [-0.05, 0, 0]], [1.0, 1.0, 1.0, 1.0],
+                       fixed=[1, 0, 1, 1], spheres=[(0, 0.05, 0.0, [1]), (3, 0.12, 1.0e-4, [1])])
+        body = orc.OracleBody(d)
+        body.step(settings(dt=0.01, iterations=iterations))
+        x = body.state()["position"].reshape(-1, 3)
+        expected = 0.05 + 0.01 * 2.0 ** (1 - iterations)
+        assert math.isclose(x[1, 0], expected, rel_tol=1e-14), (iterations, x[1, 0], expected)
+        assert x[1, 1] == 0.0 and x[1, 2] == 0.0
